@@ -1,0 +1,212 @@
+// C ABI of aphpois (include/aphpois.h) over the Engine.
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "engine.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int set_err(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+}  // namespace
+
+struct aphpois {
+  aph::Engine eng;
+  aphpois_desc desc{};
+  int slot = -1;
+};
+
+struct aphpois_batch {
+  aph::Engine eng;
+  aphpois_batch_desc desc{};
+};
+
+#define APH_CHECK_H(h) \
+  if (!(h)) return set_err(APHPOIS_EINVAL, "NULL handle")
+#define APH_RET(obj, expr)                                  \
+  do {                                                      \
+    const int rc__ = (expr);                                \
+    if (rc__ != APHPOIS_OK) return set_err(rc__, (obj).eng.err()); \
+    return APHPOIS_OK;                                      \
+  } while (0)
+
+extern "C" {
+
+const char* aphpois_last_error(void) { return g_err.c_str(); }
+
+void aphpois_desc_default(aphpois_desc* d) {
+  if (!d) return;
+  std::memset(d, 0, sizeof(*d));
+  d->nx = d->ny = d->nz = 0;
+  d->h3 = 1.0;
+  d->method = APHPOIS_CONJUGATE;
+  d->precision = APHPOIS_FP64;
+  d->resmode = APHPOIS_RES_L2;
+  d->tol = 0;
+  d->miniter = 1;
+  d->maxiter = 100;
+  d->nullspace = APHPOIS_NULL_AUTO;
+  d->delta = 0;
+  d->check_every = 0;
+  d->device = 0;
+  d->mem_budget = 0;
+}
+
+int aphpois_create(const aphpois_desc* desc, aphpois** out) {
+  if (!desc || !out) return set_err(APHPOIS_EINVAL, "NULL argument");
+  *out = nullptr;
+  aphpois* h = new (std::nothrow) aphpois();
+  if (!h) return set_err(APHPOIS_ENOMEM, "host allocation failed");
+  h->desc = *desc;
+  int rc = h->eng.init(desc->device, desc->method, desc->precision, desc->resmode, desc->tol,
+                       desc->miniter, desc->maxiter, desc->nullspace, desc->delta,
+                       desc->check_every, desc->mem_budget, 1);
+  if (rc == APHPOIS_OK) {
+    rc = h->eng.add_system(desc->nx, desc->ny, desc->nz, desc->h3, &h->slot);
+  }
+  if (rc != APHPOIS_OK) {
+    set_err(rc, h->eng.err());
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return APHPOIS_OK;
+}
+
+int aphpois_set_system(aphpois* h, const double* expr, long long start, long long lead_y,
+                       long long lead_z) {
+  APH_CHECK_H(h);
+  APH_RET(*h, h->eng.set_system(h->slot, expr, false, start, lead_y, lead_z));
+}
+
+int aphpois_set_system_device(aphpois* h, const double* expr_dev) {
+  APH_CHECK_H(h);
+  const auto& d = h->desc;
+  APH_RET(*h, h->eng.set_system(h->slot, expr_dev, true, 0, d.nx, (long long)d.nx * d.ny));
+}
+
+int aphpois_set_rhs(aphpois* h, const double* e7, long long start, long long lead_y,
+                    long long lead_z) {
+  APH_CHECK_H(h);
+  APH_RET(*h, h->eng.set_rhs(h->slot, e7, false, start, lead_y, lead_z));
+}
+
+static int solve_impl(aphpois* h, const double* x0, double* x, bool dev, long long start,
+                      long long ly, long long lz, aphpois_info* info) {
+  if (!x) return set_err(APHPOIS_EINVAL, "x is NULL");
+  int rc = h->eng.prepare(h->slot, x0, dev, start, ly, lz, false);
+  if (rc == APHPOIS_OK) rc = h->eng.run();
+  if (rc == APHPOIS_OK) rc = h->eng.fetch(h->slot, x, dev, start, ly, lz, info);
+  if (rc != APHPOIS_OK) return set_err(rc, h->eng.err());
+  return APHPOIS_OK;
+}
+
+int aphpois_solve(aphpois* h, const double* x0, double* x, long long start, long long lead_y,
+                  long long lead_z, aphpois_info* info) {
+  APH_CHECK_H(h);
+  return solve_impl(h, x0, x, false, start, lead_y, lead_z, info);
+}
+
+int aphpois_solve_device(aphpois* h, const double* x0_dev, double* x_dev, aphpois_info* info) {
+  APH_CHECK_H(h);
+  const auto& d = h->desc;
+  return solve_impl(h, x0_dev, x_dev, true, 0, d.nx, (long long)d.nx * d.ny, info);
+}
+
+int aphpois_set_conf(aphpois* h, double tol, int miniter, int maxiter) {
+  APH_CHECK_H(h);
+  if (maxiter < 0) return set_err(APHPOIS_EINVAL, "maxiter < 0");
+  h->desc.tol = tol;
+  h->desc.miniter = miniter;
+  h->desc.maxiter = maxiter;
+  h->eng.set_conf(tol, miniter, maxiter);
+  return APHPOIS_OK;
+}
+
+long long aphpois_device_bytes(const aphpois* h) { return h ? h->eng.bytes() : 0; }
+
+int aphpois_destroy(aphpois* h) {
+  delete h;
+  return APHPOIS_OK;
+}
+
+// ---------------------------------------------------------------- batch
+
+int aphpois_batch_create(const aphpois_batch_desc* d, aphpois_batch** out) {
+  if (!d || !out) return set_err(APHPOIS_EINVAL, "NULL argument");
+  *out = nullptr;
+  aphpois_batch* b = new (std::nothrow) aphpois_batch();
+  if (!b) return set_err(APHPOIS_ENOMEM, "host allocation failed");
+  b->desc = *d;
+  const int rc = b->eng.init(d->device, APHPOIS_PCG, d->precision, d->resmode, d->tol, 0,
+                             d->maxiter, APHPOIS_NULL_AUTO, d->delta, d->check_every,
+                             d->mem_budget, d->max_systems);
+  if (rc != APHPOIS_OK) {
+    set_err(rc, b->eng.err());
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return APHPOIS_OK;
+}
+
+int aphpois_batch_add(aphpois_batch* b, int nx, int ny, int nz, double h3, const double* expr,
+                      const double* x0, int* slot) {
+  APH_CHECK_H(b);
+  if (!slot) return set_err(APHPOIS_EINVAL, "slot is NULL");
+  int s = -1;
+  int rc = b->eng.add_system(nx, ny, nz, h3, &s);
+  if (rc == APHPOIS_OK) rc = b->eng.set_system(s, expr, false, 0, nx, (long long)nx * ny);
+  if (rc == APHPOIS_OK) rc = b->eng.prepare(s, x0, false, 0, nx, (long long)nx * ny, false);
+  if (rc != APHPOIS_OK) {
+    set_err(rc, b->eng.err());
+    if (s >= 0) b->eng.remove(s);
+    return rc;
+  }
+  *slot = s;
+  return APHPOIS_OK;
+}
+
+int aphpois_batch_set_rhs(aphpois_batch* b, int slot, const double* e7, const double* x0) {
+  APH_CHECK_H(b);
+  aph::System* s = b->eng.sys(slot);
+  if (!s) return set_err(APHPOIS_EINVAL, "no such slot");
+  int rc = APHPOIS_OK;
+  if (e7) rc = b->eng.set_rhs(slot, e7, false, 0, s->nx, (long long)s->nx * s->ny);
+  if (rc == APHPOIS_OK)
+    rc = b->eng.prepare(slot, x0, false, 0, s->nx, (long long)s->nx * s->ny, x0 == nullptr);
+  if (rc != APHPOIS_OK) return set_err(rc, b->eng.err());
+  return APHPOIS_OK;
+}
+
+int aphpois_batch_solve(aphpois_batch* b) {
+  APH_CHECK_H(b);
+  APH_RET(*b, b->eng.run());
+}
+
+int aphpois_batch_get(aphpois_batch* b, int slot, double* x, aphpois_info* info) {
+  APH_CHECK_H(b);
+  aph::System* s = b->eng.sys(slot);
+  if (!s) return set_err(APHPOIS_EINVAL, "no such slot");
+  APH_RET(*b, b->eng.fetch(slot, x, false, 0, s->nx, (long long)s->nx * s->ny, info));
+}
+
+int aphpois_batch_remove(aphpois_batch* b, int slot) {
+  APH_CHECK_H(b);
+  APH_RET(*b, b->eng.remove(slot));
+}
+
+long long aphpois_batch_device_bytes(const aphpois_batch* b) { return b ? b->eng.bytes() : 0; }
+
+int aphpois_batch_destroy(aphpois_batch* b) {
+  delete b;
+  return APHPOIS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,93 @@
+// Shared device/host structures of the aphpois engine.
+//
+// Layout in HBM (per system, all SoA, x fastest, no halo slots; periodic
+// neighbours are found by index wrap):
+//   PCG   : kx, ky, kz  face conductances of the plus faces (type T)
+//           S           1/sqrt(diag) scale (type T), m optional mass term (T)
+//           r, p[2], q, xh  Krylov vectors of the scaled system (type T)
+//           x64, b64    FP64 solution and right-hand side b = -e7
+//           s4[4]       FP64 e0, e_xp, e_yp, e_zp (true residual)
+//   CONJUGATE / JACOBI : f7[8] = e0..e7 (FP64), u, un/p[2], r, q (FP64)
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace aph {
+
+constexpr int kBlock = 256;
+constexpr int kXcd = 8;  // MI355X: 8 XCDs, workgroups dispatched round-robin
+
+enum Method { kConjugate = 0, kJacobi = 1, kPcg = 2 };
+
+// Per-system scalars, resident in device memory and owned by the kernels.
+struct Scalars {
+  double rho;         // (r,r) of the current iterate (scaled space for PCG)
+  double rho_pend;    // (r,r) computed by the update kernel, pending a decision
+  double alpha, beta;
+  double rmax2;       // (r,r) at the last reliable update
+  double gamma;       // stop metric / sqrt(rho) measured at the last update
+  double tol, delta, h3;
+  double bnorm2;      // (b,b) in FP64
+  double relres, res_l2, res_max, residual;
+  double mean_pending;  // mean to subtract from x64 (nullspace removal)
+  double pad0;
+  int iter, miniter, maxiter, done;
+  int ru, converged, nru, status;
+  int resmode, singular, first, cur;  // cur: index of the current p/u buffer
+  int method, pad1;
+};
+
+struct SysDev {
+  int nx, ny, nz, has_m;
+  long long n;
+  // PCG operator and vectors (element type float or double, see precision)
+  const void* kx;
+  const void* ky;
+  const void* kz;
+  const void* S;
+  const void* m;
+  void* r;
+  void* p[2];
+  void* q;
+  void* xh;
+  double* x64;
+  const double* b64;
+  const double* s4[4];
+  // reference-mode storage
+  const double* f7[8];
+  double* u[2];
+  double* rr;
+  double* pp[2];
+  double* qq;
+  // reductions
+  double* partials;  // 4 * nblk_max doubles
+  unsigned* counter;
+  Scalars* sc;
+  int neg;  // 1 if the diagonal is negative (A = -L): r^ = -S r64
+  int pad;
+};
+
+// One record per system in a launch: which blocks belong to it and how they
+// tile the grid.  Streaming launches use tiles_x = nblk, tiles_y = tiles_z = 1.
+struct Rec {
+  int sys;
+  int blk0;
+  int tiles_x, tiles_y, tiles_z;
+  int lx;  // threads per row (stencil), rows per block = kBlock / lx
+  int bz;  // planes per block (stencil)
+  int nblk;
+};
+
+// Physical workgroup id -> logical id such that the workgroups that run on
+// one XCD (ids x, x+8, x+16, ...) get a contiguous range of logical tiles and
+// share halo rows in that XCD's L2.
+__device__ __forceinline__ int xcd_remap(int b, int g) {
+  const int per = g / kXcd, rem = g % kXcd;
+  const int x = b % kXcd, local = b / kXcd;
+  return x < rem ? x * (per + 1) + local : rem * (per + 1) + (x - rem) * per + local;
+}
+
+}  // namespace aph

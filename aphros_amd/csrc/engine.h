@@ -1,0 +1,143 @@
+// Host engine: resident systems, launch tables, captured iteration graphs.
+//
+// One Engine owns a HIP stream and any number of resident systems that share
+// one method and precision.  A solve launches the same kernels for every
+// active system at once (a single system is a batch of one).  Scalars stay on
+// the device; the host only polls the per-system done flags between replays
+// of a captured graph of `check_every` iterations.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/aphpois.h"
+#include "common.h"
+#include "kernels.h"
+
+namespace aph {
+
+struct Tuning {
+  int lx = 0;  // 0 = automatic
+  int bz = 0;
+};
+
+struct System {
+  int nx = 0, ny = 0, nz = 0;
+  long long n = 0;
+  double h3 = 1;
+  bool has_system = false;
+  bool singular = false;
+  bool neg = false;
+  bool has_m = false;
+  bool active = false;  // takes part in the next solve
+  bool have_x = false;  // x64/u holds a previous solution (warm start)
+  std::vector<void*> allocs;
+  long long bytes = 0;
+  void* partials = nullptr;  // reduction scratch, outside the budget
+  long long partials_cap = 0;
+  SysDev dev{};
+  aphpois_info info{};
+  double seconds = 0;
+};
+
+class Engine {
+ public:
+  Engine() = default;
+  ~Engine();
+  // returns APHPOIS_* code; message in err()
+  int init(int device, int method, int precision, int resmode, double tol, int miniter,
+           int maxiter, int nullspace, double delta, int check_every, long long budget,
+           int capacity);
+  int add_system(int nx, int ny, int nz, double h3, int* slot);
+  int set_system(int slot, const double* expr, bool on_device, long long start, long long ly,
+                 long long lz);
+  int set_rhs(int slot, const double* e7, bool on_device, long long start, long long ly,
+              long long lz);
+  // x0: nullptr (zero guess), or host/device strided field; keep_x: warm start
+  // from the resident solution.
+  int prepare(int slot, const double* x0, bool on_device, long long start, long long ly,
+              long long lz, bool keep_x);
+  int run();  // solves every prepared system
+  int fetch(int slot, double* x, bool on_device, long long start, long long ly, long long lz,
+            aphpois_info* info);
+  int remove(int slot);
+  void set_conf(double tol, int miniter, int maxiter) {
+    tol_ = tol;
+    miniter_ = miniter;
+    maxiter_ = maxiter;
+  }
+  void set_tuning(const Tuning& t) {
+    tuning_ = t;
+    dirty_ = true;
+  }
+  long long bytes() const;
+  int nslots() const { return (int)slots_.size(); }
+  System* sys(int slot) {
+    return slot >= 0 && slot < (int)slots_.size() ? slots_[slot].get() : nullptr;
+  }
+  const std::string& err() const { return err_; }
+  hipStream_t stream() const { return st_; }
+  int method() const { return method_; }
+  int vx() const { return vx_; }
+  // launch geometry of the last solve (for measurement)
+  int blocks_stencil() const { return L_.nst; }
+  int blocks_stream() const { return L_.nsm; }
+  // timing of the iteration kernels of the last run (device events), ms
+  double last_run_ms() const { return last_run_ms_; }
+  int last_run_iters() const { return last_run_iters_; }
+
+ private:
+  int fail(int code, const std::string& msg);
+  int hip(hipError_t e, const char* what);
+  int alloc(System& s, void** p, long long bytes);
+  int build_tables();
+  int capture();
+  int upload_strided(const double* src, bool on_device, long long start, long long ly,
+                     long long lz, int nx, int ny, int nz, int comps, const double** dptr);
+  void free_system(System& s);
+
+  int device_ = 0;
+  int method_ = kConjugate;
+  int fp32_ = 0;
+  int resmode_ = 0;
+  double tol_ = 0;
+  int miniter_ = 1, maxiter_ = 100;
+  int nullspace_ = 0;
+  double delta_ = 0.1;
+  int check_every_ = 0;
+  long long budget_ = 0;
+  int capacity_ = 0;
+  int vx_ = 1;
+  Tuning tuning_;
+  hipStream_t st_ = nullptr;
+  std::vector<std::unique_ptr<System>> slots_;
+  // device tables
+  Scalars* d_sc_ = nullptr;
+  Scalars* h_sc_ = nullptr;  // pinned mirror
+  unsigned* d_counter_ = nullptr;
+  SysDev* d_sds_ = nullptr;
+  Rec* d_rec_ = nullptr;
+  int* d_b2r_ = nullptr;
+  long long rec_cap_ = 0, b2r_cap_ = 0;
+  LaunchSet L_{};
+  std::vector<int> launch_slots_;
+  bool dirty_ = true;
+  hipGraphExec_t gexec_ = nullptr;
+  hipGraph_t graph_ = nullptr;
+  int graph_k_ = 0;
+  // scratch
+  void* staging_ = nullptr;
+  long long staging_bytes_ = 0;
+  unsigned* d_flags_ = nullptr;
+  double* d_red_ = nullptr;  // partials for k_sumsq (1024) + result
+  unsigned* d_red_counter_ = nullptr;
+  hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+  double last_run_ms_ = 0;
+  int last_run_iters_ = 0;
+  std::string err_;
+};
+
+}  // namespace aph

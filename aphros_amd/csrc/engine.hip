@@ -1,0 +1,692 @@
+// Host engine of aphpois (see engine.h).
+#include "engine.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace aph {
+
+namespace {
+
+int pow2floor(int v) {
+  int p = 1;
+  while (p * 2 <= v) p *= 2;
+  return p;
+}
+
+long long cdiv(long long a, long long b) { return (a + b - 1) / b; }
+
+// Stencil tiling of one system: lx lanes of VX cells per row, kBlock/lx rows,
+// bz planes per workgroup.
+Rec make_stencil_rec(int nx, int ny, int nz, int vx, int blocks_target, const Tuning& tu) {
+  Rec r{};
+  const int cols = std::max(1, nx / vx);
+  int lx = tu.lx > 0 ? tu.lx : std::min(64, pow2floor(cols));
+  lx = std::max(1, std::min(lx, kBlock));
+  const int rows = kBlock / lx;
+  r.lx = lx;
+  r.tiles_x = (int)cdiv(cols, lx);
+  r.tiles_y = (int)cdiv(ny, rows);
+  int bz;
+  if (tu.bz > 0) {
+    bz = tu.bz;
+  } else {
+    const long long cols_tiles = (long long)r.tiles_x * r.tiles_y;
+    const long long tz = std::max(1LL, cdiv(blocks_target, cols_tiles));
+    bz = (int)cdiv(nz, tz);
+  }
+  bz = std::max(1, std::min(bz, nz));
+  r.bz = bz;
+  r.tiles_z = (int)cdiv(nz, bz);
+  r.nblk = r.tiles_x * r.tiles_y * r.tiles_z;
+  return r;
+}
+
+}  // namespace
+
+int Engine::fail(int code, const std::string& msg) {
+  err_ = msg;
+  return code;
+}
+
+int Engine::hip(hipError_t e, const char* what) {
+  if (e == hipSuccess) return APHPOIS_OK;
+  err_ = std::string(what) + ": " + hipGetErrorString(e);
+  return e == hipErrorOutOfMemory ? APHPOIS_ENOMEM : APHPOIS_EDEVICE;
+}
+
+#define APH_TRY(expr)                 \
+  do {                                \
+    const int rc__ = (expr);          \
+    if (rc__ != APHPOIS_OK) return rc__; \
+  } while (0)
+#define APH_HIP(expr) APH_TRY(hip((expr), #expr))
+
+Engine::~Engine() {
+  // teardown: errors cannot be reported any more and are ignored
+  if (st_) (void)hipStreamSynchronize(st_);
+  for (auto& s : slots_) {
+    if (s) free_system(*s);
+  }
+  if (gexec_) (void)hipGraphExecDestroy(gexec_);
+  if (graph_) (void)hipGraphDestroy(graph_);
+  if (d_sc_) (void)hipFree(d_sc_);
+  if (h_sc_) (void)hipHostFree(h_sc_);
+  if (d_counter_) (void)hipFree(d_counter_);
+  if (d_sds_) (void)hipFree(d_sds_);
+  if (d_rec_) (void)hipFree(d_rec_);
+  if (d_b2r_) (void)hipFree(d_b2r_);
+  if (staging_) (void)hipFree(staging_);
+  if (d_flags_) (void)hipFree(d_flags_);
+  if (d_red_) (void)hipFree(d_red_);
+  if (d_red_counter_) (void)hipFree(d_red_counter_);
+  if (ev0_) (void)hipEventDestroy(ev0_);
+  if (ev1_) (void)hipEventDestroy(ev1_);
+  if (st_) (void)hipStreamDestroy(st_);
+}
+
+int Engine::init(int device, int method, int precision, int resmode, double tol, int miniter,
+                 int maxiter, int nullspace, double delta, int check_every, long long budget,
+                 int capacity) {
+  if (method < 0 || method > 2) return fail(APHPOIS_EINVAL, "unknown method");
+  if (precision < 0 || precision > 1) return fail(APHPOIS_EINVAL, "unknown precision");
+  if (resmode < 0 || resmode > 2) return fail(APHPOIS_EINVAL, "unknown resmode");
+  if (resmode == 2 && method != kPcg)
+    return fail(APHPOIS_EINVAL, "RES_REL_TRUE needs method PCG");
+  if (precision == 1 && method != kPcg)
+    return fail(APHPOIS_EINVAL, "MIXED precision needs method PCG");
+  if (maxiter < 0) return fail(APHPOIS_EINVAL, "maxiter < 0");
+  if (capacity <= 0) capacity = 1024;
+  int ndev = 0;
+  APH_HIP(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(APHPOIS_EINVAL, "no such device");
+  device_ = device;
+  APH_HIP(hipSetDevice(device_));
+  method_ = method;
+  fp32_ = precision == 1;
+  resmode_ = resmode;
+  tol_ = tol;
+  miniter_ = miniter;
+  maxiter_ = maxiter;
+  nullspace_ = nullspace;
+  delta_ = delta > 0 ? delta : 0.1;
+  check_every_ = check_every;
+  budget_ = budget;
+  capacity_ = capacity;
+  APH_HIP(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+  APH_HIP(hipMalloc(&d_sc_, sizeof(Scalars) * capacity_));
+  APH_HIP(hipHostMalloc(&h_sc_, sizeof(Scalars) * capacity_, hipHostMallocDefault));
+  std::memset(h_sc_, 0, sizeof(Scalars) * capacity_);
+  APH_HIP(hipMalloc(&d_counter_, sizeof(unsigned) * capacity_));
+  APH_HIP(hipMemset(d_counter_, 0, sizeof(unsigned) * capacity_));
+  APH_HIP(hipMemset(d_sc_, 0, sizeof(Scalars) * capacity_));
+  APH_HIP(hipMalloc(&d_flags_, sizeof(unsigned)));
+  APH_HIP(hipMalloc(&d_red_, sizeof(double) * 1032));
+  APH_HIP(hipMalloc(&d_red_counter_, sizeof(unsigned)));
+  APH_HIP(hipMemset(d_red_counter_, 0, sizeof(unsigned)));
+  APH_HIP(hipEventCreate(&ev0_));
+  APH_HIP(hipEventCreate(&ev1_));
+  return APHPOIS_OK;
+}
+
+long long Engine::bytes() const {
+  long long b = staging_bytes_;
+  for (auto& s : slots_) {
+    if (s) b += s->bytes;
+  }
+  return b;
+}
+
+int Engine::alloc(System& s, void** p, long long nbytes) {
+  if (budget_ > 0 && bytes() + nbytes > budget_) {
+    return fail(APHPOIS_EBUDGET, "memory budget exceeded (" + std::to_string(bytes() + nbytes) +
+                                     " > " + std::to_string(budget_) + " bytes)");
+  }
+  hipError_t e = hipMalloc(p, (size_t)std::max(nbytes, 256LL));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return hip(e, "hipMalloc");
+  }
+  s.allocs.push_back(*p);
+  s.bytes += nbytes;
+  return APHPOIS_OK;
+}
+
+void Engine::free_system(System& s) {
+  for (void* p : s.allocs) (void)hipFree(p);
+  s.allocs.clear();
+  s.bytes = 0;
+  if (s.partials) (void)hipFree(s.partials);
+  s.partials = nullptr;
+  s.partials_cap = 0;
+}
+
+int Engine::add_system(int nx, int ny, int nz, double h3, int* slot) {
+  if (nx < 1 || ny < 1 || nz < 1) return fail(APHPOIS_EINVAL, "grid size must be >= 1");
+  if ((long long)nx * ny * nz > (1LL << 40)) return fail(APHPOIS_EINVAL, "grid too large");
+  if (!(h3 > 0)) return fail(APHPOIS_EINVAL, "h3 must be > 0");
+  int id = -1;
+  for (int i = 0; i < (int)slots_.size(); ++i) {
+    if (!slots_[i]) {
+      id = i;
+      break;
+    }
+  }
+  if (id < 0) {
+    if ((int)slots_.size() >= capacity_) return fail(APHPOIS_EBUDGET, "no free system slot");
+    id = (int)slots_.size();
+    slots_.emplace_back();
+  }
+  auto s = std::make_unique<System>();
+  s->nx = nx;
+  s->ny = ny;
+  s->nz = nz;
+  s->n = (long long)nx * ny * nz;
+  s->h3 = h3;
+  const long long n = s->n;
+  // storage (everything except the optional mass term, added at set_system)
+  const int es = fp32_ ? 4 : 8;
+  System& S = *s;
+  S.dev.nx = nx;
+  S.dev.ny = ny;
+  S.dev.nz = nz;
+  S.dev.n = n;
+  S.dev.sc = d_sc_ + id;
+  S.dev.counter = d_counter_ + id;
+  void* p = nullptr;
+  int rc = APHPOIS_OK;
+  auto A = [&](void** dst, long long b) {
+    if (rc == APHPOIS_OK) rc = alloc(S, dst, b);
+  };
+  if (method_ == kPcg) {
+    void *kx, *ky, *kz, *Sc, *r, *p0, *p1, *q, *xh, *x64, *b64, *s4;
+    A(&kx, n * es);
+    A(&ky, n * es);
+    A(&kz, n * es);
+    A(&Sc, n * es);
+    A(&r, n * es);
+    A(&p0, n * es);
+    A(&p1, n * es);
+    A(&q, n * es);
+    A(&xh, n * es);
+    A(&x64, n * 8);
+    A(&b64, n * 8);
+    A(&s4, n * 32);
+    if (rc != APHPOIS_OK) {
+      free_system(S);
+      slots_[id].reset();
+      return rc;
+    }
+    S.dev.kx = kx;
+    S.dev.ky = ky;
+    S.dev.kz = kz;
+    S.dev.S = Sc;
+    S.dev.r = r;
+    S.dev.p[0] = p0;
+    S.dev.p[1] = p1;
+    S.dev.q = q;
+    S.dev.xh = xh;
+    S.dev.x64 = (double*)x64;
+    S.dev.b64 = (const double*)b64;
+    for (int k = 0; k < 4; ++k) S.dev.s4[k] = (const double*)s4 + k * n;
+  } else {
+    void* f7;
+    A(&f7, n * 64);
+    void *u0 = nullptr, *u1 = nullptr, *rr = nullptr, *p0 = nullptr, *p1 = nullptr, *q = nullptr;
+    A(&u0, n * 8);
+    if (method_ == kJacobi) {
+      A(&u1, n * 8);
+    } else {
+      A(&rr, n * 8);
+      A(&p0, n * 8);
+      A(&p1, n * 8);
+      A(&q, n * 8);
+    }
+    if (rc != APHPOIS_OK) {
+      free_system(S);
+      slots_[id].reset();
+      return rc;
+    }
+    for (int k = 0; k < 8; ++k) S.dev.f7[k] = (const double*)f7 + k * n;
+    S.dev.u[0] = (double*)u0;
+    S.dev.u[1] = method_ == kJacobi ? (double*)u1 : (double*)u0;
+    S.dev.rr = (double*)rr;
+    S.dev.pp[0] = (double*)p0;
+    S.dev.pp[1] = (double*)p1;
+    S.dev.qq = (double*)q;
+  }
+  (void)p;
+  slots_[id] = std::move(s);
+  dirty_ = true;
+  *slot = id;
+  return APHPOIS_OK;
+}
+
+int Engine::remove(int slot) {
+  System* s = sys(slot);
+  if (!s) return fail(APHPOIS_EINVAL, "no such slot");
+  APH_HIP(hipStreamSynchronize(st_));
+  free_system(*s);
+  slots_[slot].reset();
+  dirty_ = true;
+  return APHPOIS_OK;
+}
+
+// Copies the strided range of a host or device field into the staging buffer;
+// *dptr then addresses slot 0 of the range (use start' = 0).
+int Engine::upload_strided(const double* src, bool on_device, long long start, long long ly,
+                           long long lz, int nx, int ny, int nz, int comps,
+                           const double** dptr) {
+  if (start < 0 || ly < nx || lz < ly * (long long)ny)
+    return fail(APHPOIS_EINVAL, "bad strides (need lead_y >= nx, lead_z >= lead_y*ny)");
+  const long long last = start + (long long)(nz - 1) * lz + (long long)(ny - 1) * ly + (nx - 1);
+  const long long count = (last - start + 1) * comps;
+  const long long need = count * 8;
+  if (need > staging_bytes_) {
+    if (staging_) {
+      APH_HIP(hipStreamSynchronize(st_));
+      APH_HIP(hipFree(staging_));
+      staging_ = nullptr;
+      staging_bytes_ = 0;
+    }
+    if (budget_ > 0 && bytes() + need > budget_)
+      return fail(APHPOIS_EBUDGET, "memory budget exceeded by the upload staging buffer");
+    APH_HIP(hipMalloc(&staging_, need));
+    staging_bytes_ = need;
+  }
+  APH_HIP(hipMemcpyAsync(staging_, src + start * comps, need,
+                         on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st_));
+  *dptr = (const double*)staging_;
+  return APHPOIS_OK;
+}
+
+int Engine::set_system(int slot, const double* expr, bool on_device, long long start,
+                       long long ly, long long lz) {
+  System* sp = sys(slot);
+  if (!sp) return fail(APHPOIS_EINVAL, "no such slot");
+  if (!expr) return fail(APHPOIS_EINVAL, "expr is NULL");
+  System& s = *sp;
+  const double* d = nullptr;
+  APH_TRY(upload_strided(expr, on_device, start, ly, lz, s.nx, s.ny, s.nz, 8, &d));
+  APH_HIP(hipMemsetAsync(d_flags_, 0, sizeof(unsigned), st_));
+  APH_HIP(launch_check(d, 0, ly, lz, s.nx, s.ny, s.nz, d_flags_, st_));
+  unsigned flags = 0;
+  APH_HIP(hipMemcpyAsync(&flags, d_flags_, sizeof(unsigned), hipMemcpyDeviceToHost, st_));
+  APH_HIP(hipStreamSynchronize(st_));
+  if (flags & 1u) return fail(APHPOIS_ENUMERIC, "system contains NaN or Inf");
+  if (method_ == kPcg) {
+    if (flags & 2u)
+      return fail(APHPOIS_EINVAL,
+                  "system is not symmetric (e_xm(c) != e_xp(c-x) etc.); PCG needs a symmetric "
+                  "system, use method CONJUGATE");
+    if (flags & 4u) return fail(APHPOIS_EINVAL, "zero diagonal coefficient");
+    if ((flags & 16u) && (flags & 32u))
+      return fail(APHPOIS_EINVAL, "diagonal changes sign; PCG needs a definite system");
+    s.neg = (flags & 32u) != 0;
+    const bool has_m = (flags & 64u) != 0;
+    if (has_m && !s.dev.m) {
+      void* m = nullptr;
+      APH_TRY(alloc(s, &m, s.n * (fp32_ ? 4 : 8)));
+      s.dev.m = m;
+    }
+    s.has_m = has_m;
+    s.dev.has_m = has_m ? 1 : 0;
+    s.dev.neg = s.neg ? 1 : 0;
+    s.singular = nullspace_ == APHPOIS_NULL_ON || (nullspace_ == APHPOIS_NULL_AUTO && !has_m);
+    APH_HIP(launch_convert_pcg(fp32_, d, 0, ly, lz, s.nx, s.ny, s.nz, s.neg ? -1.0 : 1.0,
+                               const_cast<void*>(s.dev.kx), const_cast<void*>(s.dev.ky),
+                               const_cast<void*>(s.dev.kz), const_cast<void*>(s.dev.S),
+                               has_m ? const_cast<void*>(s.dev.m) : nullptr,
+                               const_cast<double*>(s.dev.s4[0]), const_cast<double*>(s.dev.b64),
+                               0, st_));
+  } else {
+    if (method_ == kJacobi && (flags & 4u))
+      return fail(APHPOIS_EINVAL, "zero diagonal coefficient (Jacobi divides by e0)");
+    APH_HIP(launch_convert_f7(d, 0, ly, lz, s.nx, s.ny, s.nz, const_cast<double*>(s.dev.f7[0]),
+                              0, st_));
+  }
+  APH_HIP(hipStreamSynchronize(st_));
+  s.has_system = true;
+  s.have_x = false;
+  dirty_ = true;
+  return APHPOIS_OK;
+}
+
+int Engine::set_rhs(int slot, const double* e7, bool on_device, long long start, long long ly,
+                    long long lz) {
+  System* sp = sys(slot);
+  if (!sp) return fail(APHPOIS_EINVAL, "no such slot");
+  if (!sp->has_system) return fail(APHPOIS_ESTATE, "set_rhs before set_system");
+  if (!e7) return fail(APHPOIS_EINVAL, "e7 is NULL");
+  System& s = *sp;
+  const double* d = nullptr;
+  APH_TRY(upload_strided(e7, on_device, start, ly, lz, s.nx, s.ny, s.nz, 1, &d));
+  if (method_ == kPcg) {
+    APH_HIP(launch_convert_rhs(d, 0, ly, lz, s.nx, s.ny, s.nz, -1.0,
+                               const_cast<double*>(s.dev.b64), st_));
+  } else {
+    APH_HIP(launch_convert_rhs(d, 0, ly, lz, s.nx, s.ny, s.nz, 1.0,
+                               const_cast<double*>(s.dev.f7[7]), st_));
+  }
+  APH_HIP(hipStreamSynchronize(st_));
+  return APHPOIS_OK;
+}
+
+int Engine::prepare(int slot, const double* x0, bool on_device, long long start, long long ly,
+                    long long lz, bool keep_x) {
+  System* sp = sys(slot);
+  if (!sp) return fail(APHPOIS_EINVAL, "no such slot");
+  if (!sp->has_system) return fail(APHPOIS_ESTATE, "solve before set_system");
+  System& s = *sp;
+  const long long n = s.n;
+  double* xdst = method_ == kPcg ? s.dev.x64 : s.dev.u[0];
+  if (x0) {
+    const double* d = nullptr;
+    APH_TRY(upload_strided(x0, on_device, start, ly, lz, s.nx, s.ny, s.nz, 1, &d));
+    APH_HIP(launch_gather(d, 0, ly, lz, s.nx, s.ny, s.nz, xdst, 0.0, st_));
+  } else if (keep_x && s.have_x) {
+    if (method_ == kPcg) {
+      // x64 holds x + mean_pending; fold the pending mean in
+      const double mp = h_sc_[slot].mean_pending;
+      if (mp != 0.0) {
+        APH_HIP(launch_gather(s.dev.x64, 0, s.nx, (long long)s.nx * s.ny, s.nx, s.ny, s.nz,
+                              s.dev.x64, mp, st_));
+      }
+    } else if (method_ == kJacobi && h_sc_[slot].cur == 1) {
+      APH_HIP(hipMemcpyAsync(s.dev.u[0], s.dev.u[1], n * 8, hipMemcpyDeviceToDevice, st_));
+    }
+  } else {
+    APH_HIP(launch_fill(xdst, n, 8, st_));
+  }
+  if (method_ == kPcg) {
+    const int es = fp32_ ? 4 : 8;
+    APH_HIP(launch_fill(s.dev.xh, n, es, st_));
+    APH_HIP(launch_fill(s.dev.p[0], n, es, st_));
+    APH_HIP(launch_fill(s.dev.p[1], n, es, st_));
+  }
+  // bnorm2 (PCG): sum of b^2
+  double bnorm2 = 0;
+  if (method_ == kPcg) {
+    APH_HIP(launch_sumsq(s.dev.b64, n, d_red_, d_red_counter_, d_red_ + 1024, st_));
+    APH_HIP(hipMemcpyAsync(&bnorm2, d_red_ + 1024, sizeof(double), hipMemcpyDeviceToHost, st_));
+    APH_HIP(hipStreamSynchronize(st_));
+  }
+  Scalars sc{};
+  sc.tol = tol_;
+  sc.delta = delta_;
+  sc.h3 = s.h3;
+  sc.bnorm2 = bnorm2;
+  sc.gamma = 1.0;
+  sc.iter = 0;
+  sc.miniter = miniter_;
+  sc.maxiter = maxiter_;
+  sc.done = 0;
+  sc.ru = method_ == kPcg ? 1 : 0;
+  sc.resmode = resmode_;
+  sc.singular = s.singular ? 1 : 0;
+  sc.first = 1;
+  sc.cur = 0;
+  sc.method = method_;
+  h_sc_[slot] = sc;
+  APH_HIP(hipMemcpyAsync(d_sc_ + slot, h_sc_ + slot, sizeof(Scalars), hipMemcpyHostToDevice,
+                         st_));
+  if (!s.active) dirty_ = true;
+  s.active = true;
+  return APHPOIS_OK;
+}
+
+int Engine::build_tables() {
+  launch_slots_.clear();
+  long long ntot = 0;
+  bool all4 = true, all2 = true;
+  for (int i = 0; i < (int)slots_.size(); ++i) {
+    System* s = slots_[i].get();
+    if (s && s->active) {
+      launch_slots_.push_back(i);
+      ntot += s->n;
+      all4 = all4 && s->nx % 4 == 0;
+      all2 = all2 && s->nx % 2 == 0;
+    }
+  }
+  if (method_ == kPcg) {
+    vx_ = fp32_ ? (all4 ? 4 : (all2 ? 2 : 1)) : (all2 ? 2 : 1);
+  } else {
+    vx_ = 1;
+  }
+  const int ns = (int)launch_slots_.size();
+  std::vector<Rec> rst, rsm, rru;
+  std::vector<int> bst, bsm, bru;
+  std::vector<SysDev> sds;
+  const long long target = 3072;
+  for (int k = 0; k < ns; ++k) {
+    System& s = *slots_[launch_slots_[k]];
+    const long long tgt = std::max(1LL, (long long)((double)target * (double)s.n / (double)ntot));
+    Rec a = make_stencil_rec(s.nx, s.ny, s.nz, method_ == kPcg ? vx_ : 1, tgt, tuning_);
+    a.sys = k;
+    a.blk0 = (int)bst.size();
+    Tuning none;
+    Rec c = make_stencil_rec(s.nx, s.ny, s.nz, 1, tgt, none);
+    c.sys = k;
+    c.blk0 = (int)bru.size();
+    Rec b{};
+    b.sys = k;
+    b.blk0 = (int)bsm.size();
+    const long long per = (long long)kBlock * (method_ == kPcg ? vx_ : 1) * 16;
+    b.nblk = (int)std::min(4096LL, std::max(1LL, cdiv(s.n, per)));
+    b.tiles_x = b.nblk;
+    b.tiles_y = b.tiles_z = 1;
+    b.lx = kBlock;
+    b.bz = 1;
+    for (int i = 0; i < a.nblk; ++i) bst.push_back((int)rst.size());
+    for (int i = 0; i < b.nblk; ++i) bsm.push_back((int)rsm.size());
+    for (int i = 0; i < c.nblk; ++i) bru.push_back((int)rru.size());
+    rst.push_back(a);
+    rsm.push_back(b);
+    rru.push_back(c);
+    // partials: 3 values per block of the largest launch of this system
+    const int nmax = std::max(a.nblk, std::max(b.nblk, c.nblk));
+    const long long need = 3LL * nmax * 8;
+    if (need > s.partials_cap) {
+      if (s.partials) APH_HIP(hipFree(s.partials));
+      s.partials = nullptr;
+      s.partials_cap = 0;
+      APH_HIP(hipMalloc(&s.partials, need));
+      s.partials_cap = need;
+    }
+    s.dev.partials = (double*)s.partials;
+    sds.push_back(s.dev);
+  }
+  // device tables in one buffer each
+  const long long nrec = (long long)rst.size() + rsm.size() + rru.size();
+  const long long nb2r = (long long)bst.size() + bsm.size() + bru.size();
+  if (nrec > rec_cap_) {
+    if (d_rec_) APH_HIP(hipFree(d_rec_));
+    APH_HIP(hipMalloc(&d_rec_, sizeof(Rec) * std::max(1LL, nrec)));
+    rec_cap_ = nrec;
+  }
+  if (nb2r > b2r_cap_) {
+    if (d_b2r_) APH_HIP(hipFree(d_b2r_));
+    APH_HIP(hipMalloc(&d_b2r_, sizeof(int) * std::max(1LL, nb2r)));
+    b2r_cap_ = nb2r;
+  }
+  if (d_sds_) APH_HIP(hipFree(d_sds_));
+  d_sds_ = nullptr;
+  APH_HIP(hipMalloc(&d_sds_, sizeof(SysDev) * std::max(1, ns)));
+  std::vector<Rec> recs;
+  recs.insert(recs.end(), rst.begin(), rst.end());
+  recs.insert(recs.end(), rsm.begin(), rsm.end());
+  recs.insert(recs.end(), rru.begin(), rru.end());
+  std::vector<int> b2r;
+  b2r.insert(b2r.end(), bst.begin(), bst.end());
+  for (int v : bsm) b2r.push_back(v);
+  for (int v : bru) b2r.push_back(v);
+  if (ns > 0) {
+    APH_HIP(hipMemcpy(d_sds_, sds.data(), sizeof(SysDev) * ns, hipMemcpyHostToDevice));
+    APH_HIP(hipMemcpy(d_rec_, recs.data(), sizeof(Rec) * nrec, hipMemcpyHostToDevice));
+    APH_HIP(hipMemcpy(d_b2r_, b2r.data(), sizeof(int) * nb2r, hipMemcpyHostToDevice));
+  }
+  L_.sds = d_sds_;
+  L_.rec_st = d_rec_;
+  L_.rec_sm = d_rec_ + rst.size();
+  L_.rec_ru = d_rec_ + rst.size() + rsm.size();
+  L_.b2r_st = d_b2r_;
+  L_.b2r_sm = d_b2r_ + bst.size();
+  L_.b2r_ru = d_b2r_ + bst.size() + bsm.size();
+  L_.nst = (int)bst.size();
+  L_.nsm = (int)bsm.size();
+  L_.nru = (int)bru.size();
+  dirty_ = false;
+  return APHPOIS_OK;
+}
+
+int Engine::capture() {
+  if (gexec_) {
+    APH_HIP(hipGraphExecDestroy(gexec_));
+    gexec_ = nullptr;
+  }
+  if (graph_) {
+    APH_HIP(hipGraphDestroy(graph_));
+    graph_ = nullptr;
+  }
+  int k = check_every_;
+  if (k <= 0) {
+    long long ntot = 0;
+    for (int i : launch_slots_) ntot += slots_[i]->n;
+    const double bpc = method_ == kPcg ? (fp32_ ? 60.0 : 120.0) : 150.0;
+    const double t_iter = (double)ntot * bpc / 4e12 + 20e-6;
+    k = (int)std::lround(4e-3 / t_iter);
+    k = std::max(2, std::min(64, k));
+  }
+  if (method_ == kJacobi && (k & 1)) ++k;
+  APH_HIP(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
+  for (int i = 0; i < k; ++i) {
+    hipError_t e = launch_iter(method_, fp32_, vx_, L_, st_);
+    if (e != hipSuccess) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(st_, &g);
+      if (g) (void)hipGraphDestroy(g);
+      return hip(e, "launch_iter (capture)");
+    }
+  }
+  APH_HIP(hipStreamEndCapture(st_, &graph_));
+  APH_HIP(hipGraphInstantiate(&gexec_, graph_, nullptr, nullptr, 0));
+  graph_k_ = k;
+  return APHPOIS_OK;
+}
+
+int Engine::run() {
+  bool any = false;
+  for (auto& s : slots_) any = any || (s && s->active);
+  if (!any) return APHPOIS_OK;
+  if (dirty_) {
+    APH_TRY(build_tables());
+    APH_TRY(capture());
+  }
+  APH_HIP(hipEventRecord(ev0_, st_));
+  APH_HIP(launch_start(method_, fp32_, vx_, L_, st_));
+  const int maxloops = maxiter_ / std::max(1, graph_k_) + 4;
+  int loops = 0;
+  bool alldone = false;
+  while (!alldone) {
+    APH_HIP(hipGraphLaunch(gexec_, st_));
+    ++loops;
+    for (int i : launch_slots_) {
+      APH_HIP(hipMemcpyAsync(h_sc_ + i, d_sc_ + i, sizeof(Scalars), hipMemcpyDeviceToHost, st_));
+    }
+    APH_HIP(hipStreamSynchronize(st_));
+    alldone = true;
+    for (int i : launch_slots_) alldone = alldone && h_sc_[i].done;
+    if (loops > maxloops + 2 && !alldone) {
+      // every system hits maxiter within maxloops replays; reaching here means
+      // a flag was never set
+      return fail(APHPOIS_EDEVICE, "solver did not terminate within maxiter");
+    }
+  }
+  APH_HIP(hipEventRecord(ev1_, st_));
+  if (method_ != kPcg) {
+    APH_HIP(launch_true_res_f7(L_, st_));
+  }
+  for (int i : launch_slots_) {
+    APH_HIP(hipMemcpyAsync(h_sc_ + i, d_sc_ + i, sizeof(Scalars), hipMemcpyDeviceToHost, st_));
+  }
+  APH_HIP(hipStreamSynchronize(st_));
+  float ms = 0;
+  APH_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+  last_run_ms_ = ms;
+  last_run_iters_ = 0;
+  for (int i : launch_slots_) {
+    System& s = *slots_[i];
+    const Scalars& c = h_sc_[i];
+    aphpois_info& inf = s.info;
+    std::memset(&inf, 0, sizeof(inf));
+    inf.iter = c.iter;
+    inf.residual = c.residual;
+    inf.res_l2 = c.res_l2;
+    inf.res_max = c.res_max;
+    inf.true_relres = c.relres;
+    inf.mean = c.mean_pending;
+    inf.converged = c.converged;
+    inf.refinements = c.nru;
+    inf.status = c.status;
+    inf.seconds = ms * 1e-3;
+    last_run_iters_ = std::max(last_run_iters_, c.iter);
+    s.active = false;
+    s.have_x = true;
+  }
+  dirty_ = true;  // the active set changes with the next prepare
+  return APHPOIS_OK;
+}
+
+int Engine::fetch(int slot, double* x, bool on_device, long long start, long long ly,
+                  long long lz, aphpois_info* info) {
+  System* sp = sys(slot);
+  if (!sp) return fail(APHPOIS_EINVAL, "no such slot");
+  System& s = *sp;
+  if (info) *info = s.info;
+  if (!x) return APHPOIS_OK;
+  if (!s.have_x) return fail(APHPOIS_ESTATE, "no solution yet");
+  const double* src;
+  double sub = 0;
+  if (method_ == kPcg) {
+    src = s.dev.x64;
+    sub = h_sc_[slot].mean_pending;
+  } else if (method_ == kJacobi) {
+    src = s.dev.u[h_sc_[slot].cur];
+  } else {
+    src = s.dev.u[0];
+  }
+  const bool dense = start == 0 && ly == s.nx && lz == (long long)s.nx * s.ny;
+  if (on_device) {
+    APH_HIP(launch_scatter(src, s.nx, s.ny, s.nz, x, start, ly, lz, sub, st_));
+    APH_HIP(hipStreamSynchronize(st_));
+    return APHPOIS_OK;
+  }
+  // dense result in the staging buffer, then host copy
+  if (s.n * 8 > staging_bytes_) {
+    if (staging_) {
+      APH_HIP(hipFree(staging_));
+      staging_ = nullptr;
+      staging_bytes_ = 0;
+    }
+    APH_HIP(hipMalloc(&staging_, s.n * 8));
+    staging_bytes_ = s.n * 8;
+  }
+  double* st = (double*)staging_;
+  APH_HIP(launch_scatter(src, s.nx, s.ny, s.nz, st, 0, s.nx, (long long)s.nx * s.ny, sub, st_));
+  if (dense) {
+    APH_HIP(hipMemcpyAsync(x, st, s.n * 8, hipMemcpyDeviceToHost, st_));
+    APH_HIP(hipStreamSynchronize(st_));
+  } else {
+    std::vector<double> tmp(s.n);
+    APH_HIP(hipMemcpyAsync(tmp.data(), st, s.n * 8, hipMemcpyDeviceToHost, st_));
+    APH_HIP(hipStreamSynchronize(st_));
+    for (int z = 0; z < s.nz; ++z)
+      for (int y = 0; y < s.ny; ++y)
+        std::memcpy(x + start + z * lz + y * ly, tmp.data() + ((long long)z * s.ny + y) * s.nx,
+                    sizeof(double) * s.nx);
+  }
+  return APHPOIS_OK;
+}
+
+}  // namespace aph

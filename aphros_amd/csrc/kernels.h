@@ -1,0 +1,42 @@
+// Launch wrappers of the aphpois kernels (kernels.hip).
+#pragma once
+
+#include "common.h"
+
+namespace aph {
+
+// Device tables of one engine launch configuration.
+struct LaunchSet {
+  const SysDev* sds = nullptr;
+  const Rec* rec_st = nullptr;  // stencil launch of the iteration kernel
+  const int* b2r_st = nullptr;
+  const Rec* rec_sm = nullptr;  // streaming launches
+  const int* b2r_sm = nullptr;
+  const Rec* rec_ru = nullptr;  // one-cell-per-thread stencil launches (FP64 residuals)
+  const int* b2r_ru = nullptr;
+  int nst = 0, nsm = 0, nru = 0;
+};
+
+hipError_t launch_iter(int method, int fp32, int vx, const LaunchSet& L, hipStream_t st);
+hipError_t launch_start(int method, int fp32, int vx, const LaunchSet& L, hipStream_t st);
+hipError_t launch_true_res_f7(const LaunchSet& L, hipStream_t st);
+hipError_t launch_check(const double* aos, long long start, long long ly, long long lz, int nx,
+                        int ny, int nz, unsigned* flags, hipStream_t st);
+hipError_t launch_convert_f7(const double* aos, long long start, long long ly, long long lz,
+                             int nx, int ny, int nz, double* f7, int coefs_only, hipStream_t st);
+hipError_t launch_convert_pcg(int fp32, const double* aos, long long start, long long ly,
+                              long long lz, int nx, int ny, int nz, double sgn, void* kx,
+                              void* ky, void* kz, void* S, void* m, double* s4, double* b,
+                              int coefs_only, hipStream_t st);
+hipError_t launch_convert_rhs(const double* src, long long start, long long ly, long long lz,
+                              int nx, int ny, int nz, double scale, double* dst, hipStream_t st);
+hipError_t launch_gather(const double* src, long long start, long long ly, long long lz, int nx,
+                         int ny, int nz, double* dst, double sub, hipStream_t st);
+hipError_t launch_scatter(const double* src, int nx, int ny, int nz, double* dst,
+                          long long start, long long ly, long long lz, double sub,
+                          hipStream_t st);
+hipError_t launch_sumsq(const double* v, long long n, double* partials, unsigned* counter,
+                        double* out, hipStream_t st);
+hipError_t launch_fill(void* p, long long n, int elem_bytes, hipStream_t st);
+
+}  // namespace aph

@@ -1,0 +1,264 @@
+"""Python mirror of Aphros' linear-solver plugin interface, over libaphpois.
+
+Names, argument meaning and error behaviour follow the reference
+(src/linear/linear.h:15-76, src/linear/linear.ipp:239-265,
+src/util/linear.ipp:9-28):
+
+    factory = ModuleLinear.GetInstance(var.String["linsolver_symm"])
+    solver = factory.Make(var, "symm", m)
+    info = solver.Solve(fc_system, fc_init, fc_sol, m)   # Info(residual, iter)
+
+Fields are numpy arrays over the block's index box: fc_system has 8 doubles
+per slot (Expr), fc_sol one double per slot; `m` (Mesh) gives the inner size
+and the halo width, from which the start/lead_y/lead_z strides follow.  The
+inner cells are solved on the GPU; halos of fc_sol are filled periodically
+after the solve, which is what m.Comm(&fc_sol) does in the reference
+(src/linear/linear.ipp:116-118).
+
+Registered modules:
+    "hip_conjugate"  SolverConjugate restated on the GPU (FP64, same stop rule)
+    "hip_jacobi"     SolverJacobi restated on the GPU
+    "hip"            Jacobi-preconditioned CG with FP64 reliable updates;
+                     linsolver_<prefix>_precision: "mixed" (default) | "fp64"
+                     linsolver_<prefix>_relres: 1 -> tol is ||b-Ax||/||b||,
+                     0 (default) -> tol is the reference norm sqrt(sum r^2/h^3)
+Errors raise RuntimeError (the reference's fassert throws std::runtime_error,
+src/util/logger.h:45-70).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+
+class Vars:
+    """Typed configuration maps (src/parse/vars.h:11-40)."""
+
+    class _Map(dict):
+        def __call__(self, key, default):
+            return self.get(key, default)
+
+        def __getitem__(self, key):
+            if key not in self:
+                raise RuntimeError(f"variable '{key}' not found")
+            return dict.__getitem__(self, key)
+
+    def __init__(self, Int=None, Double=None, String=None):
+        self.Int = Vars._Map(Int or {})
+        self.Double = Vars._Map(Double or {})
+        self.String = Vars._Map(String or {})
+
+
+class Mesh:
+    """Block geometry: inner cells `size` (nx, ny, nz), cell size `h`, halo
+    width `hl`.  The index box is size + 2*hl + 1 per direction, x fastest
+    (src/geom/mesh.ipp:68,76, src/geom/block.h:149-157)."""
+
+    def __init__(self, size, h=None, hl=0, pad=None):
+        self.size = tuple(int(s) for s in size)
+        self.h = tuple(h) if h is not None else tuple(1.0 / s for s in self.size)
+        self.hl = int(hl)
+        self.pad = (1 if hl > 0 else 0) if pad is None else int(pad)
+
+    def box(self):
+        return tuple(s + 2 * self.hl + self.pad for s in self.size)
+
+    def strides(self):
+        bx, by, _ = self.box()
+        lead_y, lead_z = bx, bx * by
+        start = self.hl * (1 + lead_y + lead_z)
+        return start, lead_y, lead_z
+
+    def GetCellSize(self):
+        return self.h
+
+    def nslots(self):
+        bx, by, bz = self.box()
+        return bx * by * bz
+
+    def fill_halos(self, fc):
+        """Periodic halo fill of a scalar field over the index box."""
+        if self.hl == 0:
+            return
+        nx, ny, nz = self.size
+        bx, by, bz = self.box()
+        a = fc.reshape(bz, by, bx)
+        hl = self.hl
+        for ax, n in ((2, nx), (1, ny), (0, nz)):
+            src = np.arange(-hl, n + hl + self.pad) % n + hl
+            a[...] = np.take(a, src, axis=ax)
+
+
+class Conf:
+    def __init__(self, tol=0.0, miniter=1, maxiter=100):
+        self.tol = tol
+        self.miniter = miniter
+        self.maxiter = maxiter
+
+
+class Info:
+    def __init__(self, residual=0.0, iter=0, extra=None):
+        self.residual = residual
+        self.iter = iter
+        self.extra = extra or {}
+
+
+class Solver:
+    """linear::Solver<M> (src/linear/linear.h:15-57)."""
+
+    def __init__(self, conf):
+        self.conf = conf
+
+    def Solve(self, fc_system, fc_init, fc_sol, m):
+        raise NotImplementedError
+
+    def SetConf(self, conf):
+        self.conf = conf
+
+    def GetConf(self):
+        return self.conf
+
+
+class SolverHip(Solver):
+    """GPU solver behind the C ABI; one resident device system per solver."""
+
+    def __init__(self, conf, method, precision=L.FP64, resmode=L.RES_L2,
+                 nullspace=L.NULL_AUTO, device=0, name="hip"):
+        super().__init__(conf)
+        self.method, self.precision, self.resmode = method, precision, resmode
+        self.nullspace, self.device, self.name = nullspace, device, name
+        self._h = None
+        self._size = None
+        self.last_info = None
+
+    def _handle(self, m):
+        if self._h is not None and self._size == m.size:
+            return self._h
+        self.close()
+        d = L.Desc()
+        L.lib().aphpois_desc_default(C.byref(d))
+        d.nx, d.ny, d.nz = m.size
+        h = m.GetCellSize()
+        d.h3 = h[0] * h[1] * h[2]
+        d.method, d.precision, d.resmode = self.method, self.precision, self.resmode
+        d.tol, d.miniter, d.maxiter = self.conf.tol, self.conf.miniter, self.conf.maxiter
+        d.nullspace, d.device = self.nullspace, self.device
+        hp = C.c_void_p()
+        L.check(L.lib().aphpois_create(C.byref(d), C.byref(hp)))
+        self._h, self._size = hp, m.size
+        return hp
+
+    def Solve(self, fc_system, fc_init, fc_sol, m):
+        h = self._handle(m)
+        start, ly, lz = m.strides()
+        fs = np.ascontiguousarray(fc_system, dtype=np.float64).reshape(-1)
+        if fs.size != 8 * m.nslots():
+            raise RuntimeError("fc_system does not match the mesh index box")
+        if fc_sol.dtype != np.float64 or not fc_sol.flags["C_CONTIGUOUS"] or \
+                fc_sol.size != m.nslots():
+            raise RuntimeError("fc_sol must be a contiguous float64 field over the index box")
+        L.check(L.lib().aphpois_set_conf(h, self.conf.tol, self.conf.miniter,
+                                         self.conf.maxiter))
+        L.check(L.lib().aphpois_set_system(h, L.dptr(fs), start, ly, lz))
+        x0 = None
+        if fc_init is not None:
+            x0 = fc_init if fc_init is fc_sol else np.ascontiguousarray(fc_init, np.float64)
+        info = L.Info()
+        L.check(L.lib().aphpois_solve(h, L.dptr(x0), L.dptr(fc_sol), start, ly, lz,
+                                      C.byref(info)))
+        m.fill_halos(fc_sol)
+        self.last_info = info.as_dict()
+        return Info(info.residual, info.iter, self.last_info)
+
+    def close(self):
+        if self._h is not None:
+            L.lib().aphpois_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ModuleLinear:
+    """Registry of linear-solver modules (src/linear/linear.h:59-76,
+    src/util/module.h:21-85)."""
+
+    _instances = {}
+
+    def __init__(self, name):
+        self.name = name
+        ModuleLinear._instances[name] = self
+
+    @classmethod
+    def GetInstance(cls, name):
+        return cls._instances.get(name)
+
+    @classmethod
+    def GetInstances(cls):
+        return dict(cls._instances)
+
+    @staticmethod
+    def GetConf(var, prefix):
+        """Conf from hypre_<prefix>_{tol,maxiter,miniter} (linear.h:66-75)."""
+        def add(name):
+            return "hypre_" + prefix + "_" + name
+        return Conf(tol=var.Double[add("tol")], maxiter=var.Int[add("maxiter")],
+                    miniter=var.Int(add("miniter"), 0))
+
+    def Make(self, var, prefix, m):
+        raise NotImplementedError
+
+
+class ModuleLinearHipConjugate(ModuleLinear):
+    def __init__(self):
+        super().__init__("hip_conjugate")
+
+    def Make(self, var, prefix, m):
+        maxnorm = var.Int["linsolver_" + prefix + "_maxnorm"]
+        return SolverHip(self.GetConf(var, prefix), L.CONJUGATE,
+                         resmode=L.RES_MAX if maxnorm else L.RES_L2, name="hip_conjugate")
+
+
+class ModuleLinearHipJacobi(ModuleLinear):
+    def __init__(self):
+        super().__init__("hip_jacobi")
+
+    def Make(self, var, prefix, m):
+        return SolverHip(self.GetConf(var, prefix), L.JACOBI, name="hip_jacobi")
+
+
+class ModuleLinearHip(ModuleLinear):
+    def __init__(self):
+        super().__init__("hip")
+
+    def Make(self, var, prefix, m):
+        def key(name):
+            return "linsolver_" + prefix + "_" + name
+        prec = var.String(key("precision"), "mixed")
+        if prec not in ("mixed", "fp64"):
+            raise RuntimeError(f"unknown {key('precision')} '{prec}'")
+        if var.Int(key("relres"), 0):
+            resmode = L.RES_REL_TRUE
+        else:
+            resmode = L.RES_MAX if var.Int(key("maxnorm"), 0) else L.RES_L2
+        return SolverHip(self.GetConf(var, prefix), L.PCG,
+                         precision=L.MIXED if prec == "mixed" else L.FP64,
+                         resmode=resmode, name="hip")
+
+
+ModuleLinearHipConjugate()
+ModuleLinearHipJacobi()
+ModuleLinearHip()
+
+
+def MakeLinearSolver(var, prefix, m):
+    """ULinear::MakeLinearSolver (src/util/linear.ipp:9-28)."""
+    name = var.String["linsolver_" + prefix]
+    factory = ModuleLinear.GetInstance(name)
+    if factory is None:
+        raise RuntimeError("Solver not found: " + name)
+    return factory.Make(var, prefix, m)

@@ -1,0 +1,204 @@
+/*
+ * aphpois — MI355X-native pressure-Poisson solver, C ABI.
+ *
+ * Drop-in for the symmetric linear-solver plugin of Aphros (`linsolver_symm`,
+ * src/linear/linear.h:15-57).  The system is the per-cell 7-point Expr
+ *     e0*x[c] + e1*x[xm] + e2*x[xp] + e3*x[ym] + e4*x[yp] + e5*x[zm] + e6*x[zp]
+ *       + e7 = 0
+ * (src/linear/linear.h:37-39, src/geom/mesh.h:482-485), neighbour order
+ * q = 2*d + side (src/geom/mesh.ipp:100-107).  Plain pointers and sizes only.
+ *
+ * Every function returns APHPOIS_OK (0) or an error code; the message of the
+ * last error on the calling thread is returned by aphpois_last_error().
+ * Nothing aborts: a failing call leaves other handles untouched.
+ */
+#ifndef APHPOIS_H
+#define APHPOIS_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+enum {
+  APHPOIS_OK = 0,
+  APHPOIS_EINVAL = 1,   /* bad argument or unsupported system */
+  APHPOIS_ENOMEM = 2,   /* device allocation failed */
+  APHPOIS_EDEVICE = 3,  /* HIP runtime error */
+  APHPOIS_ESTATE = 4,   /* call out of order (e.g. solve before set_system) */
+  APHPOIS_EBUDGET = 5,  /* memory budget would be exceeded */
+  APHPOIS_ENUMERIC = 6, /* NaN/Inf in the system or breakdown in the solve */
+  APHPOIS_ECANCEL = 7   /* request cancelled */
+};
+
+/* ---- method: which solver runs ---- */
+enum {
+  /* SolverConjugate restated on the GPU: unpreconditioned CG, FP64, the
+   * reference stop rule (src/linear/linear.ipp:24-129). */
+  APHPOIS_CONJUGATE = 0,
+  /* SolverJacobi restated on the GPU (src/linear/linear.ipp:158-217). */
+  APHPOIS_JACOBI = 1,
+  /* Jacobi-preconditioned CG with reliable updates in FP64 (new; symmetric
+   * systems with a same-sign diagonal). */
+  APHPOIS_PCG = 2
+};
+
+/* ---- precision of the PCG inner iteration ---- */
+enum {
+  APHPOIS_FP64 = 0,
+  /* FP32 face conductances, scale and Krylov vectors; FP64 true residual,
+   * FP64 reductions, FP64 solution and mean removal. */
+  APHPOIS_MIXED = 1
+};
+
+/* ---- residual / stop rule ---- */
+enum {
+  /* sqrt(sum r^2 / h3) < tol  (src/linear/linear.ipp:103-111) */
+  APHPOIS_RES_L2 = 0,
+  /* max|r| / h3 < tol  (linsolver_symm_maxnorm = 1) */
+  APHPOIS_RES_MAX = 1,
+  /* ||b - A x||_2 / ||b||_2 <= tol, b = -e7, FP64 (PCG only) */
+  APHPOIS_RES_REL_TRUE = 2
+};
+
+/* ---- nullspace handling for PCG ---- */
+enum {
+  APHPOIS_NULL_AUTO = 0, /* remove the mean when the system is singular */
+  APHPOIS_NULL_OFF = 1,
+  APHPOIS_NULL_ON = 2
+};
+
+typedef struct aphpois_desc {
+  int nx, ny, nz;   /* inner cells; the grid is periodic in x, y and z */
+  double h3;        /* GetCellSize().prod(), used by the reference norms */
+  int method;       /* APHPOIS_CONJUGATE | APHPOIS_JACOBI | APHPOIS_PCG */
+  int precision;    /* APHPOIS_FP64 | APHPOIS_MIXED (PCG only) */
+  int resmode;      /* APHPOIS_RES_* */
+  double tol;       /* Conf::tol */
+  int miniter;      /* Conf::miniter */
+  int maxiter;      /* Conf::maxiter */
+  int nullspace;    /* APHPOIS_NULL_* (PCG only) */
+  double delta;     /* reliable-update trigger (PCG), 0 = default 0.1 */
+  int check_every;  /* iterations between host polls, 0 = auto */
+  int device;       /* HIP device ordinal */
+  long long mem_budget; /* bytes of device memory this handle may use, 0 = no limit */
+} aphpois_desc;
+
+typedef struct aphpois_info {
+  int iter;           /* Info::iter */
+  double residual;    /* Info::residual in the handle's resmode */
+  double res_l2;      /* sqrt(sum r^2 / h3) of the final residual */
+  double res_max;     /* max|r| / h3 of the final residual */
+  double true_relres; /* ||b - A x|| / ||b|| recomputed in FP64 at the end */
+  double mean;        /* mean removed from x (PCG with nullspace removal) */
+  int converged;      /* 1 if the stop rule was met before maxiter */
+  int refinements;    /* PCG: FP64 true-residual (reliable) updates */
+  int status;         /* APHPOIS_OK or APHPOIS_ENUMERIC on breakdown */
+  double seconds;     /* wall time of the solve on the device stream */
+} aphpois_info;
+
+typedef struct aphpois aphpois;
+
+/* Fills *d with defaults: method CONJUGATE, FP64, RES_L2, tol 0, miniter 1,
+ * maxiter 100 (linear::Solver::Conf defaults, src/linear/linear.h:21-25). */
+void aphpois_desc_default(aphpois_desc* d);
+
+/* Creates a solver handle (replaces ModuleLinear::Make / the Solver
+ * constructor, src/linear/linear.ipp:243-252). */
+int aphpois_create(const aphpois_desc* desc, aphpois** out);
+
+/* Uploads the system.  expr points to host memory holding 8 doubles per slot
+ * (Expr, AoS); cell (ix,iy,iz) of the inner grid is at slot
+ *   start + iz*lead_z + iy*lead_y + ix
+ * exactly as d_system is addressed by the reference's OpenCL kernel
+ * (src/linear/conjugate_cl.cl:14-18, src/opencl/opencl.ipp:310-312).
+ * For a dense x-fastest array pass start=0, lead_y=nx, lead_z=nx*ny. */
+int aphpois_set_system(aphpois* h, const double* expr, long long start,
+                       long long lead_y, long long lead_z);
+
+/* Same, with expr in device memory (dense x-fastest, 8 doubles per cell). */
+int aphpois_set_system_device(aphpois* h, const double* expr_dev);
+
+/* Replaces only the constant term e7 (host, same strides as set_system,
+ * one double per slot).  Coefficients stay resident. */
+int aphpois_set_rhs(aphpois* h, const double* e7, long long start,
+                    long long lead_y, long long lead_z);
+
+/* Solves (replaces Solver::Solve, src/linear/linear.h:45-47).  x0 may be NULL
+ * (zero guess) or equal x (warm start); both are host scalar fields with the
+ * strides given; only inner cells are read and written. */
+int aphpois_solve(aphpois* h, const double* x0, double* x, long long start,
+                  long long lead_y, long long lead_z, aphpois_info* info);
+
+/* Same with dense device arrays (x0_dev may be NULL or equal x_dev). */
+int aphpois_solve_device(aphpois* h, const double* x0_dev, double* x_dev,
+                         aphpois_info* info);
+
+/* Replaces Solver::SetConf (src/linear/linear.h:48-50). */
+int aphpois_set_conf(aphpois* h, double tol, int miniter, int maxiter);
+
+/* Bytes of device memory held by the handle. */
+long long aphpois_device_bytes(const aphpois* h);
+
+int aphpois_destroy(aphpois* h);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* aphpois_last_error(void);
+
+/* ---------------- batched solves (many independent systems) -------------- */
+
+typedef struct aphpois_batch aphpois_batch;
+
+typedef struct aphpois_batch_desc {
+  int precision;        /* APHPOIS_FP64 | APHPOIS_MIXED; method is PCG */
+  int resmode;          /* APHPOIS_RES_REL_TRUE recommended */
+  double tol;
+  int maxiter;
+  double delta;         /* 0 = default */
+  int check_every;      /* 0 = auto */
+  int device;
+  long long mem_budget; /* bytes, 0 = no limit */
+  int max_systems;      /* slot capacity, 0 = 1024 */
+} aphpois_batch_desc;
+
+int aphpois_batch_create(const aphpois_batch_desc* d, aphpois_batch** out);
+/* Adds a periodic system (dense host Expr AoS, x-fastest).  x0 may be NULL.
+ * Returns the slot id in *slot. */
+int aphpois_batch_add(aphpois_batch* b, int nx, int ny, int nz, double h3,
+                      const double* expr, const double* x0, int* slot);
+/* Replaces the constant term of a resident system (dense host, one double per
+ * cell) and, if x0 is not NULL, its initial guess; if x0 is NULL the previous
+ * solution is the initial guess (warm start). */
+int aphpois_batch_set_rhs(aphpois_batch* b, int slot, const double* e7,
+                          const double* x0);
+/* Solves every resident system that is not yet converged, all in the same
+ * launches. */
+int aphpois_batch_solve(aphpois_batch* b);
+/* Copies a system's solution (dense host) and info. */
+int aphpois_batch_get(aphpois_batch* b, int slot, double* x, aphpois_info* info);
+int aphpois_batch_remove(aphpois_batch* b, int slot);
+long long aphpois_batch_device_bytes(const aphpois_batch* b);
+int aphpois_batch_destroy(aphpois_batch* b);
+
+/* ------------------------- synthetic systems (host) ---------------------- */
+
+/* Density field of the benchmark systems: unit periodic cube, n^3 cells,
+ * `nspheres` spheres with centres uniform in [0,1)^3 and radii uniform in
+ * [rmin, rmax] drawn from splitmix64 streams of `seed`; rho = 1 inside,
+ * `ratio` outside (periodic distance). */
+int aphpois_synth_density(int n, int nspheres, double rmin, double rmax,
+                          double ratio, unsigned long long seed, double* rho);
+/* RHS f uniform in [-1,1] from the splitmix64 stream of `seed`, mean removed. */
+int aphpois_synth_rhs(int n, unsigned long long seed, double* f);
+/* Pressure Expr (AoS, dense) from cell density and volume source with the
+ * reference's harmonic face density and projection sign
+ * (src/solver/proj.ipp:343-383, src/solver/approx_eb.h:351-363), h = 1/n,
+ * dt = 1, e7 = -f*h^3. */
+int aphpois_synth_assemble(int n, const double* rho, const double* f,
+                           double* expr);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* APHPOIS_H */

@@ -1,0 +1,17 @@
+#!/bin/bash
+# Builds oracle/_ref/ref_driver: the harness oracle/ref_driver.cpp linked with
+# the six reference source files it needs, compiled directly from
+# /root/reference/src (never copied).  Output only under oracle/_ref/.
+set -euo pipefail
+HERE="$(cd "$(dirname "$0")" && pwd)"
+REF=/root/reference/src
+OUT="$HERE/_ref"
+mkdir -p "$OUT"
+FLAGS="-O2 -std=c++14 -D_ALIGNBYTES_=16 -D_USE_BACKEND_LOCAL_=1 -D_USE_BACKEND_NATIVE_=1 \
+ -D_USE_DIM1_=0 -D_USE_DIM2_=0 -D_USE_DIM3_=1 -D_USE_DIM4_=0 -D_USE_FPZIP_=0 -D_USE_AMGX_=0 \
+ -D_USE_AVX_=0 -D_USE_BACKEND_CUBISM_=0 -D_USE_HDF_=0 -D_USE_HYPRE_=0 -D_USE_MPI_=0 \
+ -D_USE_OPENCL_=0 -D_USE_OPENMP_=0 -I$REF"
+g++ $FLAGS -o "$OUT/ref_driver.tmp" "$HERE/ref_driver.cpp" \
+  "$REF/geom/mesh.cpp" "$REF/util/suspender.cpp" "$REF/parse/vars.cpp" \
+  "$REF/util/logger.cpp" "$REF/util/format.cpp"
+mv "$OUT/ref_driver.tmp" "$OUT/ref_driver"
