@@ -69,6 +69,12 @@ SIGNATURES = {
     "aphpois_device_bytes": (C.c_longlong, [_vp]),
     "aphpois_destroy": (C.c_int, [_vp]),
     "aphpois_last_error": (C.c_char_p, []),
+    "aphpois_set_profiling": (C.c_int, [_vp, C.c_int]),
+    "aphpois_kernel_times": (C.c_int, [_vp, _dp, C.POINTER(C.c_longlong), C.POINTER(C.c_int)]),
+    "aphpois_launch_info": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int)]),
+    "aphpois_set_tile": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "aphpois_batch_set_tile": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int]),
     "aphpois_batch_create": (C.c_int, [C.POINTER(BatchDesc), C.POINTER(_vp)]),
     "aphpois_batch_add": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_double, _dp, _dp,
                                     C.POINTER(C.c_int)]),

@@ -131,6 +131,44 @@ int aphpois_set_conf(aphpois* h, double tol, int miniter, int maxiter) {
 
 long long aphpois_device_bytes(const aphpois* h) { return h ? h->eng.bytes() : 0; }
 
+int aphpois_set_profiling(aphpois* h, int on) {
+  APH_CHECK_H(h);
+  h->eng.set_profile(on != 0);
+  return APHPOIS_OK;
+}
+
+int aphpois_kernel_times(aphpois* h, double* avg_ms, long long* count, int* nkernels) {
+  APH_CHECK_H(h);
+  const int nk = h->eng.prof_kernels();
+  if (nkernels) *nkernels = nk;
+  for (int j = 0; j < nk; ++j) {
+    if (avg_ms) avg_ms[j] = h->eng.prof_avg_ms(j);
+    if (count) count[j] = h->eng.prof_count(j);
+  }
+  return APHPOIS_OK;
+}
+
+int aphpois_launch_info(aphpois* h, int* stencil_blocks, int* stream_blocks, int* vx) {
+  APH_CHECK_H(h);
+  if (stencil_blocks) *stencil_blocks = h->eng.blocks_stencil();
+  if (stream_blocks) *stream_blocks = h->eng.blocks_stream();
+  if (vx) *vx = h->eng.vx();
+  return APHPOIS_OK;
+}
+
+int aphpois_set_tile(aphpois* h, int lx, int bz, int vx, int sm) {
+  APH_CHECK_H(h);
+  if (lx < 0 || lx > 256 || (lx & (lx - 1)) || bz < 0 || vx < 0 || vx > 4 || sm < 0)
+    return set_err(APHPOIS_EINVAL, "bad tile (lx a power of two <= 256, vx in 0,1,2,4)");
+  aph::Tuning t;
+  t.lx = lx;
+  t.bz = bz;
+  t.vx = vx;
+  t.sm = sm;
+  h->eng.set_tuning(t);
+  return APHPOIS_OK;
+}
+
 int aphpois_destroy(aphpois* h) {
   delete h;
   return APHPOIS_OK;
@@ -203,6 +241,19 @@ int aphpois_batch_remove(aphpois_batch* b, int slot) {
 }
 
 long long aphpois_batch_device_bytes(const aphpois_batch* b) { return b ? b->eng.bytes() : 0; }
+
+int aphpois_batch_set_tile(aphpois_batch* b, int lx, int bz, int vx, int sm) {
+  APH_CHECK_H(b);
+  if (lx < 0 || lx > 256 || (lx & (lx - 1)) || bz < 0 || vx < 0 || vx > 4 || sm < 0)
+    return set_err(APHPOIS_EINVAL, "bad tile (lx a power of two <= 256, vx in 0,1,2,4)");
+  aph::Tuning t;
+  t.lx = lx;
+  t.bz = bz;
+  t.vx = vx;
+  t.sm = sm;
+  b->eng.set_tuning(t);
+  return APHPOIS_OK;
+}
 
 int aphpois_batch_destroy(aphpois_batch* b) {
   delete b;

@@ -67,7 +67,9 @@ struct SysDev {
   unsigned* counter;
   Scalars* sc;
   int neg;  // 1 if the diagonal is negative (A = -L): r^ = -S r64
-  int pad;
+  // workgroups of this system in the stencil, streaming and residual
+  // launches (= number of partials each finalize kernel reads)
+  int nblk_st, nblk_sm, nblk_ru;
 };
 
 // One record per system in a launch: which blocks belong to it and how they

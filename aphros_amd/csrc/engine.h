@@ -20,8 +20,10 @@
 namespace aph {
 
 struct Tuning {
-  int lx = 0;  // 0 = automatic
-  int bz = 0;
+  int lx = 0;  // threads per row of a stencil tile, 0 = automatic
+  int bz = 0;  // planes per stencil workgroup, 0 = automatic
+  int vx = 0;  // cells per thread in x (PCG), 0 = automatic
+  int sm = 0;  // cells per thread of the streaming kernels, 0 = automatic
 };
 
 struct System {
@@ -88,6 +90,22 @@ class Engine {
   // timing of the iteration kernels of the last run (device events), ms
   double last_run_ms() const { return last_run_ms_; }
   int last_run_iters() const { return last_run_iters_; }
+  // Per-kernel timing: events recorded between the kernels of every captured
+  // iteration; averages over the iterations that ran (not the no-op tail).
+  void set_profile(bool on) {
+    profile_ = on;
+    dirty_ = true;
+    prof_reset();
+  }
+  void prof_reset() {
+    for (int j = 0; j < 8; ++j) {
+      prof_ms_[j] = 0;
+      prof_n_[j] = 0;
+    }
+  }
+  int prof_kernels() const { return iter_kernels(method_); }
+  double prof_avg_ms(int j) const { return prof_n_[j] ? prof_ms_[j] / prof_n_[j] : 0.0; }
+  long long prof_count(int j) const { return prof_n_[j]; }
 
  private:
   int fail(int code, const std::string& msg);
@@ -135,6 +153,10 @@ class Engine {
   double* d_red_ = nullptr;  // partials for k_sumsq (1024) + result
   unsigned* d_red_counter_ = nullptr;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+  bool profile_ = false;
+  std::vector<hipEvent_t> pev_;
+  double prof_ms_[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  long long prof_n_[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   double last_run_ms_ = 0;
   int last_run_iters_ = 0;
   std::string err_;

@@ -83,6 +83,7 @@ Engine::~Engine() {
   if (d_red_counter_) (void)hipFree(d_red_counter_);
   if (ev0_) (void)hipEventDestroy(ev0_);
   if (ev1_) (void)hipEventDestroy(ev1_);
+  for (hipEvent_t e : pev_) (void)hipEventDestroy(e);
   if (st_) (void)hipStreamDestroy(st_);
 }
 
@@ -438,19 +439,19 @@ int Engine::prepare(int slot, const double* x0, bool on_device, long long start,
 
 int Engine::build_tables() {
   launch_slots_.clear();
-  long long ntot = 0;
   bool all4 = true, all2 = true;
   for (int i = 0; i < (int)slots_.size(); ++i) {
     System* s = slots_[i].get();
     if (s && s->active) {
       launch_slots_.push_back(i);
-      ntot += s->n;
       all4 = all4 && s->nx % 4 == 0;
       all2 = all2 && s->nx % 2 == 0;
     }
   }
   if (method_ == kPcg) {
     vx_ = fp32_ ? (all4 ? 4 : (all2 ? 2 : 1)) : (all2 ? 2 : 1);
+    const int tv = tuning_.vx;
+    if (tv == 1 || (tv == 2 && all2) || (tv == 4 && all4 && fp32_)) vx_ = tv;
   } else {
     vx_ = 1;
   }
@@ -458,10 +459,11 @@ int Engine::build_tables() {
   std::vector<Rec> rst, rsm, rru;
   std::vector<int> bst, bsm, bru;
   std::vector<SysDev> sds;
-  const long long target = 3072;
   for (int k = 0; k < ns; ++k) {
     System& s = *slots_[launch_slots_[k]];
-    const long long tgt = std::max(1LL, (long long)((double)target * (double)s.n / (double)ntot));
+    // The tiling of a system depends on that system only, so its reduction
+    // trees (and results) are the same alone and inside a batch.
+    const long long tgt = std::min(3072LL, std::max(1LL, cdiv(s.n, 2048)));
     Rec a = make_stencil_rec(s.nx, s.ny, s.nz, method_ == kPcg ? vx_ : 1, tgt, tuning_);
     a.sys = k;
     a.blk0 = (int)bst.size();
@@ -472,8 +474,8 @@ int Engine::build_tables() {
     Rec b{};
     b.sys = k;
     b.blk0 = (int)bsm.size();
-    const long long per = (long long)kBlock * (method_ == kPcg ? vx_ : 1) * 16;
-    b.nblk = (int)std::min(4096LL, std::max(1LL, cdiv(s.n, per)));
+    const int sm = tuning_.sm > 0 ? tuning_.sm : (method_ == kPcg ? 4 * vx_ : 4);
+    b.nblk = (int)std::min(16384LL, std::max(1LL, cdiv(s.n, (long long)kBlock * sm)));
     b.tiles_x = b.nblk;
     b.tiles_y = b.tiles_z = 1;
     b.lx = kBlock;
@@ -495,6 +497,9 @@ int Engine::build_tables() {
       s.partials_cap = need;
     }
     s.dev.partials = (double*)s.partials;
+    s.dev.nblk_st = a.nblk;
+    s.dev.nblk_sm = b.nblk;
+    s.dev.nblk_ru = c.nblk;
     sds.push_back(s.dev);
   }
   // device tables in one buffer each
@@ -536,6 +541,7 @@ int Engine::build_tables() {
   L_.nst = (int)bst.size();
   L_.nsm = (int)bsm.size();
   L_.nru = (int)bru.size();
+  L_.ns = ns;
   dirty_ = false;
   return APHPOIS_OK;
 }
@@ -559,14 +565,25 @@ int Engine::capture() {
     k = std::max(2, std::min(64, k));
   }
   if (method_ == kJacobi && (k & 1)) ++k;
+  const int nk = iter_kernels(method_);
+  if (profile_) {
+    const size_t ne = (size_t)k * nk + 1;
+    while (pev_.size() < ne) {
+      hipEvent_t e;
+      APH_HIP(hipEventCreate(&e));
+      pev_.push_back(e);
+    }
+  }
   APH_HIP(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
   for (int i = 0; i < k; ++i) {
-    hipError_t e = launch_iter(method_, fp32_, vx_, L_, st_);
-    if (e != hipSuccess) {
-      hipGraph_t g = nullptr;
-      (void)hipStreamEndCapture(st_, &g);
-      if (g) (void)hipGraphDestroy(g);
-      return hip(e, "launch_iter (capture)");
+    for (int j = 0; j < nk; ++j) {
+      const hipError_t e = launch_kernel(method_, fp32_, vx_, L_, st_, j);
+      if (e != hipSuccess) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(st_, &g);
+        if (g) (void)hipGraphDestroy(g);
+        return hip(e, "launch (capture)");
+      }
     }
   }
   APH_HIP(hipStreamEndCapture(st_, &graph_));
@@ -588,15 +605,48 @@ int Engine::run() {
   const int maxloops = maxiter_ / std::max(1, graph_k_) + 4;
   int loops = 0;
   bool alldone = false;
+  int iters_before = 0;
   while (!alldone) {
-    APH_HIP(hipGraphLaunch(gexec_, st_));
+    if (profile_) {
+      // same kernels as the graph, launched one by one with an event before
+      // each kernel (events cannot be timed inside a replayed graph)
+      const int nk = iter_kernels(method_);
+      for (int sl = 0; sl < graph_k_; ++sl) {
+        for (int j = 0; j < nk; ++j) {
+          APH_HIP(hipEventRecord(pev_[(size_t)sl * nk + j], st_));
+          APH_HIP(launch_kernel(method_, fp32_, vx_, L_, st_, j));
+        }
+      }
+      APH_HIP(hipEventRecord(pev_[(size_t)graph_k_ * nk], st_));
+    } else {
+      APH_HIP(hipGraphLaunch(gexec_, st_));
+    }
     ++loops;
     for (int i : launch_slots_) {
       APH_HIP(hipMemcpyAsync(h_sc_ + i, d_sc_ + i, sizeof(Scalars), hipMemcpyDeviceToHost, st_));
     }
     APH_HIP(hipStreamSynchronize(st_));
     alldone = true;
-    for (int i : launch_slots_) alldone = alldone && h_sc_[i].done;
+    int iters_after = 0;
+    for (int i : launch_slots_) {
+      alldone = alldone && h_sc_[i].done;
+      iters_after = std::max(iters_after, h_sc_[i].iter);
+    }
+    if (profile_) {
+      // slots that ran a full iteration for at least one system
+      const int nk = iter_kernels(method_);
+      const int ran = std::min(graph_k_, iters_after - iters_before);
+      for (int sl = 0; sl < ran; ++sl) {
+        for (int j = 0; j < nk; ++j) {
+          float ms = 0;
+          APH_HIP(hipEventElapsedTime(&ms, pev_[(size_t)sl * nk + j],
+                                      pev_[(size_t)sl * nk + j + 1]));
+          prof_ms_[j] += ms;
+          prof_n_[j] += 1;
+        }
+      }
+    }
+    iters_before = iters_after;
     if (loops > maxloops + 2 && !alldone) {
       // every system hits maxiter within maxloops replays; reaching here means
       // a flag was never set

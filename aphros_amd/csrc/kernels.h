@@ -15,8 +15,15 @@ struct LaunchSet {
   const Rec* rec_ru = nullptr;  // one-cell-per-thread stencil launches (FP64 residuals)
   const int* b2r_ru = nullptr;
   int nst = 0, nsm = 0, nru = 0;
+  int ns = 0;  // systems (one workgroup each in the finalize kernels)
 };
 
+// kernels of one iteration, each reducing kernel followed by its finalize
+// kernel: PCG: spmv, fin, update, fin, ru_accum, fin, ru_resid, fin;
+// CONJUGATE: spmv, fin, update, fin; JACOBI: sweep, fin
+int iter_kernels(int method);
+hipError_t launch_kernel(int method, int fp32, int vx, const LaunchSet& L, hipStream_t st,
+                         int j);
 hipError_t launch_iter(int method, int fp32, int vx, const LaunchSet& L, hipStream_t st);
 hipError_t launch_start(int method, int fp32, int vx, const LaunchSet& L, hipStream_t st);
 hipError_t launch_true_res_f7(const LaunchSet& L, hipStream_t st);

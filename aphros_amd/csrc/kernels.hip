@@ -2,10 +2,11 @@
 //
 // Launch model: one launch covers every active system of an engine; a
 // per-launch table maps each workgroup to (system, tile).  Scalars (alpha,
-// beta, convergence flags) live in device memory; each reduction finishes in
-// the last workgroup of its system (reduce.h), so a CG iteration needs no host
-// round trip and a sequence of iterations can be captured in a hipGraph.  A
-// converged system's workgroups return at entry.
+// beta, convergence flags) live in device memory; every reducing kernel writes
+// per-workgroup partials and is followed by a finalize kernel with one
+// workgroup per system (reduce.h), so a CG iteration needs no host round trip
+// and a sequence of iterations can be captured in a hipGraph.  A converged
+// system's workgroups return at entry.
 //
 // Stencil kernels march a (lx*VX) x (256/lx) tile of (x, y) columns through
 // bz planes of z, keeping the z-1, z, z+1 values of the column in registers.
@@ -26,6 +27,16 @@ __device__ __forceinline__ vec_t<double, VX> ldd(const T* __restrict__ p, long l
 template <typename T, int VX>
 __device__ __forceinline__ void std_(T* __restrict__ p, long long i, vec_t<double, VX> v) {
   *reinterpret_cast<vec_t<T, VX>*>(p + i) = __builtin_convertvector(v, vec_t<T, VX>);
+}
+
+template <typename T, int VX>
+__device__ __forceinline__ vec_t<T, VX> ldv(const T* __restrict__ p, long long i) {
+  return *reinterpret_cast<const vec_t<T, VX>*>(p + i);
+}
+
+template <int VX, typename V>
+__device__ __forceinline__ vec_t<double, VX> cvt(V v) {
+  return __builtin_convertvector(v, vec_t<double, VX>);
 }
 
 // round a double vector through T (the value that gets stored)
@@ -66,11 +77,21 @@ __device__ __forceinline__ Tile tile_of(const Rec& rc, const SysDev& s, int lb) 
 // p = r + beta * p_in is fused: the six neighbours' p are recomputed from r and
 // p_in, and the new p goes to the other buffer p_out.
 // ===========================================================================
+//
+// Each thread owns VX consecutive cells of one row of the tile and marches
+// through z.  u = S * p_new of the own cells is computed once per plane and
+// kept in registers for z-1, z, z+1; the plane's u of the whole tile is also
+// put in LDS (double-buffered by plane) so that x and y neighbours inside the
+// tile come from LDS.  Neighbours outside the tile (tile edges, periodic
+// wrap) are recomputed from global memory by the edge threads.  u is formed
+// in FP64 from the stored (rounded) p and S, so the differences u_c - u_nb are
+// exact and q carries the rounding of its final store only.
 template <typename T, int VX>
 __global__ __launch_bounds__(kBlock) void k_pcg_spmv(const SysDev* __restrict__ sds,
                                                      const Rec* __restrict__ recs,
                                                      const int* __restrict__ b2r) {
   using D = vec_t<double, VX>;
+  __shared__ double su[2][kBlock * VX];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   const Rec rc = recs[b2r[b]];
   const SysDev& s = sds[rc.sys];
@@ -91,81 +112,132 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv(const SysDev* __restrict__ 
   const T* __restrict__ S = (const T*)s.S;
   const T* __restrict__ mm = (const T*)s.m;
   const int has_m = s.has_m;
-  double acc[1] = {0.0};
+  const int lx = rc.lx;
+  const int tx = lx * VX;                 // tile width in cells (LDS row)
+  const int ix = threadIdx.x % lx, iy = threadIdx.x / lx;
+  const int ty = kBlock / lx;
+  const int lofs = iy * tx + ix * VX;     // own slot in the LDS plane
+  // which neighbours are inside the tile (and the grid): others from global
+  const bool xm_in = ix > 0;
+  const bool xp_in = ix + 1 < lx && t.x0 + VX < nx;
+  const bool ym_in = iy > 0;
+  const bool yp_in = iy + 1 < ty && t.y + 1 < ny;
+  const long long pl = (long long)nx * ny;
+  const long long row = (long long)t.y * nx;
+  const long long rowm = (long long)wrapm(t.y, ny) * nx;
+  const long long rowp = (long long)wrapp(t.y, ny) * nx;
+  const int xm = t.x0 == 0 ? nx - 1 : t.x0 - 1;
+  const int xp = t.x0 + VX == nx ? 0 : t.x0 + VX;
+  using V = vec_t<T, VX>;
+  // stored value of p_new = r + beta * p_in (rounded to T)
+  auto pnew_st = [&](long long i) -> V {
+    return __builtin_convertvector(ldd<T, VX>(r, i) + beta * ldd<T, VX>(pin, i), V);
+  };
+  auto unew = [&](long long i) -> D {
+    return ldd<T, VX>(S, i) * cvt<VX>(pnew_st(i));
+  };
+  auto unew1 = [&](long long i) -> double {
+    return (double)S[i] * (double)(T)((double)r[i] + beta * (double)pin[i]);
+  };
+  // registers: u of planes z-1, z, z+1 in FP64; p, S and kz kept as stored (T)
+  D um = 0.0, uc = 0.0;
+  V pc = (T)0, Sc = (T)0, kzm = (T)0;
   if (t.active) {
-    const long long pl = (long long)nx * ny;
-    const long long row = (long long)t.y * nx;
-    const long long rowm = (long long)wrapm(t.y, ny) * nx;
-    const long long rowp = (long long)wrapp(t.y, ny) * nx;
-    const int xm = t.x0 == 0 ? nx - 1 : t.x0 - 1;
-    const int xp = t.x0 + VX == nx ? 0 : t.x0 + VX;
-    auto pnew = [&](long long i) -> D {
-      return rnd<T, VX>(ldd<T, VX>(r, i) + beta * ldd<T, VX>(pin, i));
-    };
-    auto pnew1 = [&](long long i) -> double {
-      return (double)(T)((double)r[i] + beta * (double)pin[i]);
-    };
-    // prologue: plane za-1 and za
     const int zm0 = t.za == 0 ? nz - 1 : t.za - 1;
     long long i = (long long)zm0 * pl + row + t.x0;
-    D um = ldd<T, VX>(S, i) * pnew(i);
-    D kzm = ldd<T, VX>(kz, i);
+    um = unew(i);
+    kzm = ldv<T, VX>(kz, i);
     i = (long long)t.za * pl + row + t.x0;
-    D pc = pnew(i);
-    D Sc = ldd<T, VX>(S, i);
-    D uc = Sc * pc;
-    double dot = 0.0;
-    for (int z = t.za; z < t.zb; ++z) {
+    pc = pnew_st(i);
+    Sc = ldv<T, VX>(S, i);
+    uc = cvt<VX>(Sc) * cvt<VX>(pc);
+#pragma unroll
+    for (int v = 0; v < VX; ++v) su[0][lofs + v] = uc[v];
+  }
+  __syncthreads();
+  double dot = 0.0;
+  for (int z = t.za; z < t.zb; ++z) {
+    const int buf = (z - t.za) & 1;
+    if (t.active) {
       const long long base = (long long)z * pl;
       const long long ic = base + row + t.x0;
       const long long in = (long long)wrapp(z, nz) * pl + row + t.x0;
-      const D pn = pnew(in);
-      const D Sn = ldd<T, VX>(S, in);
-      const D un = Sn * pn;
-      const long long iym = base + rowm + t.x0, iyp = base + rowp + t.x0;
-      const D uym = ldd<T, VX>(S, iym) * pnew(iym);
-      const D uyp = ldd<T, VX>(S, iyp) * pnew(iyp);
-      const long long ixm = base + row + xm, ixp = base + row + xp;
-      const double uxm = (double)S[ixm] * pnew1(ixm);
-      const double uxp = (double)S[ixp] * pnew1(ixp);
-      const D kxc = ldd<T, VX>(kx, ic);
-      const double kxm = (double)kx[ixm];
-      const D kyc = ldd<T, VX>(ky, ic);
-      const D kym = ldd<T, VX>(ky, iym);
-      const D kzc = ldd<T, VX>(kz, ic);
-      D l = kyc * (uc - uyp) + kym * (uc - uym) + kzc * (uc - un) + kzm * (uc - um);
+      const V pn = pnew_st(in);
+      const V Sn = ldv<T, VX>(S, in);
+      const D un = cvt<VX>(Sn) * cvt<VX>(pn);
+      if (z + 1 < t.zb) {
 #pragma unroll
-      for (int v = 0; v < VX; ++v) {
-        const double up = v + 1 < VX ? uc[(v + 1) % VX] : uxp;
-        const double uw = v > 0 ? uc[(v + VX - 1) % VX] : uxm;
-        const double kw = v > 0 ? kxc[(v + VX - 1) % VX] : kxm;
-        l[v] += kxc[v] * (uc[v] - up) + kw * (uc[v] - uw);
+        for (int v = 0; v < VX; ++v) su[buf ^ 1][lofs + v] = un[v];
       }
-      if (has_m) l += ldd<T, VX>(mm, ic) * uc;
-      const D qv = rnd<T, VX>(Sc * l);
-      std_<T, VX>(pout, ic, pc);
-      std_<T, VX>(q, ic, qv);
+      const double* sp = su[buf];
+      // z and y couplings first (frees registers early)
+      D l = cvt<VX>(ldv<T, VX>(kz, ic)) * (uc - un) + cvt<VX>(kzm) * (uc - um);
+      kzm = ldv<T, VX>(kz, ic);
+      {
+        D uy;
+        if (ym_in) {
 #pragma unroll
-      for (int v = 0; v < VX; ++v) dot += pc[v] * qv[v];
+          for (int v = 0; v < VX; ++v) uy[v] = sp[lofs - tx + v];
+        } else {
+          uy = unew(base + rowm + t.x0);
+        }
+        l += cvt<VX>(ldv<T, VX>(ky, base + rowm + t.x0)) * (uc - uy);
+        if (yp_in) {
+#pragma unroll
+          for (int v = 0; v < VX; ++v) uy[v] = sp[lofs + tx + v];
+        } else {
+          uy = unew(base + rowp + t.x0);
+        }
+        l += cvt<VX>(ldv<T, VX>(ky, ic)) * (uc - uy);
+      }
+      {
+        const double uxm = xm_in ? sp[lofs - 1] : unew1(base + row + xm);
+        const double uxp = xp_in ? sp[lofs + VX] : unew1(base + row + xp);
+        const D kxc = cvt<VX>(ldv<T, VX>(kx, ic));
+        const double kxm = (double)kx[base + row + xm];
+#pragma unroll
+        for (int v = 0; v < VX; ++v) {
+          const double up = v + 1 < VX ? uc[(v + 1) % VX] : uxp;
+          const double uw = v > 0 ? uc[(v + VX - 1) % VX] : uxm;
+          const double kw = v > 0 ? kxc[(v + VX - 1) % VX] : kxm;
+          l[v] += kxc[v] * (uc[v] - up) + kw * (uc[v] - uw);
+        }
+      }
+      if (has_m) l += cvt<VX>(ldv<T, VX>(mm, ic)) * uc;
+      const V qv = __builtin_convertvector(cvt<VX>(Sc) * l, V);
+      *reinterpret_cast<V*>(pout + ic) = pc;
+      *reinterpret_cast<V*>(q + ic) = qv;
+      const D pcd = cvt<VX>(pc), qd = cvt<VX>(qv);
+#pragma unroll
+      for (int v = 0; v < VX; ++v) dot += pcd[v] * qd[v];
       um = uc;
       uc = un;
       pc = pn;
       Sc = Sn;
-      kzm = kzc;
     }
-    acc[0] = dot;
+    __syncthreads();
   }
-  if (grid_reduce<1, 0u>(acc, s.partials, rc.nblk, lb, s.counter)) {
-    if (threadIdx.x == 0) {
-      const double pq = acc[0];
-      if (!(pq != 0.0) || !isfinite(pq)) {
-        sc->status = 6;
-        sc->done = 1;
-      } else {
-        sc->alpha = sc->rho / pq;
-      }
-      sc->cur = cur ^ 1;
+  double acc[1] = {dot};
+  block_partial<1, 0u>(acc, s.partials, rc.nblk, lb);
+}
+
+// Finalize kernels: one workgroup per system, launched after the reducing
+// kernel of the same name.
+__global__ __launch_bounds__(kBlock) void k_fin_pcg_spmv(const SysDev* __restrict__ sds) {
+  const SysDev& s = sds[blockIdx.x];
+  Scalars* sc = s.sc;
+  if (sc->done) return;
+  double acc[1];
+  reduce_partials<1, 0u>(acc, s.partials, s.nblk_st);
+  if (threadIdx.x == 0) {
+    const double pq = acc[0];
+    if (!(pq != 0.0) || !isfinite(pq)) {
+      sc->status = 6;
+      sc->done = 1;
+    } else {
+      sc->alpha = sc->rho / pq;
     }
+    sc->cur ^= 1;
   }
 }
 
@@ -198,20 +270,27 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(const SysDev* __restrict_
 #pragma unroll
     for (int v = 0; v < VX; ++v) acc[0] += rv[v] * rv[v];
   }
-  if (grid_reduce<1, 0u>(acc, s.partials, rc.nblk, lb, s.counter)) {
-    if (threadIdx.x == 0) {
-      const double rn = acc[0];
-      sc->iter += 1;
-      const double est = sc->gamma * sqrt(rn);
-      const bool trig = !isfinite(rn) || sqrt(rn) < sc->delta * sqrt(sc->rmax2) ||
-                        est <= 0.9 * sc->tol || sc->iter >= sc->maxiter;
-      if (trig) {
-        sc->ru = 1;
-        sc->rho_pend = rn;
-      } else {
-        sc->beta = rn / sc->rho;
-        sc->rho = rn;
-      }
+  block_partial<1, 0u>(acc, s.partials, rc.nblk, lb);
+}
+
+__global__ __launch_bounds__(kBlock) void k_fin_pcg_update(const SysDev* __restrict__ sds) {
+  const SysDev& s = sds[blockIdx.x];
+  Scalars* sc = s.sc;
+  if (sc->done) return;
+  double acc[1];
+  reduce_partials<1, 0u>(acc, s.partials, s.nblk_sm);
+  if (threadIdx.x == 0) {
+    const double rn = acc[0];
+    sc->iter += 1;
+    const double est = sc->gamma * sqrt(rn);
+    const bool trig = !isfinite(rn) || sqrt(rn) < sc->delta * sqrt(sc->rmax2) ||
+                      est <= 0.9 * sc->tol || sc->iter >= sc->maxiter;
+    if (trig) {
+      sc->ru = 1;
+      sc->rho_pend = rn;
+    } else {
+      sc->beta = rn / sc->rho;
+      sc->rho = rn;
     }
   }
 }
@@ -242,10 +321,17 @@ __global__ __launch_bounds__(kBlock) void k_ru_accum(const SysDev* __restrict__ 
 #pragma unroll
     for (int v = 0; v < VX; ++v) acc[0] += xv[v];
   }
-  if (grid_reduce<1, 0u>(acc, s.partials, rc.nblk, lb, s.counter)) {
-    if (threadIdx.x == 0) {
-      sc->mean_pending = sc->singular ? acc[0] / (double)s.n : 0.0;
-    }
+  block_partial<1, 0u>(acc, s.partials, rc.nblk, lb);
+}
+
+__global__ __launch_bounds__(kBlock) void k_fin_ru_accum(const SysDev* __restrict__ sds) {
+  const SysDev& s = sds[blockIdx.x];
+  Scalars* sc = s.sc;
+  if (sc->done || !sc->ru) return;
+  double acc[1];
+  reduce_partials<1, 0u>(acc, s.partials, s.nblk_sm);
+  if (threadIdx.x == 0) {
+    sc->mean_pending = sc->singular ? acc[0] / (double)s.n : 0.0;
   }
 }
 
@@ -304,33 +390,40 @@ __global__ __launch_bounds__(kBlock) void k_ru_resid(const SysDev* __restrict__ 
       }
     }
   }
-  if (grid_reduce<3, 4u>(acc, s.partials, rc.nblk, lb, s.counter)) {
-    if (threadIdx.x == 0) {
-      const double rh2 = acc[0], rr = acc[1], mx = acc[2];
-      const double relres = sc->bnorm2 > 0 ? sqrt(rr / sc->bnorm2) : sqrt(rr);
-      const double l2 = sqrt(rr / sc->h3), lmax = mx / sc->h3;
-      const double metric = sc->resmode == 2 ? relres : (sc->resmode == 1 ? lmax : l2);
-      const bool conv = sc->resmode == 2 ? metric <= sc->tol : metric < sc->tol;
-      sc->relres = relres;
-      sc->res_l2 = l2;
-      sc->res_max = lmax;
-      sc->residual = metric;
-      sc->nru += 1;
-      if (!isfinite(rr) || !isfinite(rh2)) {
-        sc->status = 6;
-        sc->done = 1;
-      } else if ((conv && sc->iter >= sc->miniter) || sc->iter >= sc->maxiter) {
-        sc->converged = conv ? 1 : 0;
-        sc->done = 1;
-      } else {
-        sc->beta = sc->first ? 0.0 : rh2 / sc->rho;
-        sc->rho = rh2;
-        sc->rmax2 = rh2;
-        sc->gamma = rh2 > 0 ? metric / sqrt(rh2) : 1.0;
-        sc->first = 0;
-      }
-      sc->ru = 0;
+  block_partial<3, 4u>(acc, s.partials, rc.nblk, lb);
+}
+
+__global__ __launch_bounds__(kBlock) void k_fin_ru_resid(const SysDev* __restrict__ sds) {
+  const SysDev& s = sds[blockIdx.x];
+  Scalars* sc = s.sc;
+  if (sc->done || !sc->ru) return;
+  double acc[3];
+  reduce_partials<3, 4u>(acc, s.partials, s.nblk_ru);
+  if (threadIdx.x == 0) {
+    const double rh2 = acc[0], rr = acc[1], mx = acc[2];
+    const double relres = sc->bnorm2 > 0 ? sqrt(rr / sc->bnorm2) : sqrt(rr);
+    const double l2 = sqrt(rr / sc->h3), lmax = mx / sc->h3;
+    const double metric = sc->resmode == 2 ? relres : (sc->resmode == 1 ? lmax : l2);
+    const bool conv = sc->resmode == 2 ? metric <= sc->tol : metric < sc->tol;
+    sc->relres = relres;
+    sc->res_l2 = l2;
+    sc->res_max = lmax;
+    sc->residual = metric;
+    sc->nru += 1;
+    if (!isfinite(rr) || !isfinite(rh2)) {
+      sc->status = 6;
+      sc->done = 1;
+    } else if ((conv && sc->iter >= sc->miniter) || sc->iter >= sc->maxiter) {
+      sc->converged = conv ? 1 : 0;
+      sc->done = 1;
+    } else {
+      sc->beta = sc->first ? 0.0 : rh2 / sc->rho;
+      sc->rho = rh2;
+      sc->rmax2 = rh2;
+      sc->gamma = rh2 > 0 ? metric / sqrt(rh2) : 1.0;
+      sc->first = 0;
     }
+    sc->ru = 0;
   }
 }
 
@@ -377,12 +470,19 @@ __global__ __launch_bounds__(kBlock) void k_ref_init(const SysDev* __restrict__ 
       acc[0] += rv * rv;
     }
   }
-  if (grid_reduce<1, 0u>(acc, s.partials, rc.nblk, lb, s.counter)) {
-    if (threadIdx.x == 0) {
-      sc->rho = acc[0];
-      sc->cur = 0;
-      sc->first = 1;
-    }
+  block_partial<1, 0u>(acc, s.partials, rc.nblk, lb);
+}
+
+__global__ __launch_bounds__(kBlock) void k_fin_ref_init(const SysDev* __restrict__ sds) {
+  const SysDev& s = sds[blockIdx.x];
+  Scalars* sc = s.sc;
+  if (sc->done) return;
+  double acc[1];
+  reduce_partials<1, 0u>(acc, s.partials, s.nblk_st);
+  if (threadIdx.x == 0) {
+    sc->rho = acc[0];
+    sc->cur = 0;
+    sc->first = 1;
   }
 }
 
@@ -430,11 +530,18 @@ __global__ __launch_bounds__(kBlock) void k_ref_spmv(const SysDev* __restrict__ 
       acc[0] += pc * v;
     }
   }
-  if (grid_reduce<1, 0u>(acc, s.partials, rc.nblk, lb, s.counter)) {
-    if (threadIdx.x == 0) {
-      sc->alpha = sc->rho / (acc[0] + 1e-100);
-      sc->cur = cur ^ 1;
-    }
+  block_partial<1, 0u>(acc, s.partials, rc.nblk, lb);
+}
+
+__global__ __launch_bounds__(kBlock) void k_fin_ref_spmv(const SysDev* __restrict__ sds) {
+  const SysDev& s = sds[blockIdx.x];
+  Scalars* sc = s.sc;
+  if (sc->done) return;
+  double acc[1];
+  reduce_partials<1, 0u>(acc, s.partials, s.nblk_st);
+  if (threadIdx.x == 0) {
+    sc->alpha = sc->rho / (acc[0] + 1e-100);
+    sc->cur ^= 1;
   }
 }
 
@@ -463,19 +570,26 @@ __global__ __launch_bounds__(kBlock) void k_ref_update(const SysDev* __restrict_
     acc[0] += rv * rv;
     acc[1] = fmax(acc[1], fabs(rv));
   }
-  if (grid_reduce<2, 2u>(acc, s.partials, rc.nblk, lb, s.counter)) {
-    if (threadIdx.x == 0) {
-      const double dot_r = acc[0];
-      const double res = sc->resmode == 1 ? acc[1] / sc->h3 : sqrt(dot_r / sc->h3);
-      sc->residual = res;
-      sc->iter += 1;
-      sc->beta = dot_r / (sc->rho + 1e-100);
-      sc->rho = dot_r;
-      sc->first = 0;
-      if (sc->iter >= sc->miniter && (sc->iter > sc->maxiter || res < sc->tol)) {
-        sc->converged = res < sc->tol ? 1 : 0;
-        sc->done = 1;
-      }
+  block_partial<2, 2u>(acc, s.partials, rc.nblk, lb);
+}
+
+__global__ __launch_bounds__(kBlock) void k_fin_ref_update(const SysDev* __restrict__ sds) {
+  const SysDev& s = sds[blockIdx.x];
+  Scalars* sc = s.sc;
+  if (sc->done) return;
+  double acc[2];
+  reduce_partials<2, 2u>(acc, s.partials, s.nblk_sm);
+  if (threadIdx.x == 0) {
+    const double dot_r = acc[0];
+    const double res = sc->resmode == 1 ? acc[1] / sc->h3 : sqrt(dot_r / sc->h3);
+    sc->residual = res;
+    sc->iter += 1;
+    sc->beta = dot_r / (sc->rho + 1e-100);
+    sc->rho = dot_r;
+    sc->first = 0;
+    if (sc->iter >= sc->miniter && (sc->iter > sc->maxiter || res < sc->tol)) {
+      sc->converged = res < sc->tol ? 1 : 0;
+      sc->done = 1;
     }
   }
 }
@@ -518,16 +632,23 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(const SysDev* __restrict__ sd
       acc[0] = fmax(acc[0], fabs(v - u[c]));
     }
   }
-  if (grid_reduce<1, 1u>(acc, s.partials, rc.nblk, lb, s.counter)) {
-    if (threadIdx.x == 0) {
-      const double res = acc[0];
-      sc->residual = res;
-      sc->iter += 1;
-      sc->cur = cur ^ 1;
-      if (sc->iter >= sc->miniter && (sc->iter > sc->maxiter || res < sc->tol)) {
-        sc->converged = res < sc->tol ? 1 : 0;
-        sc->done = 1;
-      }
+  block_partial<1, 1u>(acc, s.partials, rc.nblk, lb);
+}
+
+__global__ __launch_bounds__(kBlock) void k_fin_jacobi(const SysDev* __restrict__ sds) {
+  const SysDev& s = sds[blockIdx.x];
+  Scalars* sc = s.sc;
+  if (sc->done) return;
+  double acc[1];
+  reduce_partials<1, 1u>(acc, s.partials, s.nblk_st);
+  if (threadIdx.x == 0) {
+    const double res = acc[0];
+    sc->residual = res;
+    sc->iter += 1;
+    sc->cur ^= 1;
+    if (sc->iter >= sc->miniter && (sc->iter > sc->maxiter || res < sc->tol)) {
+      sc->converged = res < sc->tol ? 1 : 0;
+      sc->done = 1;
     }
   }
 }
@@ -568,12 +689,18 @@ __global__ __launch_bounds__(kBlock) void k_true_res_f7(const SysDev* __restrict
       acc[2] = fmax(acc[2], fabs(v));
     }
   }
-  if (grid_reduce<3, 4u>(acc, s.partials, rc.nblk, lb, s.counter)) {
-    if (threadIdx.x == 0) {
-      sc->relres = acc[1] > 0 ? sqrt(acc[0] / acc[1]) : sqrt(acc[0]);
-      sc->res_l2 = sqrt(acc[0] / sc->h3);
-      sc->res_max = acc[2] / sc->h3;
-    }
+  block_partial<3, 4u>(acc, s.partials, rc.nblk, lb);
+}
+
+__global__ __launch_bounds__(kBlock) void k_fin_true_res_f7(const SysDev* __restrict__ sds) {
+  const SysDev& s = sds[blockIdx.x];
+  Scalars* sc = s.sc;
+  double acc[3];
+  reduce_partials<3, 4u>(acc, s.partials, s.nblk_ru);
+  if (threadIdx.x == 0) {
+    sc->relres = acc[1] > 0 ? sqrt(acc[0] / acc[1]) : sqrt(acc[0]);
+    sc->res_l2 = sqrt(acc[0] / sc->h3);
+    sc->res_max = acc[2] / sc->h3;
   }
 }
 
@@ -725,9 +852,15 @@ __global__ __launch_bounds__(kBlock) void k_sumsq(const double* __restrict__ v, 
        i += (long long)gridDim.x * kBlock) {
     acc[0] += v[i] * v[i];
   }
-  if (grid_reduce<1, 0u>(acc, partials, gridDim.x, blockIdx.x, counter)) {
-    if (threadIdx.x == 0) *out = acc[0];
-  }
+  (void)counter;
+  block_partial<1, 0u>(acc, partials, gridDim.x, blockIdx.x);
+}
+
+__global__ __launch_bounds__(kBlock) void k_fin_sumsq(const double* __restrict__ partials,
+                                                      int nblk, double* __restrict__ out) {
+  double acc[1];
+  reduce_partials<1, 0u>(acc, partials, nblk);
+  if (threadIdx.x == 0) *out = acc[0];
 }
 
 template <typename T>
@@ -745,57 +878,70 @@ __global__ __launch_bounds__(kBlock) void k_fill(T* __restrict__ p, long long n,
 #define APH_LAUNCH(kern, grid, stream, ...) \
   hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, stream, __VA_ARGS__)
 
-hipError_t launch_iter(int method, int fp32, int vx, const LaunchSet& L, hipStream_t st) {
+int iter_kernels(int method) { return method == kConjugate ? 4 : (method == kJacobi ? 2 : 8); }
+
+template <typename T, int VX>
+static void launch_pcg_kernel(const LaunchSet& L, hipStream_t st, int j) {
+  switch (j) {
+    case 0: APH_LAUNCH((k_pcg_spmv<T, VX>), L.nst, st, L.sds, L.rec_st, L.b2r_st); break;
+    case 1: APH_LAUNCH(k_fin_pcg_spmv, L.ns, st, L.sds); break;
+    case 2: APH_LAUNCH((k_pcg_update<T, VX>), L.nsm, st, L.sds, L.rec_sm, L.b2r_sm); break;
+    case 3: APH_LAUNCH(k_fin_pcg_update, L.ns, st, L.sds); break;
+    case 4: APH_LAUNCH((k_ru_accum<T, VX>), L.nsm, st, L.sds, L.rec_sm, L.b2r_sm); break;
+    case 5: APH_LAUNCH(k_fin_ru_accum, L.ns, st, L.sds); break;
+    case 6: APH_LAUNCH((k_ru_resid<T, 1>), L.nru, st, L.sds, L.rec_ru, L.b2r_ru); break;
+    default: APH_LAUNCH(k_fin_ru_resid, L.ns, st, L.sds); break;
+  }
+}
+
+hipError_t launch_kernel(int method, int fp32, int vx, const LaunchSet& L, hipStream_t st,
+                         int j) {
   if (method == kConjugate) {
-    APH_LAUNCH(k_ref_spmv, L.nst, st, L.sds, L.rec_st, L.b2r_st);
-    APH_LAUNCH(k_ref_update, L.nsm, st, L.sds, L.rec_sm, L.b2r_sm);
-  } else if (method == kJacobi) {
-    APH_LAUNCH(k_jacobi, L.nst, st, L.sds, L.rec_st, L.b2r_st);
-  } else {
-#define APH_PCG(T, VX)                                                  \
-  do {                                                                  \
-    APH_LAUNCH((k_pcg_spmv<T, VX>), L.nst, st, L.sds, L.rec_st, L.b2r_st);   \
-    APH_LAUNCH((k_pcg_update<T, VX>), L.nsm, st, L.sds, L.rec_sm, L.b2r_sm); \
-    APH_LAUNCH((k_ru_accum<T, VX>), L.nsm, st, L.sds, L.rec_sm, L.b2r_sm);   \
-    APH_LAUNCH((k_ru_resid<T, 1>), L.nru, st, L.sds, L.rec_ru, L.b2r_ru);    \
-  } while (0)
-    if (fp32) {
-      if (vx == 4) APH_PCG(float, 4);
-      else if (vx == 2) APH_PCG(float, 2);
-      else APH_PCG(float, 1);
-    } else {
-      if (vx == 2) APH_PCG(double, 2);
-      else APH_PCG(double, 1);
+    switch (j) {
+      case 0: APH_LAUNCH(k_ref_spmv, L.nst, st, L.sds, L.rec_st, L.b2r_st); break;
+      case 1: APH_LAUNCH(k_fin_ref_spmv, L.ns, st, L.sds); break;
+      case 2: APH_LAUNCH(k_ref_update, L.nsm, st, L.sds, L.rec_sm, L.b2r_sm); break;
+      default: APH_LAUNCH(k_fin_ref_update, L.ns, st, L.sds); break;
     }
-#undef APH_PCG
+  } else if (method == kJacobi) {
+    if (j == 0) APH_LAUNCH(k_jacobi, L.nst, st, L.sds, L.rec_st, L.b2r_st);
+    else APH_LAUNCH(k_fin_jacobi, L.ns, st, L.sds);
+  } else if (fp32) {
+    if (vx == 4) launch_pcg_kernel<float, 4>(L, st, j);
+    else if (vx == 2) launch_pcg_kernel<float, 2>(L, st, j);
+    else launch_pcg_kernel<float, 1>(L, st, j);
+  } else {
+    if (vx == 2) launch_pcg_kernel<double, 2>(L, st, j);
+    else launch_pcg_kernel<double, 1>(L, st, j);
   }
   return hipGetLastError();
+}
+
+hipError_t launch_iter(int method, int fp32, int vx, const LaunchSet& L, hipStream_t st) {
+  for (int j = 0; j < iter_kernels(method); ++j) {
+    const hipError_t e = launch_kernel(method, fp32, vx, L, st, j);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 hipError_t launch_start(int method, int fp32, int vx, const LaunchSet& L, hipStream_t st) {
   if (method == kConjugate) {
     APH_LAUNCH(k_ref_init, L.nst, st, L.sds, L.rec_st, L.b2r_st);
+    APH_LAUNCH(k_fin_ref_init, L.ns, st, L.sds);
   } else if (method == kPcg) {
-#define APH_RU(T, VX)                                                  \
-  do {                                                                 \
-    APH_LAUNCH((k_ru_accum<T, VX>), L.nsm, st, L.sds, L.rec_sm, L.b2r_sm); \
-    APH_LAUNCH((k_ru_resid<T, 1>), L.nru, st, L.sds, L.rec_ru, L.b2r_ru);  \
-  } while (0)
-    if (fp32) {
-      if (vx == 4) APH_RU(float, 4);
-      else if (vx == 2) APH_RU(float, 2);
-      else APH_RU(float, 1);
-    } else {
-      if (vx == 2) APH_RU(double, 2);
-      else APH_RU(double, 1);
+    // the start is a reliable update (ru = 1): x64 = x0, r^ = S (b - A x0)
+    for (int j = 4; j < 8; ++j) {
+      const hipError_t e = launch_kernel(method, fp32, vx, L, st, j);
+      if (e != hipSuccess) return e;
     }
-#undef APH_RU
   }
   return hipGetLastError();
 }
 
 hipError_t launch_true_res_f7(const LaunchSet& L, hipStream_t st) {
   APH_LAUNCH(k_true_res_f7, L.nru, st, L.sds, L.rec_ru, L.b2r_ru);
+  APH_LAUNCH(k_fin_true_res_f7, L.ns, st, L.sds);
   return hipGetLastError();
 }
 
@@ -862,6 +1008,7 @@ hipError_t launch_sumsq(const double* v, long long n, double* partials, unsigned
   int g = grid_for(n);
   if (g > 1024) g = 1024;
   APH_LAUNCH(k_sumsq, g, st, v, n, partials, counter, out);
+  APH_LAUNCH(k_fin_sumsq, 1, st, partials, g, out);
   return hipGetLastError();
 }
 

@@ -1,0 +1,97 @@
+"""Dense-array front end of the C ABI: one resident system per handle."""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+
+class Poisson:
+    """aphpois handle for a dense (nz, ny, nx) periodic system.
+
+    Poisson(shape, method=PCG, precision=MIXED, resmode=RES_REL_TRUE, tol=1e-8,
+            maxiter=..., h3=...) ; set_system(expr) ; solve(x0) -> (x, info)
+    """
+
+    def __init__(self, shape, method=L.PCG, precision=L.FP64, resmode=L.RES_REL_TRUE,
+                 tol=1e-8, miniter=0, maxiter=100000, h3=None, nullspace=L.NULL_AUTO,
+                 delta=0.0, check_every=0, device=0, mem_budget=0):
+        nz, ny, nx = shape
+        self.shape = (nz, ny, nx)
+        d = L.Desc()
+        L.lib().aphpois_desc_default(C.byref(d))
+        d.nx, d.ny, d.nz = nx, ny, nz
+        d.h3 = h3 if h3 is not None else 1.0 / (nx * ny * nz)
+        d.method, d.precision, d.resmode = method, precision, resmode
+        d.tol, d.miniter, d.maxiter = tol, miniter, maxiter
+        d.nullspace, d.delta, d.check_every = nullspace, delta, check_every
+        d.device, d.mem_budget = device, mem_budget
+        self.desc = d
+        self.h = C.c_void_p()
+        L.check(L.lib().aphpois_create(C.byref(d), C.byref(self.h)))
+
+    def set_system(self, expr):
+        e = np.ascontiguousarray(expr, np.float64).reshape(-1)
+        nz, ny, nx = self.shape
+        if e.size != 8 * nx * ny * nz:
+            raise ValueError("expr must hold 8 doubles per cell")
+        L.check(L.lib().aphpois_set_system(self.h, L.dptr(e), 0, nx, nx * ny))
+
+    def set_rhs(self, e7):
+        e = np.ascontiguousarray(e7, np.float64).reshape(-1)
+        nz, ny, nx = self.shape
+        L.check(L.lib().aphpois_set_rhs(self.h, L.dptr(e), 0, nx, nx * ny))
+
+    def set_conf(self, tol, miniter, maxiter):
+        L.check(L.lib().aphpois_set_conf(self.h, tol, miniter, maxiter))
+
+    def solve(self, x0=None, out=None):
+        nz, ny, nx = self.shape
+        x = out if out is not None else np.empty(self.shape)
+        x0p = None
+        if x0 is not None:
+            x0p = x0 if x0 is x else np.ascontiguousarray(x0, np.float64)
+        info = L.Info()
+        L.check(L.lib().aphpois_solve(self.h, L.dptr(x0p), L.dptr(x), 0, nx, nx * ny,
+                                      C.byref(info)))
+        return x, info.as_dict()
+
+    def solve_device(self, x_dev_ptr, x0_dev_ptr=None):
+        info = L.Info()
+        L.check(L.lib().aphpois_solve_device(self.h, x0_dev_ptr, x_dev_ptr, C.byref(info)))
+        return info.as_dict()
+
+    def set_system_device(self, expr_dev_ptr):
+        L.check(L.lib().aphpois_set_system_device(self.h, expr_dev_ptr))
+
+    def set_profiling(self, on=True):
+        L.check(L.lib().aphpois_set_profiling(self.h, int(on)))
+
+    def kernel_times(self):
+        avg = (C.c_double * 8)()
+        cnt = (C.c_longlong * 8)()
+        nk = C.c_int()
+        L.check(L.lib().aphpois_kernel_times(self.h, avg, cnt, C.byref(nk)))
+        return [(avg[j], cnt[j]) for j in range(nk.value)]
+
+    def launch_info(self):
+        a, b, v = C.c_int(), C.c_int(), C.c_int()
+        L.check(L.lib().aphpois_launch_info(self.h, C.byref(a), C.byref(b), C.byref(v)))
+        return {"stencil_blocks": a.value, "stream_blocks": b.value, "vx": v.value}
+
+    def set_tile(self, lx=0, bz=0, vx=0, sm=0):
+        L.check(L.lib().aphpois_set_tile(self.h, lx, bz, vx, sm))
+
+    def device_bytes(self):
+        return L.lib().aphpois_device_bytes(self.h)
+
+    def close(self):
+        if self.h:
+            L.lib().aphpois_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

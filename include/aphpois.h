@@ -145,6 +145,29 @@ int aphpois_destroy(aphpois* h);
 /* Message of the last failing call on this thread ("" if none). */
 const char* aphpois_last_error(void);
 
+/* ---------------- measurement hooks (no reference counterpart) ----------- */
+
+/* Profiling mode: the iteration kernels are launched one by one on the
+ * handle's stream with a HIP event before each (instead of replaying the
+ * captured graph), so that per-kernel device time can be read back.  Off by
+ * default. */
+int aphpois_set_profiling(aphpois* h, int on);
+/* Average device time (ms) and count of each iteration kernel over the
+ * iterations that ran since profiling was switched on.  Kernel order (each
+ * reducing kernel is followed by its finalize kernel): PCG {spmv, fin,
+ * update, fin, ru_accum, fin, ru_resid, fin}; CONJUGATE {spmv, fin, update,
+ * fin}; JACOBI {sweep, fin}.  avg_ms and count hold 8 entries; *nkernels
+ * receives the number used. */
+int aphpois_kernel_times(aphpois* h, double* avg_ms, long long* count, int* nkernels);
+/* Launch geometry of the last solve: workgroups of the stencil and streaming
+ * launches and the x vector width. */
+int aphpois_launch_info(aphpois* h, int* stencil_blocks, int* stream_blocks, int* vx);
+/* Overrides the launch shape (0 = automatic for each): lx threads per tile
+ * row (power of two <= 256), bz planes per stencil workgroup, vx cells per
+ * thread in x (1, 2 or 4; 4 only in MIXED; must divide nx), sm cells per
+ * thread of the streaming kernels. */
+int aphpois_set_tile(aphpois* h, int lx, int bz, int vx, int sm);
+
 /* ---------------- batched solves (many independent systems) -------------- */
 
 typedef struct aphpois_batch aphpois_batch;
@@ -178,6 +201,8 @@ int aphpois_batch_solve(aphpois_batch* b);
 int aphpois_batch_get(aphpois_batch* b, int slot, double* x, aphpois_info* info);
 int aphpois_batch_remove(aphpois_batch* b, int slot);
 long long aphpois_batch_device_bytes(const aphpois_batch* b);
+/* Overrides the launch shape for the batch (see aphpois_set_tile). */
+int aphpois_batch_set_tile(aphpois_batch* b, int lx, int bz, int vx, int sm);
 int aphpois_batch_destroy(aphpois_batch* b);
 
 /* ------------------------- synthetic systems (host) ---------------------- */

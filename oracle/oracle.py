@@ -240,7 +240,11 @@ def tlinear_system(n=32):
     E[..., 4] = np.roll(ky, -1, axis=1)
     E[..., 5] = kz
     E[..., 6] = np.roll(kz, -1, axis=0)
-    E[..., 0] = -(E[..., 1] + E[..., 2] + E[..., 3] + E[..., 4] + E[..., 5] + E[..., 6])
+    # AppendExpr accumulates the diagonal over q = 0..5 (mesh.h:575-579)
+    e0 = np.zeros((n, n, n))
+    for q in range(6):
+        e0 = e0 - E[..., 1 + q]
+    E[..., 0] = e0
     E[..., 7] = -apply(E, ue)
     return E, ue, h
 

@@ -1,0 +1,140 @@
+"""GPU parity of the HIP path against the CPU oracle (which restates the
+reference's SolverConjugate / SolverJacobi bit for bit; see test_oracle.py).
+
+Tolerances: CONJUGATE and JACOBI keep the reference's per-cell operation order
+without FMA, so Jacobi (max-reductions only) is bit-exact and CG differs only
+through the order of the dot-product sums (iteration counts equal, solutions
+within 1e-9 absolute).  PCG is checked on the FP64 true relative residual
+(<= 1e-8, recomputed by the oracle) and against an FP64 oracle solution.
+"""
+import numpy as np
+import pytest
+
+from aphros_amd import _lib as L
+from aphros_amd import synth
+from aphros_amd.solver import Poisson
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def tlin32():
+    return O.tlinear_system(32)
+
+
+@pytest.fixture(scope="module")
+def c1():
+    e, rho, f = synth.config("c1_64")
+    return e.reshape(64, 64, 64, 8)
+
+
+def test_conjugate_tlinear_matches_oracle(tlin32):
+    E, ue, h = tlin32
+    _say("conjugate t.linear 32^3")
+    xo, ito, reso, _ = O.conjugate(E, h ** 3, tol=1e-5, miniter=0, maxiter=1000)
+    s = Poisson(E.shape[:3], method=L.CONJUGATE, precision=L.FP64, resmode=L.RES_L2,
+                tol=1e-5, miniter=0, maxiter=1000, h3=h ** 3)
+    s.set_system(E)
+    x, info = s.solve(None)
+    assert info["iter"] == ito
+    assert abs(info["residual"] - reso) <= 1e-6 * reso
+    assert np.abs(x - xo).max() < 1e-9
+    # the reference test's own rule (src/test/linear/test:29)
+    assert O.max_diff_mean_removed(x, ue) < 2 * 6.926950e-03 + 1e-12
+
+
+def test_jacobi_tlinear_bitwise(tlin32):
+    E, ue, h = tlin32
+    _say("jacobi t.linear 32^3")
+    xo, ito, reso, _ = O.jacobi(E, tol=1e-5, miniter=0, maxiter=1000)
+    s = Poisson(E.shape[:3], method=L.JACOBI, precision=L.FP64, resmode=L.RES_L2,
+                tol=1e-5, miniter=0, maxiter=1000, h3=h ** 3)
+    s.set_system(E)
+    x, info = s.solve(None)
+    assert info["iter"] == ito
+    assert info["residual"] == reso
+    assert np.array_equal(x, xo)
+    assert O.max_diff_mean_removed(x, ue) < 2 * 8.284385e-03 + 1e-12
+
+
+def test_conjugate_sphere_history(c1):
+    """Fixed iteration counts (tol 0) on the 64^3 sphere system: every capped
+    run ends where the oracle's does."""
+    h3 = (1 / 64) ** 3
+    for maxiter in (0, 1, 7, 50):
+        _say(f"conjugate c1 maxiter={maxiter}")
+        xo, ito, reso, _ = O.conjugate(c1, h3, tol=0, miniter=0, maxiter=maxiter)
+        s = Poisson(c1.shape[:3], method=L.CONJUGATE, resmode=L.RES_L2, tol=0, miniter=0,
+                    maxiter=maxiter, h3=h3)
+        s.set_system(c1)
+        x, info = s.solve(None)
+        assert info["iter"] == ito == maxiter + 1
+        assert abs(info["residual"] - reso) <= 1e-9 * reso
+        assert np.abs(x - xo).max() <= 1e-10 * max(1.0, np.abs(xo).max())
+
+
+@pytest.mark.parametrize("precision", [L.FP64, L.MIXED])
+def test_pcg_c1_true_residual(c1, precision):
+    _say(f"pcg c1 precision={precision}")
+    s = Poisson(c1.shape[:3], method=L.PCG, precision=precision, resmode=L.RES_REL_TRUE,
+                tol=1e-8, maxiter=20000)
+    s.set_system(c1)
+    x, info = s.solve(None)
+    assert info["converged"] == 1 and info["status"] == 0
+    assert info["true_relres"] <= 1e-8
+    # independent FP64 recomputation on the CPU
+    assert O.true_relres(c1, x) <= 1.0e-8 * (1 + 1e-6)
+    assert abs(x.mean()) < 1e-12
+    xo, ito, rro = O.pcg_jacobi(c1, tol=1e-11)
+    err = np.abs(x - xo).max() / np.abs(xo).max()
+    _say(f"  iter={info['iter']} oracle_iter={ito} rel_err={err:.3e} ru={info['refinements']}")
+    assert err < 1e-5
+
+
+def test_pcg_tlinear_negative_diagonal(tlin32):
+    """e0 < 0 convention (t.linear / SolvePoisson)."""
+    E, ue, h = tlin32
+    s = Poisson(E.shape[:3], method=L.PCG, precision=L.MIXED, resmode=L.RES_REL_TRUE,
+                tol=1e-10, maxiter=5000, h3=h ** 3)
+    s.set_system(E)
+    x, info = s.solve(None)
+    assert info["converged"] == 1
+    assert O.true_relres(E, x) <= 1e-10 * (1 + 1e-6)
+    assert O.max_diff_mean_removed(x, ue) < 1e-7
+
+
+def test_warm_start_alias_and_setconf(c1):
+    s = Poisson(c1.shape[:3], method=L.PCG, precision=L.MIXED, resmode=L.RES_REL_TRUE,
+                tol=1e-6, maxiter=20000)
+    s.set_system(c1)
+    x, info1 = s.solve(None)
+    s.set_conf(1e-8, 0, 20000)
+    x2, info2 = s.solve(x, out=x)  # init aliases sol
+    assert x2 is x
+    assert info2["converged"] == 1 and info2["true_relres"] <= 1e-8
+    assert info2["iter"] < info1["iter"] * 2
+
+
+def test_errors(c1):
+    s = Poisson(c1.shape[:3], method=L.PCG)
+    with pytest.raises(L.AphpoisError) as ei:
+        s.solve(None)
+    assert ei.value.code == L.ESTATE
+    bad = c1.copy()
+    bad[3, 4, 5, 2] *= 1.5  # breaks symmetry
+    with pytest.raises(L.AphpoisError) as ei:
+        s.set_system(bad)
+    assert ei.value.code == L.EINVAL
+    bad = c1.copy()
+    bad[1, 2, 3, 7] = np.nan
+    with pytest.raises(L.AphpoisError) as ei:
+        s.set_system(bad)
+    assert ei.value.code == L.ENUMERIC
+    with pytest.raises(L.AphpoisError) as ei:
+        Poisson(c1.shape[:3], method=L.PCG, mem_budget=1 << 20)
+    assert ei.value.code == L.EBUDGET
