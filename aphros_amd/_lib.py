@@ -46,6 +46,34 @@ class BatchDesc(C.Structure):
                 ("device", C.c_int), ("mem_budget", C.c_longlong), ("max_systems", C.c_int)]
 
 
+class ServerDesc(C.Structure):
+    _fields_ = [("precision", C.c_int), ("tol", C.c_double), ("maxiter", C.c_int),
+                ("delta", C.c_double), ("check_every", C.c_int), ("device", C.c_int),
+                ("mem_budget", C.c_longlong), ("max_active", C.c_int)]
+
+
+class Request(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("h3", C.c_double),
+                ("expr", C.POINTER(C.c_double)), ("nsteps", C.c_int),
+                ("e7", C.POINTER(C.POINTER(C.c_double))), ("perturb", C.c_double),
+                ("seed", C.c_ulonglong), ("x_out", C.POINTER(C.c_double))]
+
+
+class Result(C.Structure):
+    _fields_ = [("id", C.c_longlong), ("status", C.c_int), ("steps_done", C.c_int),
+                ("iters_total", C.c_longlong), ("refinements_total", C.c_int),
+                ("last_relres", C.c_double), ("t_submit", C.c_double),
+                ("t_admit", C.c_double), ("t_done", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("queued", C.c_int), ("active", C.c_int), ("completed", C.c_longlong),
+                ("device_bytes", C.c_longlong), ("time", C.c_double)]
+
+
 class AphpoisError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"{ERROR_NAMES.get(code, code)}: {msg}")
@@ -75,6 +103,15 @@ SIGNATURES = {
                                       C.POINTER(C.c_int)]),
     "aphpois_set_tile": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int]),
     "aphpois_batch_set_tile": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "aphpois_batch_set_system_tile": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    "aphpois_batch_last_seconds": (C.c_double, [_vp]),
+    "aphpois_server_create": (C.c_int, [C.POINTER(ServerDesc), C.POINTER(_vp)]),
+    "aphpois_server_submit": (C.c_int, [_vp, C.POINTER(Request), C.POINTER(C.c_longlong)]),
+    "aphpois_server_wait": (C.c_int, [_vp, C.c_longlong, C.c_double, C.POINTER(Result)]),
+    "aphpois_server_cancel": (C.c_int, [_vp, C.c_longlong]),
+    "aphpois_server_get_stats": (C.c_int, [_vp, C.POINTER(ServerStats)]),
+    "aphpois_server_destroy": (C.c_int, [_vp]),
+    "aphpois_server_last_error": (C.c_char_p, []),
     "aphpois_batch_create": (C.c_int, [C.POINTER(BatchDesc), C.POINTER(_vp)]),
     "aphpois_batch_add": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_double, _dp, _dp,
                                     C.POINTER(C.c_int)]),

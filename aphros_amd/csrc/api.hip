@@ -242,6 +242,17 @@ int aphpois_batch_remove(aphpois_batch* b, int slot) {
 
 long long aphpois_batch_device_bytes(const aphpois_batch* b) { return b ? b->eng.bytes() : 0; }
 
+int aphpois_batch_set_system_tile(aphpois_batch* b, int slot, int lx, int bz) {
+  APH_CHECK_H(b);
+  if (lx < 0 || lx > 256 || (lx & (lx - 1)) || bz < 0)
+    return set_err(APHPOIS_EINVAL, "bad tile (lx a power of two <= 256)");
+  APH_RET(*b, b->eng.set_system_tile(slot, lx, bz));
+}
+
+double aphpois_batch_last_seconds(const aphpois_batch* b) {
+  return b ? b->eng.last_run_ms() * 1e-3 : 0.0;
+}
+
 int aphpois_batch_set_tile(aphpois_batch* b, int lx, int bz, int vx, int sm) {
   APH_CHECK_H(b);
   if (lx < 0 || lx > 256 || (lx & (lx - 1)) || bz < 0 || vx < 0 || vx > 4 || sm < 0)

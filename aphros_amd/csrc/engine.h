@@ -36,6 +36,8 @@ struct System {
   bool has_m = false;
   bool active = false;  // takes part in the next solve
   bool have_x = false;  // x64/u holds a previous solution (warm start)
+  int tune_lx = 0, tune_bz = 0;  // stencil tile of this system (0 = engine default)
+  bool done_reported = true;     // the last solve's result has been collected
   std::vector<void*> allocs;
   long long bytes = 0;
   void* partials = nullptr;  // reduction scratch, outside the budget
@@ -75,6 +77,23 @@ class Engine {
     tuning_ = t;
     dirty_ = true;
   }
+  int set_system_tile(int slot, int lx, int bz) {
+    System* s = sys(slot);
+    if (!s) return fail(APHPOIS_EINVAL, "no such slot");
+    s->tune_lx = lx;
+    s->tune_bz = bz;
+    dirty_ = true;
+    return APHPOIS_OK;
+  }
+  // Continuous batching: runs at most `max_replays` replays of the captured
+  // iteration graph over the active systems and returns the slots that
+  // finished (converged, hit maxiter or broke down) during this call; they
+  // stay in the launch tables, idle, until prepared again or removed.
+  int step(int max_replays, std::vector<int>* finished);
+  // Next right-hand side of a correlated sequence, on the device (PCG):
+  // b += amp * max|b| * (eta - mean eta), eta uniform in [-1, 1] from the
+  // splitmix64 stream `key`.
+  int perturb_rhs(int slot, double amp, unsigned long long key);
   long long bytes() const;
   int nslots() const { return (int)slots_.size(); }
   System* sys(int slot) {
@@ -113,6 +132,9 @@ class Engine {
   int alloc(System& s, void** p, long long bytes);
   int build_tables();
   int capture();
+  int pull_scalars();
+  void fill_info(int slot, double seconds);
+  bool need_start_ = false;
   int upload_strided(const double* src, bool on_device, long long start, long long ly,
                      long long lz, int nx, int ny, int nz, int comps, const double** dptr);
   void free_system(System& s);
@@ -150,7 +172,7 @@ class Engine {
   void* staging_ = nullptr;
   long long staging_bytes_ = 0;
   unsigned* d_flags_ = nullptr;
-  double* d_red_ = nullptr;  // partials for k_sumsq (1024) + result
+  double* d_red_ = nullptr;  // reduction scratch (2056 doubles)
   unsigned* d_red_counter_ = nullptr;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
   bool profile_ = false;

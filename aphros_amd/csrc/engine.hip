@@ -123,7 +123,9 @@ int Engine::init(int device, int method, int precision, int resmode, double tol,
   APH_HIP(hipMemset(d_counter_, 0, sizeof(unsigned) * capacity_));
   APH_HIP(hipMemset(d_sc_, 0, sizeof(Scalars) * capacity_));
   APH_HIP(hipMalloc(&d_flags_, sizeof(unsigned)));
-  APH_HIP(hipMalloc(&d_red_, sizeof(double) * 1032));
+  // reduction scratch of k_sumsq (1024 partials + result) and of the RHS
+  // perturbation (2 x 1024 partials + 2 results)
+  APH_HIP(hipMalloc(&d_red_, sizeof(double) * 2056));
   APH_HIP(hipMalloc(&d_red_counter_, sizeof(unsigned)));
   APH_HIP(hipMemset(d_red_counter_, 0, sizeof(unsigned)));
   APH_HIP(hipEventCreate(&ev0_));
@@ -434,6 +436,8 @@ int Engine::prepare(int slot, const double* x0, bool on_device, long long start,
                          st_));
   if (!s.active) dirty_ = true;
   s.active = true;
+  s.done_reported = false;
+  need_start_ = true;
   return APHPOIS_OK;
 }
 
@@ -464,7 +468,11 @@ int Engine::build_tables() {
     // The tiling of a system depends on that system only, so its reduction
     // trees (and results) are the same alone and inside a batch.
     const long long tgt = std::min(3072LL, std::max(1LL, cdiv(s.n, 2048)));
-    Rec a = make_stencil_rec(s.nx, s.ny, s.nz, method_ == kPcg ? vx_ : 1, tgt, tuning_);
+    // per-system tile (autotuner) over the engine-wide override
+    Tuning tu = tuning_;
+    if (s.tune_lx > 0) tu.lx = s.tune_lx;
+    if (s.tune_bz > 0) tu.bz = s.tune_bz;
+    Rec a = make_stencil_rec(s.nx, s.ny, s.nz, method_ == kPcg ? vx_ : 1, tgt, tu);
     a.sys = k;
     a.blk0 = (int)bst.size();
     Tuning none;
@@ -592,6 +600,77 @@ int Engine::capture() {
   return APHPOIS_OK;
 }
 
+// Copies the scalars of every system in the launch tables (one contiguous
+// range of slots) to the pinned mirror and waits for the stream.
+int Engine::pull_scalars() {
+  if (launch_slots_.empty()) return APHPOIS_OK;
+  const int lo = launch_slots_.front(), hi = launch_slots_.back();
+  APH_HIP(hipMemcpyAsync(h_sc_ + lo, d_sc_ + lo, sizeof(Scalars) * (hi - lo + 1),
+                         hipMemcpyDeviceToHost, st_));
+  APH_HIP(hipStreamSynchronize(st_));
+  return APHPOIS_OK;
+}
+
+void Engine::fill_info(int slot, double seconds) {
+  System& s = *slots_[slot];
+  const Scalars& c = h_sc_[slot];
+  aphpois_info& inf = s.info;
+  std::memset(&inf, 0, sizeof(inf));
+  inf.iter = c.iter;
+  inf.residual = c.residual;
+  inf.res_l2 = c.res_l2;
+  inf.res_max = c.res_max;
+  inf.true_relres = c.relres;
+  inf.mean = c.mean_pending;
+  inf.converged = c.converged;
+  inf.refinements = c.nru;
+  inf.status = c.status;
+  inf.seconds = seconds;
+}
+
+int Engine::perturb_rhs(int slot, double amp, unsigned long long key) {
+  System* sp = sys(slot);
+  if (!sp) return fail(APHPOIS_EINVAL, "no such slot");
+  if (method_ != kPcg || !sp->has_system) return fail(APHPOIS_ESTATE, "no PCG system");
+  APH_HIP(launch_perturb(const_cast<double*>(sp->dev.b64), sp->n, key, amp, d_red_, st_));
+  return APHPOIS_OK;
+}
+
+int Engine::step(int max_replays, std::vector<int>* finished) {
+  finished->clear();
+  if (method_ != kPcg) return fail(APHPOIS_EINVAL, "step() needs method PCG");
+  bool any = false;
+  for (auto& s : slots_) any = any || (s && s->active);
+  if (!any) return APHPOIS_OK;
+  if (dirty_) {
+    APH_TRY(build_tables());
+    APH_TRY(capture());
+  }
+  if (need_start_) {
+    // reliable-update kernels: only systems prepared since the last call
+    // (ru = 1) take part, running systems are at an iteration boundary
+    APH_HIP(launch_start(method_, fp32_, vx_, L_, st_));
+    need_start_ = false;
+  }
+  for (int r = 0; r < max_replays && finished->empty(); ++r) {
+    APH_HIP(hipGraphLaunch(gexec_, st_));
+    APH_TRY(pull_scalars());
+    bool alldone = true;
+    for (int i : launch_slots_) {
+      System& s = *slots_[i];
+      if (h_sc_[i].done && !s.done_reported) {
+        s.done_reported = true;
+        s.have_x = true;
+        fill_info(i, 0.0);
+        finished->push_back(i);
+      }
+      alldone = alldone && h_sc_[i].done;
+    }
+    if (alldone) break;
+  }
+  return APHPOIS_OK;
+}
+
 int Engine::run() {
   bool any = false;
   for (auto& s : slots_) any = any || (s && s->active);
@@ -622,10 +701,7 @@ int Engine::run() {
       APH_HIP(hipGraphLaunch(gexec_, st_));
     }
     ++loops;
-    for (int i : launch_slots_) {
-      APH_HIP(hipMemcpyAsync(h_sc_ + i, d_sc_ + i, sizeof(Scalars), hipMemcpyDeviceToHost, st_));
-    }
-    APH_HIP(hipStreamSynchronize(st_));
+    APH_TRY(pull_scalars());
     alldone = true;
     int iters_after = 0;
     for (int i : launch_slots_) {
@@ -657,34 +733,22 @@ int Engine::run() {
   if (method_ != kPcg) {
     APH_HIP(launch_true_res_f7(L_, st_));
   }
-  for (int i : launch_slots_) {
-    APH_HIP(hipMemcpyAsync(h_sc_ + i, d_sc_ + i, sizeof(Scalars), hipMemcpyDeviceToHost, st_));
-  }
-  APH_HIP(hipStreamSynchronize(st_));
+  APH_TRY(pull_scalars());
   float ms = 0;
   APH_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
   last_run_ms_ = ms;
   last_run_iters_ = 0;
   for (int i : launch_slots_) {
     System& s = *slots_[i];
-    const Scalars& c = h_sc_[i];
-    aphpois_info& inf = s.info;
-    std::memset(&inf, 0, sizeof(inf));
-    inf.iter = c.iter;
-    inf.residual = c.residual;
-    inf.res_l2 = c.res_l2;
-    inf.res_max = c.res_max;
-    inf.true_relres = c.relres;
-    inf.mean = c.mean_pending;
-    inf.converged = c.converged;
-    inf.refinements = c.nru;
-    inf.status = c.status;
-    inf.seconds = ms * 1e-3;
-    last_run_iters_ = std::max(last_run_iters_, c.iter);
-    s.active = false;
+    if (s.done_reported) continue;  // finished in an earlier call, idle since
+    fill_info(i, ms * 1e-3);
+    last_run_iters_ = std::max(last_run_iters_, h_sc_[i].iter);
     s.have_x = true;
+    s.done_reported = true;
   }
-  dirty_ = true;  // the active set changes with the next prepare
+  // systems stay in the launch tables (idle once done); preparing them again
+  // reuses the captured graph
+  need_start_ = false;
   return APHPOIS_OK;
 }
 

@@ -45,5 +45,9 @@ hipError_t launch_scatter(const double* src, int nx, int ny, int nz, double* dst
 hipError_t launch_sumsq(const double* v, long long n, double* partials, unsigned* counter,
                         double* out, hipStream_t st);
 hipError_t launch_fill(void* p, long long n, int elem_bytes, hipStream_t st);
+// b += amp * max|b| * (eta - mean eta), eta from the splitmix64 stream `key`;
+// partials needs 2 * 1024 + 2 doubles
+hipError_t launch_perturb(double* b, long long n, unsigned long long key, double amp,
+                          double* partials, hipStream_t st);
 
 }  // namespace aph

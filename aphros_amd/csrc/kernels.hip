@@ -863,6 +863,53 @@ __global__ __launch_bounds__(kBlock) void k_fin_sumsq(const double* __restrict__
   if (threadIdx.x == 0) *out = acc[0];
 }
 
+// Correlated right-hand sides for served request streams:
+//   b <- b + amp * max|b| * (eta - mean(eta)),  eta_i = 2 u(key, i) - 1
+// with u the splitmix64 counter generator (as in the synthetic systems).
+__device__ __forceinline__ unsigned long long d_sm64(unsigned long long x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+__device__ __forceinline__ double d_eta(unsigned long long key, long long i) {
+  return 2.0 * (double)(d_sm64(key + (unsigned long long)i) >> 11) * 0x1.0p-53 - 1.0;
+}
+
+// partials: sum eta, max |b|
+__global__ __launch_bounds__(kBlock) void k_perturb_stats(const double* __restrict__ b,
+                                                          long long n, unsigned long long key,
+                                                          double* __restrict__ partials) {
+  double acc[2] = {0.0, 0.0};
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (long long)gridDim.x * kBlock) {
+    acc[0] += d_eta(key, i);
+    acc[1] = fmax(acc[1], fabs(b[i]));
+  }
+  block_partial<2, 2u>(acc, partials, gridDim.x, blockIdx.x);
+}
+
+__global__ __launch_bounds__(kBlock) void k_fin_perturb_stats(double* __restrict__ partials,
+                                                              int nblk, long long n) {
+  double acc[2];
+  reduce_partials<2, 2u>(acc, partials, nblk);
+  if (threadIdx.x == 0) {
+    partials[2 * nblk] = acc[0] / (double)n;  // mean eta
+    partials[2 * nblk + 1] = acc[1];          // max |b|
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_perturb_apply(double* __restrict__ b, long long n,
+                                                          unsigned long long key, double amp,
+                                                          const double* __restrict__ stats) {
+  const double mean = stats[0], bmax = stats[1];
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (long long)gridDim.x * kBlock) {
+    b[i] += amp * bmax * (d_eta(key, i) - mean);
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_fill(T* __restrict__ p, long long n, T v) {
   for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
@@ -1019,6 +1066,20 @@ hipError_t launch_fill(void* p, long long n, int elem_bytes, hipStream_t st) {
   } else {
     APH_LAUNCH(k_fill<float>, grid_for(n), st, (float*)p, n, 0.0f);
   }
+  return hipGetLastError();
+}
+
+}  // namespace aph
+
+namespace aph {
+
+hipError_t launch_perturb(double* b, long long n, unsigned long long key, double amp,
+                          double* partials, hipStream_t st) {
+  int g = grid_for(n);
+  if (g > 1024) g = 1024;
+  APH_LAUNCH(k_perturb_stats, g, st, b, n, key, partials);
+  APH_LAUNCH(k_fin_perturb_stats, 1, st, partials, g, n);
+  APH_LAUNCH(k_perturb_apply, grid_for(n), st, b, n, key, amp, partials + 2 * g);
   return hipGetLastError();
 }
 

@@ -203,7 +203,75 @@ int aphpois_batch_remove(aphpois_batch* b, int slot);
 long long aphpois_batch_device_bytes(const aphpois_batch* b);
 /* Overrides the launch shape for the batch (see aphpois_set_tile). */
 int aphpois_batch_set_tile(aphpois_batch* b, int lx, int bz, int vx, int sm);
+/* Stencil tile of one system of the batch (chosen by the autotuner). */
+int aphpois_batch_set_system_tile(aphpois_batch* b, int slot, int lx, int bz);
+/* Wall time (s) of the last aphpois_batch_solve on the device stream. */
+double aphpois_batch_last_seconds(const aphpois_batch* b);
 int aphpois_batch_destroy(aphpois_batch* b);
+
+/* ------------- request server (continuous batching, one GPU) ------------- */
+/* No counterpart in the reference: the reference solves one system per call
+ * from inside the time loop.  The server keeps every admitted system's
+ * coefficients and Krylov state resident in device memory, admits requests
+ * as slots and memory free up, and advances all resident systems in the same
+ * kernel launches. */
+
+typedef struct aphpois_server aphpois_server;
+
+typedef struct aphpois_server_desc {
+  int precision;        /* APHPOIS_FP64 | APHPOIS_MIXED (method PCG) */
+  double tol;           /* true relative residual per time step */
+  int maxiter;          /* per time step */
+  double delta;         /* reliable-update trigger, 0 = default */
+  int check_every;      /* iterations per graph replay, 0 = auto */
+  int device;
+  long long mem_budget; /* admission budget in bytes of device memory, 0 = none */
+  int max_active;       /* resident systems at most, 0 = 1024 */
+} aphpois_server_desc;
+
+typedef struct aphpois_request {
+  int nx, ny, nz;
+  double h3;
+  const double* expr;     /* host Expr AoS, dense; read at admission */
+  int nsteps;             /* time steps solved in sequence, each warm-started
+                             from the previous step's solution */
+  const double* const* e7;/* optional: e7[k] = host constant term of step k
+                             (k >= 1; NULL entries or NULL array: see perturb) */
+  double perturb;         /* step k+1 without e7: b += perturb * max|b| * eta,
+                             eta zero-mean uniform in [-1, 1] */
+  unsigned long long seed;/* stream of eta */
+  double* x_out;          /* host dense solution of the last step, or NULL */
+} aphpois_request;
+
+typedef struct aphpois_result {
+  long long id;
+  int status;             /* APHPOIS_OK, ECANCEL, EBUDGET, ENUMERIC, ... */
+  int steps_done;
+  long long iters_total;  /* inner iterations over all steps */
+  int refinements_total;
+  double last_relres;     /* true relative residual of the last step */
+  double t_submit, t_admit, t_done;  /* seconds since server creation */
+} aphpois_result;
+
+typedef struct aphpois_server_stats {
+  int queued, active;
+  long long completed;
+  long long device_bytes;
+  double time;
+} aphpois_server_stats;
+
+int aphpois_server_create(const aphpois_server_desc* d, aphpois_server** out);
+/* Thread-safe and non-blocking.  The request and the arrays it points to must
+ * stay valid until the request completes. */
+int aphpois_server_submit(aphpois_server* s, const aphpois_request* r, long long* id);
+/* Waits for completion (timeout_s < 0: no limit) and releases the id. */
+int aphpois_server_wait(aphpois_server* s, long long id, double timeout_s,
+                        aphpois_result* out);
+int aphpois_server_cancel(aphpois_server* s, long long id);
+int aphpois_server_get_stats(aphpois_server* s, aphpois_server_stats* st);
+/* Cancels queued requests, stops the worker and frees the device state. */
+int aphpois_server_destroy(aphpois_server* s);
+const char* aphpois_server_last_error(void);
 
 /* ------------------------- synthetic systems (host) ---------------------- */
 

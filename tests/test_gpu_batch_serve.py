@@ -1,0 +1,111 @@
+"""Batched and served solves on the GPU.
+
+Each system's launch geometry (tiles, workgroups, reduction order) depends on
+that system only, so a system's result is bit-identical alone, in a batch,
+and in a served mix; the tests assert exact equality.  Convergence is checked
+on the FP64 true relative residual recomputed by the oracle.
+"""
+import numpy as np
+import pytest
+
+from aphros_amd import _lib as L
+from aphros_amd import synth
+from aphros_amd.batch import Batch
+from aphros_amd.serve import Server
+from aphros_amd.solver import Poisson
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SYSTEMS = [(32, 3, 0.1, 0.2, 1000.0, 21), (48, 5, 0.05, 0.15, 10.0, 22),
+           (64, 8, 0.08, 0.15, 1000.0, 23), (32, 1, 0.2, 0.3, 100.0, 24)]
+
+
+def _sys(n, ns, rmin, rmax, ratio, seed):
+    e, _, _ = synth.system(n, ns, rmin, rmax, ratio, seed)
+    return e.reshape(n, n, n, 8)
+
+
+def _single(E, precision=L.MIXED, x0=None, tol=1e-8):
+    s = Poisson(E.shape[:3], method=L.PCG, precision=precision, resmode=L.RES_REL_TRUE,
+                tol=tol, maxiter=50000)
+    s.set_system(E)
+    x, info = s.solve(x0)
+    s.close()
+    return x, info
+
+
+@pytest.mark.parametrize("precision", [L.MIXED, L.FP64])
+def test_batch_equals_single(precision):
+    systems = [_sys(*p) for p in SYSTEMS]
+    b = Batch(precision=precision, tol=1e-8, maxiter=50000)
+    slots = [b.add(E) for E in systems]
+    b.solve()
+    for E, sl in zip(systems, slots):
+        xb, ib = b.get(sl)
+        xs, is_ = _single(E, precision)
+        print(f"n={E.shape[0]} iter={ib['iter']} relres={ib['true_relres']:.2e}", flush=True)
+        assert ib["converged"] == 1 and ib["iter"] == is_["iter"]
+        assert np.array_equal(xb, xs)
+        assert O.true_relres(E, xb) <= 1e-8 * (1 + 1e-6)
+    b.close()
+
+
+def test_batch_warm_start_new_rhs():
+    E = _sys(*SYSTEMS[2])
+    b = Batch(precision=L.MIXED, tol=1e-8)
+    sl = b.add(E)
+    b.solve()
+    x1, i1 = b.get(sl)
+    E2 = E.copy()
+    rng = np.random.default_rng(3)
+    pert = rng.uniform(-1, 1, E.shape[:3])
+    pert -= pert.mean()
+    E2[..., 7] += 0.01 * np.abs(E[..., 7]).max() * pert
+    b.set_rhs(sl, E2[..., 7])           # warm start from x1
+    b.solve()
+    x2, i2 = b.get(sl)
+    assert i2["converged"] == 1 and i2["iter"] < i1["iter"]
+    assert O.true_relres(E2, x2) <= 1e-8 * (1 + 1e-6)
+    b.close()
+
+
+def test_server_alone_equals_mix():
+    A = _sys(*SYSTEMS[2])
+    srv = Server(precision=L.MIXED, tol=1e-8, mem_budget=0)
+    ra = srv.submit(A, nsteps=3, perturb=0.01, seed=5, want_x=True)
+    alone = srv.wait(ra, timeout=120)
+    assert alone["status"] == 0 and alone["steps_done"] == 3
+    ids = [srv.submit(_sys(*SYSTEMS[i]), nsteps=2, perturb=0.01, seed=10 + i)
+           for i in (0, 1, 3)]
+    rb = srv.submit(A, nsteps=3, perturb=0.01, seed=5, want_x=True)
+    ids.append(srv.submit(_sys(*SYSTEMS[1]), nsteps=1, seed=30))
+    mixed = srv.wait(rb, timeout=120)
+    others = [srv.wait(i, timeout=120) for i in ids]
+    srv.close()
+    assert mixed["status"] == 0
+    assert all(o["status"] == 0 for o in others)
+    assert mixed["iters_total"] == alone["iters_total"]
+    assert np.array_equal(mixed["x"], alone["x"])
+    assert mixed["last_relres"] <= 1e-8
+
+
+def test_server_admission_by_budget_and_cancel():
+    E = _sys(*SYSTEMS[2])
+    # one 64^3 mixed system needs about 25 MB; budget for one at a time
+    srv = Server(precision=L.MIXED, tol=1e-8, mem_budget=40 << 20)
+    r1 = srv.submit(E, nsteps=2, seed=1)
+    r2 = srv.submit(E, nsteps=2, seed=2)
+    r3 = srv.submit(E, nsteps=2, seed=3)
+    srv.cancel(r3)
+    a = srv.wait(r1, timeout=120)
+    b = srv.wait(r2, timeout=120)
+    c = srv.wait(r3, timeout=120)
+    assert a["status"] == 0 and b["status"] == 0
+    assert b["t_admit"] >= a["t_done"] - 1e-6   # admitted once memory freed
+    assert c["status"] == L.ECANCEL
+    big = _sys(96, 2, 0.1, 0.2, 10.0, 4)        # never fits
+    r4 = srv.submit(big, nsteps=1)
+    d = srv.wait(r4, timeout=120)
+    assert d["status"] == L.EBUDGET
+    srv.close()
