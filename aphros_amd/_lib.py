@@ -16,6 +16,7 @@ CONJUGATE, JACOBI, PCG = 0, 1, 2
 FP64, MIXED = 0, 1
 RES_L2, RES_MAX, RES_REL_TRUE = 0, 1, 2
 NULL_AUTO, NULL_OFF, NULL_ON = 0, 1, 2
+PRECOND_JACOBI, PRECOND_MG = 0, 1
 
 ERROR_NAMES = {OK: "OK", EINVAL: "EINVAL", ENOMEM: "ENOMEM", EDEVICE: "EDEVICE",
                ESTATE: "ESTATE", EBUDGET: "EBUDGET", ENUMERIC: "ENUMERIC",
@@ -27,7 +28,7 @@ class Desc(C.Structure):
                 ("method", C.c_int), ("precision", C.c_int), ("resmode", C.c_int),
                 ("tol", C.c_double), ("miniter", C.c_int), ("maxiter", C.c_int),
                 ("nullspace", C.c_int), ("delta", C.c_double), ("check_every", C.c_int),
-                ("device", C.c_int), ("mem_budget", C.c_longlong)]
+                ("device", C.c_int), ("mem_budget", C.c_longlong), ("precond", C.c_int)]
 
 
 class Info(C.Structure):
@@ -43,13 +44,14 @@ class Info(C.Structure):
 class BatchDesc(C.Structure):
     _fields_ = [("precision", C.c_int), ("resmode", C.c_int), ("tol", C.c_double),
                 ("maxiter", C.c_int), ("delta", C.c_double), ("check_every", C.c_int),
-                ("device", C.c_int), ("mem_budget", C.c_longlong), ("max_systems", C.c_int)]
+                ("device", C.c_int), ("mem_budget", C.c_longlong), ("max_systems", C.c_int),
+                ("precond", C.c_int)]
 
 
 class ServerDesc(C.Structure):
     _fields_ = [("precision", C.c_int), ("tol", C.c_double), ("maxiter", C.c_int),
                 ("delta", C.c_double), ("check_every", C.c_int), ("device", C.c_int),
-                ("mem_budget", C.c_longlong), ("max_active", C.c_int)]
+                ("mem_budget", C.c_longlong), ("max_active", C.c_int), ("precond", C.c_int)]
 
 
 class Request(C.Structure):

@@ -33,12 +33,28 @@ struct Scalars {
   double bnorm2;      // (b,b) in FP64
   double relres, res_l2, res_max, residual;
   double mean_pending;  // mean to subtract from x64 (nullspace removal)
-  double pad0;
+  double rz;            // (r, z) of the current iterate (multigrid mode)
   int iter, miniter, maxiter, done;
   int ru, converged, nru, status;
   int resmode, singular, first, cur;  // cur: index of the current p/u buffer
-  int method, pad1;
+  int method, pc;  // pc: 1 if a multigrid preconditioner supplies z
 };
+
+// One level of the multigrid preconditioner (level 0 = the fine grid).
+// Operator of every level: (L u)_c = sum_faces k_f (u_c - u_nb) + m_c u_c.
+struct MgLevel {
+  int nx, ny, nz, pad;
+  long long n;
+  const void* kx;  // plus-face conductances (element type of the engine)
+  const void* ky;
+  const void* kz;
+  const void* m;   // mass term or nullptr
+  const void* dinv;
+  void* r;         // right-hand side of the level (level 0: the CG residual)
+  void* z[2];      // correction, ping-pong
+};
+
+constexpr int kMaxLevels = 12;
 
 struct SysDev {
   int nx, ny, nz, has_m;
@@ -70,6 +86,12 @@ struct SysDev {
   // workgroups of this system in the stencil, streaming and residual
   // launches (= number of partials each finalize kernel reads)
   int nblk_st, nblk_sm, nblk_ru;
+  // preconditioned mode (multigrid): the CG direction is built from z
+  // (= M^-1 r) instead of r, and S is 1 (S == nullptr)
+  int nlev;        // multigrid levels (0 = Jacobi-scaled CG without z)
+  int nblk_pc;     // workgroups of the level-0 dot in the last smoothing sweep
+  const void* z;   // vector the p update reads (r in Jacobi mode)
+  const MgLevel* lev;  // device array of nlev levels
 };
 
 // One record per system in a launch: which blocks belong to it and how they

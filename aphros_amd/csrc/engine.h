@@ -37,6 +37,9 @@ struct System {
   bool active = false;  // takes part in the next solve
   bool have_x = false;  // x64/u holds a previous solution (warm start)
   int tune_lx = 0, tune_bz = 0;  // stencil tile of this system (0 = engine default)
+  // multigrid levels (host copy; device copy in d_lev)
+  std::vector<MgLevel> lev;
+  MgLevel* d_lev = nullptr;
   bool done_reported = true;     // the last solve's result has been collected
   std::vector<void*> allocs;
   long long bytes = 0;
@@ -52,9 +55,11 @@ class Engine {
   Engine() = default;
   ~Engine();
   // returns APHPOIS_* code; message in err()
+  // precond (method PCG): 0 = Jacobi scaling, 1 = multigrid V-cycle
   int init(int device, int method, int precision, int resmode, double tol, int miniter,
            int maxiter, int nullspace, double delta, int check_every, long long budget,
-           int capacity);
+           int capacity, int precond = 0);
+  bool multigrid() const { return pc_; }
   int add_system(int nx, int ny, int nz, double h3, int* slot);
   int set_system(int slot, const double* expr, bool on_device, long long start, long long ly,
                  long long lz);
@@ -117,12 +122,12 @@ class Engine {
     prof_reset();
   }
   void prof_reset() {
-    for (int j = 0; j < 8; ++j) {
+    for (int j = 0; j < 16; ++j) {
       prof_ms_[j] = 0;
       prof_n_[j] = 0;
     }
   }
-  int prof_kernels() const { return iter_kernels(method_); }
+  int prof_kernels() const { return iter_kernels(method_, pc_ ? 1 : 0); }
   double prof_avg_ms(int j) const { return prof_n_[j] ? prof_ms_[j] / prof_n_[j] : 0.0; }
   long long prof_count(int j) const { return prof_n_[j]; }
 
@@ -142,6 +147,7 @@ class Engine {
   int device_ = 0;
   int method_ = kConjugate;
   int fp32_ = 0;
+  bool pc_ = false;
   int resmode_ = 0;
   double tol_ = 0;
   int miniter_ = 1, maxiter_ = 100;
@@ -177,8 +183,8 @@ class Engine {
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
   bool profile_ = false;
   std::vector<hipEvent_t> pev_;
-  double prof_ms_[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  long long prof_n_[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  double prof_ms_[16] = {};
+  long long prof_n_[16] = {};
   double last_run_ms_ = 0;
   int last_run_iters_ = 0;
   std::string err_;

@@ -89,7 +89,7 @@ Engine::~Engine() {
 
 int Engine::init(int device, int method, int precision, int resmode, double tol, int miniter,
                  int maxiter, int nullspace, double delta, int check_every, long long budget,
-                 int capacity) {
+                 int capacity, int precond) {
   if (method < 0 || method > 2) return fail(APHPOIS_EINVAL, "unknown method");
   if (precision < 0 || precision > 1) return fail(APHPOIS_EINVAL, "unknown precision");
   if (resmode < 0 || resmode > 2) return fail(APHPOIS_EINVAL, "unknown resmode");
@@ -106,6 +106,10 @@ int Engine::init(int device, int method, int precision, int resmode, double tol,
   APH_HIP(hipSetDevice(device_));
   method_ = method;
   fp32_ = precision == 1;
+  if (precond < 0 || precond > 1) return fail(APHPOIS_EINVAL, "unknown preconditioner");
+  if (precond == 1 && method != kPcg)
+    return fail(APHPOIS_EINVAL, "the multigrid preconditioner needs method PCG");
+  pc_ = precond == 1;
   resmode_ = resmode;
   tol_ = tol;
   miniter_ = miniter;
@@ -203,11 +207,38 @@ int Engine::add_system(int nx, int ny, int nz, double h3, int* slot) {
     if (rc == APHPOIS_OK) rc = alloc(S, dst, b);
   };
   if (method_ == kPcg) {
-    void *kx, *ky, *kz, *Sc, *r, *p0, *p1, *q, *xh, *x64, *b64, *s4;
+    void *kx, *ky, *kz, *Sc = nullptr, *r, *p0, *p1, *q, *xh, *x64, *b64, *s4;
+    if (pc_) {
+      // multigrid levels: halve while every edge is even and the coarse grid
+      // keeps at least 8 cells; the coarsest level must fit the LDS solver
+      S.lev.clear();
+      int lx = nx, ly = ny, lz = nz;
+      for (;;) {
+        MgLevel L{};
+        L.nx = lx;
+        L.ny = ly;
+        L.nz = lz;
+        L.n = (long long)lx * ly * lz;
+        S.lev.push_back(L);
+        if ((int)S.lev.size() >= kMaxLevels || (lx | ly | lz) & 1 ||
+            (long long)(lx / 2) * (ly / 2) * (lz / 2) < 8)
+          break;
+        lx /= 2;
+        ly /= 2;
+        lz /= 2;
+      }
+      const long long cmax = fp32_ ? 8192 : 4096;
+      if (S.lev.size() < 2 || S.lev.back().n > cmax) {
+        slots_[id].reset();
+        return fail(APHPOIS_EINVAL,
+                    "grid cannot be coarsened to a small enough coarsest level (edges need "
+                    "enough factors of 2) for the multigrid preconditioner");
+      }
+    }
     A(&kx, n * es);
     A(&ky, n * es);
     A(&kz, n * es);
-    A(&Sc, n * es);
+    if (!pc_) A(&Sc, n * es);
     A(&r, n * es);
     A(&p0, n * es);
     A(&p1, n * es);
@@ -233,6 +264,48 @@ int Engine::add_system(int nx, int ny, int nz, double h3, int* slot) {
     S.dev.x64 = (double*)x64;
     S.dev.b64 = (const double*)b64;
     for (int k = 0; k < 4; ++k) S.dev.s4[k] = (const double*)s4 + k * n;
+    S.dev.z = r;
+    S.dev.nlev = 0;
+    if (pc_) {
+      // level 0 shares the conductances and the residual of the CG
+      for (size_t l = 0; l < S.lev.size() && rc == APHPOIS_OK; ++l) {
+        MgLevel& L = S.lev[l];
+        void *dinv, *z0, *z1;
+        if (l == 0) {
+          L.kx = kx;
+          L.ky = ky;
+          L.kz = kz;
+          L.r = r;
+        } else {
+          void *ckx, *cky, *ckz, *cr;
+          A(&ckx, L.n * es);
+          A(&cky, L.n * es);
+          A(&ckz, L.n * es);
+          A(&cr, L.n * es);
+          L.kx = ckx;
+          L.ky = cky;
+          L.kz = ckz;
+          L.r = cr;
+        }
+        A(&dinv, L.n * es);
+        A(&z0, L.n * es);
+        A(&z1, L.n * es);
+        L.dinv = dinv;
+        L.z[0] = z0;
+        L.z[1] = z1;
+      }
+      void* dl = nullptr;
+      A(&dl, (long long)sizeof(MgLevel) * (long long)S.lev.size());
+      if (rc != APHPOIS_OK) {
+        free_system(S);
+        slots_[id].reset();
+        return rc;
+      }
+      S.d_lev = (MgLevel*)dl;
+      S.dev.nlev = (int)S.lev.size();
+      S.dev.lev = S.d_lev;
+      S.dev.z = S.lev[0].z[1];  // level-0 correction after the post-smoothing
+    }
   } else {
     void* f7;
     A(&f7, n * 64);
@@ -343,6 +416,24 @@ int Engine::set_system(int slot, const double* expr, bool on_device, long long s
                                has_m ? const_cast<void*>(s.dev.m) : nullptr,
                                const_cast<double*>(s.dev.s4[0]), const_cast<double*>(s.dev.b64),
                                0, st_));
+    if (pc_) {
+      // mass terms of the coarse levels, then coarse operators and D^-1
+      for (size_t l = 0; l < s.lev.size(); ++l) {
+        MgLevel& L = s.lev[l];
+        if (!has_m) {
+          L.m = nullptr;
+        } else if (l == 0) {
+          L.m = s.dev.m;
+        } else if (!L.m) {
+          void* m = nullptr;
+          APH_TRY(alloc(s, &m, L.n * (fp32_ ? 4 : 8)));
+          L.m = m;
+        }
+      }
+      APH_HIP(hipMemcpyAsync(s.d_lev, s.lev.data(), sizeof(MgLevel) * s.lev.size(),
+                             hipMemcpyHostToDevice, st_));
+      APH_HIP(launch_mg_setup(fp32_, s.lev.data(), (int)s.lev.size(), st_));
+    }
   } else {
     if (method_ == kJacobi && (flags & 4u))
       return fail(APHPOIS_EINVAL, "zero diagonal coefficient (Jacobi divides by e0)");
@@ -431,6 +522,7 @@ int Engine::prepare(int slot, const double* x0, bool on_device, long long start,
   sc.first = 1;
   sc.cur = 0;
   sc.method = method_;
+  sc.pc = pc_ ? 1 : 0;
   h_sc_[slot] = sc;
   APH_HIP(hipMemcpyAsync(d_sc_ + slot, h_sc_ + slot, sizeof(Scalars), hipMemcpyHostToDevice,
                          st_));
@@ -463,6 +555,11 @@ int Engine::build_tables() {
   std::vector<Rec> rst, rsm, rru;
   std::vector<int> bst, bsm, bru;
   std::vector<SysDev> sds;
+  // multigrid: per level l, tilings of level l (lf) and l+1 (lc)
+  std::vector<std::vector<Rec>> rlf(kMaxLevels), rlc(kMaxLevels);
+  std::vector<std::vector<int>> blf(kMaxLevels), blc(kMaxLevels);
+  int nlev_max = 0;
+  long long coarse_max = 0;
   for (int k = 0; k < ns; ++k) {
     System& s = *slots_[launch_slots_[k]];
     // The tiling of a system depends on that system only, so its reduction
@@ -494,8 +591,32 @@ int Engine::build_tables() {
     rst.push_back(a);
     rsm.push_back(b);
     rru.push_back(c);
-    // partials: 3 values per block of the largest launch of this system
-    const int nmax = std::max(a.nblk, std::max(b.nblk, c.nblk));
+    int nblk_pc = 0;
+    if (pc_) {
+      const int nl = (int)s.lev.size();
+      nlev_max = std::max(nlev_max, nl);
+      coarse_max = std::max(coarse_max, s.lev.back().n);
+      for (int l = 0; l + 1 < nl; ++l) {
+        const MgLevel& F = s.lev[l];
+        const MgLevel& C = s.lev[l + 1];
+        Rec f = make_stencil_rec(F.nx, F.ny, F.nz, 1,
+                                 std::min(3072LL, std::max(1LL, cdiv(F.n, 2048))), none);
+        f.sys = k;
+        f.blk0 = (int)blf[l].size();
+        for (int i = 0; i < f.nblk; ++i) blf[l].push_back((int)rlf[l].size());
+        rlf[l].push_back(f);
+        Rec g = make_stencil_rec(C.nx, C.ny, C.nz, 1,
+                                 std::min(3072LL, std::max(1LL, cdiv(C.n, 2048))), none);
+        g.sys = k;
+        g.blk0 = (int)blc[l].size();
+        for (int i = 0; i < g.nblk; ++i) blc[l].push_back((int)rlc[l].size());
+        rlc[l].push_back(g);
+        if (l == 0) nblk_pc = f.nblk;
+      }
+    }
+    s.dev.nblk_pc = nblk_pc;
+    // partials: 3 values per block of the largest reducing launch of this system
+    const int nmax = std::max(std::max(a.nblk, nblk_pc), std::max(b.nblk, c.nblk));
     const long long need = 3LL * nmax * 8;
     if (need > s.partials_cap) {
       if (s.partials) APH_HIP(hipFree(s.partials));
@@ -511,8 +632,12 @@ int Engine::build_tables() {
     sds.push_back(s.dev);
   }
   // device tables in one buffer each
-  const long long nrec = (long long)rst.size() + rsm.size() + rru.size();
-  const long long nb2r = (long long)bst.size() + bsm.size() + bru.size();
+  long long nrec = (long long)rst.size() + rsm.size() + rru.size();
+  long long nb2r = (long long)bst.size() + bsm.size() + bru.size();
+  for (int l = 0; l < kMaxLevels; ++l) {
+    nrec += (long long)rlf[l].size() + rlc[l].size();
+    nb2r += (long long)blf[l].size() + blc[l].size();
+  }
   if (nrec > rec_cap_) {
     if (d_rec_) APH_HIP(hipFree(d_rec_));
     APH_HIP(hipMalloc(&d_rec_, sizeof(Rec) * std::max(1LL, nrec)));
@@ -534,6 +659,18 @@ int Engine::build_tables() {
   b2r.insert(b2r.end(), bst.begin(), bst.end());
   for (int v : bsm) b2r.push_back(v);
   for (int v : bru) b2r.push_back(v);
+  size_t rec_lf_ofs[kMaxLevels], rec_lc_ofs[kMaxLevels], b2r_lf_ofs[kMaxLevels],
+      b2r_lc_ofs[kMaxLevels];
+  for (int l = 0; l < kMaxLevels; ++l) {
+    rec_lf_ofs[l] = recs.size();
+    recs.insert(recs.end(), rlf[l].begin(), rlf[l].end());
+    rec_lc_ofs[l] = recs.size();
+    recs.insert(recs.end(), rlc[l].begin(), rlc[l].end());
+    b2r_lf_ofs[l] = b2r.size();
+    b2r.insert(b2r.end(), blf[l].begin(), blf[l].end());
+    b2r_lc_ofs[l] = b2r.size();
+    b2r.insert(b2r.end(), blc[l].begin(), blc[l].end());
+  }
   if (ns > 0) {
     APH_HIP(hipMemcpy(d_sds_, sds.data(), sizeof(SysDev) * ns, hipMemcpyHostToDevice));
     APH_HIP(hipMemcpy(d_rec_, recs.data(), sizeof(Rec) * nrec, hipMemcpyHostToDevice));
@@ -550,6 +687,19 @@ int Engine::build_tables() {
   L_.nsm = (int)bsm.size();
   L_.nru = (int)bru.size();
   L_.ns = ns;
+  L_.pc = pc_ ? 1 : 0;
+  L_.nlev_max = nlev_max;
+  L_.nu = 2;
+  L_.coarse_sweeps = 40;
+  L_.coarse_lds = (int)(2 * coarse_max * (fp32_ ? 4 : 8));
+  for (int l = 0; l < kMaxLevels; ++l) {
+    L_.rec_lf[l] = d_rec_ + rec_lf_ofs[l];
+    L_.rec_lc[l] = d_rec_ + rec_lc_ofs[l];
+    L_.b2r_lf[l] = d_b2r_ + b2r_lf_ofs[l];
+    L_.b2r_lc[l] = d_b2r_ + b2r_lc_ofs[l];
+    L_.nlf[l] = (int)blf[l].size();
+    L_.nlc[l] = (int)blc[l].size();
+  }
   dirty_ = false;
   return APHPOIS_OK;
 }
@@ -573,7 +723,7 @@ int Engine::capture() {
     k = std::max(2, std::min(64, k));
   }
   if (method_ == kJacobi && (k & 1)) ++k;
-  const int nk = iter_kernels(method_);
+  const int nk = iter_kernels(method_, pc_ ? 1 : 0);
   if (profile_) {
     const size_t ne = (size_t)k * nk + 1;
     while (pev_.size() < ne) {
@@ -689,7 +839,7 @@ int Engine::run() {
     if (profile_) {
       // same kernels as the graph, launched one by one with an event before
       // each kernel (events cannot be timed inside a replayed graph)
-      const int nk = iter_kernels(method_);
+      const int nk = iter_kernels(method_, pc_ ? 1 : 0);
       for (int sl = 0; sl < graph_k_; ++sl) {
         for (int j = 0; j < nk; ++j) {
           APH_HIP(hipEventRecord(pev_[(size_t)sl * nk + j], st_));
@@ -710,7 +860,7 @@ int Engine::run() {
     }
     if (profile_) {
       // slots that ran a full iteration for at least one system
-      const int nk = iter_kernels(method_);
+      const int nk = iter_kernels(method_, pc_ ? 1 : 0);
       const int ran = std::min(graph_k_, iters_after - iters_before);
       for (int sl = 0; sl < ran; ++sl) {
         for (int j = 0; j < nk; ++j) {

@@ -16,12 +16,30 @@ struct LaunchSet {
   const int* b2r_ru = nullptr;
   int nst = 0, nsm = 0, nru = 0;
   int ns = 0;  // systems (one workgroup each in the finalize kernels)
+  // multigrid preconditioner (pc = 1): per level l < nlev_max - 1, the
+  // one-cell-per-thread tiling of level l (rec_lf) and of level l+1 (rec_lc)
+  // over the systems that coarsen below l
+  int pc = 0;
+  int nlev_max = 0;
+  int nu = 2;          // smoothing sweeps before and after the coarse correction
+  int coarse_sweeps = 40;
+  int coarse_lds = 0;  // dynamic LDS bytes of the coarsest-level kernel
+  const Rec* rec_lf[kMaxLevels] = {};
+  const int* b2r_lf[kMaxLevels] = {};
+  int nlf[kMaxLevels] = {};
+  const Rec* rec_lc[kMaxLevels] = {};
+  const int* b2r_lc[kMaxLevels] = {};
+  int nlc[kMaxLevels] = {};
 };
 
 // kernels of one iteration, each reducing kernel followed by its finalize
-// kernel: PCG: spmv, fin, update, fin, ru_accum, fin, ru_resid, fin;
+// kernel: PCG: spmv, fin, update, fin, ru_accum, fin, ru_resid, fin and, with
+// the multigrid preconditioner, [8] the V-cycle with its finalize;
 // CONJUGATE: spmv, fin, update, fin; JACOBI: sweep, fin
-int iter_kernels(int method);
+int iter_kernels(int method, int pc = 0);
+// Coarse operators and inverse diagonals of levels 0..nlev-1 (level 0 holds
+// the fine conductances already).
+hipError_t launch_mg_setup(int fp32, const MgLevel* levels, int nlev, hipStream_t st);
 hipError_t launch_kernel(int method, int fp32, int vx, const LaunchSet& L, hipStream_t st,
                          int j);
 hipError_t launch_iter(int method, int fp32, int vx, const LaunchSet& L, hipStream_t st);

@@ -86,7 +86,10 @@ __device__ __forceinline__ Tile tile_of(const Rec& rc, const SysDev& s, int lb) 
 // wrap) are recomputed from global memory by the edge threads.  u is formed
 // in FP64 from the stored (rounded) p and S, so the differences u_c - u_nb are
 // exact and q carries the rounding of its final store only.
-template <typename T, int VX>
+//
+// SC = false (multigrid mode): S is 1 and the direction is built from the
+// preconditioned residual z: p = z + beta p_in, q = L p.
+template <typename T, int VX, bool SC>
 __global__ __launch_bounds__(kBlock) void k_pcg_spmv(const SysDev* __restrict__ sds,
                                                      const Rec* __restrict__ recs,
                                                      const int* __restrict__ b2r) {
@@ -102,7 +105,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv(const SysDev* __restrict__ 
   const int nx = s.nx, ny = s.ny, nz = s.nz;
   const int cur = sc->cur;
   const double beta = sc->beta;
-  const T* __restrict__ r = (const T*)s.r;
+  const T* __restrict__ r = (const T*)s.z;  // r (Jacobi-scaled) or z = M^-1 r
   const T* __restrict__ pin = (const T*)s.p[cur];
   T* __restrict__ pout = (T*)s.p[cur ^ 1];
   T* __restrict__ q = (T*)s.q;
@@ -133,11 +136,18 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv(const SysDev* __restrict__ 
   auto pnew_st = [&](long long i) -> V {
     return __builtin_convertvector(ldd<T, VX>(r, i) + beta * ldd<T, VX>(pin, i), V);
   };
+  auto sld = [&](long long i) -> V {
+    if constexpr (SC) return ldv<T, VX>(S, i);
+    return (T)1;
+  };
   auto unew = [&](long long i) -> D {
-    return ldd<T, VX>(S, i) * cvt<VX>(pnew_st(i));
+    if constexpr (SC) return ldd<T, VX>(S, i) * cvt<VX>(pnew_st(i));
+    return cvt<VX>(pnew_st(i));
   };
   auto unew1 = [&](long long i) -> double {
-    return (double)S[i] * (double)(T)((double)r[i] + beta * (double)pin[i]);
+    const double p = (double)(T)((double)r[i] + beta * (double)pin[i]);
+    if constexpr (SC) return (double)S[i] * p;
+    return p;
   };
   // registers: u of planes z-1, z, z+1 in FP64; p, S and kz kept as stored (T)
   D um = 0.0, uc = 0.0;
@@ -149,7 +159,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv(const SysDev* __restrict__ 
     kzm = ldv<T, VX>(kz, i);
     i = (long long)t.za * pl + row + t.x0;
     pc = pnew_st(i);
-    Sc = ldv<T, VX>(S, i);
+    Sc = sld(i);
     uc = cvt<VX>(Sc) * cvt<VX>(pc);
 #pragma unroll
     for (int v = 0; v < VX; ++v) su[0][lofs + v] = uc[v];
@@ -163,7 +173,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv(const SysDev* __restrict__ 
       const long long ic = base + row + t.x0;
       const long long in = (long long)wrapp(z, nz) * pl + row + t.x0;
       const V pn = pnew_st(in);
-      const V Sn = ldv<T, VX>(S, in);
+      const V Sn = sld(in);
       const D un = cvt<VX>(Sn) * cvt<VX>(pn);
       if (z + 1 < t.zb) {
 #pragma unroll
@@ -288,7 +298,9 @@ __global__ __launch_bounds__(kBlock) void k_fin_pcg_update(const SysDev* __restr
     if (trig) {
       sc->ru = 1;
       sc->rho_pend = rn;
-    } else {
+    } else if (!sc->pc) {
+      // Jacobi-scaled CG: z = r, so (r, z) = (r, r); with a multigrid
+      // preconditioner beta comes from (r, z) after the V-cycle
       sc->beta = rn / sc->rho;
       sc->rho = rn;
     }
@@ -315,7 +327,8 @@ __global__ __launch_bounds__(kBlock) void k_ru_accum(const SysDev* __restrict__ 
   double acc[1] = {0.0};
   const long long stride = (long long)rc.nblk * kBlock * VX;
   for (long long i = ((long long)lb * kBlock + threadIdx.x) * VX; i < s.n; i += stride) {
-    const D xv = (*reinterpret_cast<const D*>(x + i) - mp) + ldd<T, VX>(S, i) * ldd<T, VX>(xh, i);
+    const D sv = S ? ldd<T, VX>(S, i) : (D)(1.0);
+    const D xv = (*reinterpret_cast<const D*>(x + i) - mp) + sv * ldd<T, VX>(xh, i);
     *reinterpret_cast<D*>(x + i) = xv;
     std_<T, VX>(xh, i, (D)(0.0));
 #pragma unroll
@@ -382,7 +395,7 @@ __global__ __launch_bounds__(kBlock) void k_ru_resid(const SysDev* __restrict__ 
         ax += ez[bm + row + xx] * (x[bm + row + xx] - mp);
         ax += ez[c] * (x[bp + row + xx] - mp);
         const double r64 = bb[c] - ax;
-        const T rh = (T)(sgn * (double)S[c] * r64);
+        const T rh = (T)(sgn * (S ? (double)S[c] : 1.0) * r64);
         r[c] = rh;
         acc[0] += (double)rh * (double)rh;
         acc[1] += r64 * r64;
@@ -417,13 +430,277 @@ __global__ __launch_bounds__(kBlock) void k_fin_ru_resid(const SysDev* __restric
       sc->converged = conv ? 1 : 0;
       sc->done = 1;
     } else {
-      sc->beta = sc->first ? 0.0 : rh2 / sc->rho;
-      sc->rho = rh2;
+      if (!sc->pc) {
+        sc->beta = sc->first ? 0.0 : rh2 / sc->rho;
+        sc->rho = rh2;
+      }
       sc->rmax2 = rh2;
       sc->gamma = rh2 > 0 ? metric / sqrt(rh2) : 1.0;
-      sc->first = 0;
+      if (!sc->pc) sc->first = 0;
     }
     sc->ru = 0;
+  }
+}
+
+// ===========================================================================
+// Multigrid preconditioner z = M^-1 r: one symmetric V-cycle over the levels
+// of each system.  Level operators (L u)_c = sum_f k_f (u_c - u_nb) + m_c u_c
+// with the coarse conductances from coarsening (k_mg_coarsen); damped Jacobi
+// smoothing (weight kOmega, as many sweeps after the coarse correction as
+// before it); restriction = sum over the 2x2x2 children, prolongation =
+// injection (its transpose), so the V-cycle is a symmetric linear operator.
+// Arithmetic in the engine's element type T.
+// ===========================================================================
+constexpr double kOmega = 0.8;
+
+template <typename T>
+struct LevelView {
+  int nx, ny, nz;
+  long long pl;
+  const T* kx;
+  const T* ky;
+  const T* kz;
+  const T* m;
+  const T* dinv;
+  __device__ explicit LevelView(const MgLevel& L)
+      : nx(L.nx), ny(L.ny), nz(L.nz), pl((long long)L.nx * L.ny), kx((const T*)L.kx),
+        ky((const T*)L.ky), kz((const T*)L.kz), m((const T*)L.m), dinv((const T*)L.dinv) {}
+  __device__ long long at(int x, int y, int z) const { return (long long)z * pl + (long long)y * nx + x; }
+  // (L u)_c where U(i) returns u at flat index i
+  template <class U>
+  __device__ T lap(int x, int y, int z, U u) const {
+    const long long c = at(x, y, z);
+    const long long xm = at(wrapm(x, nx), y, z), xp = at(wrapp(x, nx), y, z);
+    const long long ym = at(x, wrapm(y, ny), z), yp = at(x, wrapp(y, ny), z);
+    const long long zm = at(x, y, wrapm(z, nz)), zp = at(x, y, wrapp(z, nz));
+    const T uc = u(c);
+    T s = kx[c] * (uc - u(xp)) + kx[xm] * (uc - u(xm)) + ky[c] * (uc - u(yp)) +
+          ky[ym] * (uc - u(ym)) + kz[c] * (uc - u(zp)) + kz[zm] * (uc - u(zm));
+    if (m) s += m[c] * uc;
+    return s;
+  }
+};
+
+// Cell of a one-cell-per-thread stencil tile of level grid (nx, ny, nz).
+struct Cell {
+  int x, y, za, zb;
+  bool active;
+};
+
+__device__ __forceinline__ Cell cell_of(const Rec& rc, int lb, int nx, int ny, int nz) {
+  Cell c;
+  const int tx = lb % rc.tiles_x;
+  const int t2 = lb / rc.tiles_x;
+  const int ty = t2 % rc.tiles_y;
+  const int tz = t2 / rc.tiles_y;
+  const int rows = kBlock / rc.lx;
+  const int ix = threadIdx.x % rc.lx, iy = threadIdx.x / rc.lx;
+  c.x = tx * rc.lx + ix;
+  c.y = ty * rows + iy;
+  c.za = tz * rc.bz;
+  c.zb = min(c.za + rc.bz, nz);
+  c.active = iy < rows && c.x < nx && c.y < ny;
+  return c;
+}
+
+// Smoothing sweep at level l.  FIRST: z_out = w D^-1 r (zero initial guess);
+// else z_out = z_in + w D^-1 (r - L z_in).  PROLONG: z_in is first corrected
+// with the coarse correction zc (injection from level l+1).  DOT: partials of
+// (r, z_out) for the preconditioned CG (level 0, last sweep).
+template <typename T, bool FIRST, bool PROLONG, bool DOT>
+__global__ __launch_bounds__(kBlock) void k_mg_smooth(const SysDev* __restrict__ sds,
+                                                      const Rec* __restrict__ recs,
+                                                      const int* __restrict__ b2r, int l,
+                                                      int zin, int zout) {
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const Rec rc = recs[b2r[b]];
+  const SysDev& s = sds[rc.sys];
+  if (s.sc->done) return;
+  const MgLevel& Lv = s.lev[l];
+  const LevelView<T> V(Lv);
+  const int lb = b - rc.blk0;
+  const Cell c = cell_of(rc, lb, V.nx, V.ny, V.nz);
+  const T* __restrict__ r = (const T*)Lv.r;
+  const T* __restrict__ zi = (const T*)Lv.z[zin];
+  T* __restrict__ zo = (T*)Lv.z[zout];
+  const T w = (T)kOmega;
+  double acc[1] = {0.0};
+  if (c.active) {
+    const MgLevel* Lc = PROLONG ? &s.lev[l + 1] : nullptr;
+    // the coarse correction of the coarsest level sits in z[0], of the
+    // others in z[1] (after their post-smoothing)
+    const T* __restrict__ zc =
+        PROLONG ? (const T*)Lc->z[l + 1 == s.nlev - 1 ? 0 : 1] : nullptr;
+    const int cnx = PROLONG ? Lc->nx : 0, cny = PROLONG ? Lc->ny : 0;
+    auto u = [&](long long i) -> T {
+      if constexpr (FIRST) {
+        return (T)0;
+      } else if constexpr (PROLONG) {
+        const int x = (int)(i % V.nx);
+        const long long t = i / V.nx;
+        const int y = (int)(t % V.ny);
+        const int z = (int)(t / V.ny);
+        return zi[i] + zc[((long long)(z >> 1) * cny + (y >> 1)) * cnx + (x >> 1)];
+      } else {
+        return zi[i];
+      }
+    };
+    for (int z = c.za; z < c.zb; ++z) {
+      const long long i = V.at(c.x, c.y, z);
+      T v;
+      if constexpr (FIRST) {
+        v = w * V.dinv[i] * r[i];
+      } else {
+        v = u(i) + w * V.dinv[i] * (r[i] - V.lap(c.x, c.y, z, u));
+      }
+      zo[i] = v;
+      if constexpr (DOT) acc[0] += (double)r[i] * (double)v;
+    }
+  }
+  if constexpr (DOT) block_partial<1, 0u>(acc, s.partials, rc.nblk, lb);
+}
+
+// Residual of level l restricted to level l+1: r_{l+1}[C] = sum over the
+// 2x2x2 children c of (r - L z)_c.  One thread per coarse cell.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mg_restrict(const SysDev* __restrict__ sds,
+                                                        const Rec* __restrict__ recs,
+                                                        const int* __restrict__ b2r, int l,
+                                                        int zin) {
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const Rec rc = recs[b2r[b]];
+  const SysDev& s = sds[rc.sys];
+  if (s.sc->done) return;
+  const MgLevel& Lf = s.lev[l];
+  const MgLevel& Lc = s.lev[l + 1];
+  const LevelView<T> F(Lf);
+  const int lb = b - rc.blk0;
+  const Cell c = cell_of(rc, lb, Lc.nx, Lc.ny, Lc.nz);
+  if (!c.active) return;
+  const T* __restrict__ r = (const T*)Lf.r;
+  const T* __restrict__ zi = (const T*)Lf.z[zin];
+  T* __restrict__ rcoarse = (T*)Lc.r;
+  auto u = [&](long long i) -> T { return zi[i]; };
+  for (int z = c.za; z < c.zb; ++z) {
+    T sum = 0;
+    for (int dz = 0; dz < 2; ++dz)
+      for (int dy = 0; dy < 2; ++dy)
+        for (int dx = 0; dx < 2; ++dx) {
+          const int fx = 2 * c.x + dx, fy = 2 * c.y + dy, fz = 2 * z + dz;
+          sum += r[F.at(fx, fy, fz)] - F.lap(fx, fy, fz, u);
+        }
+    rcoarse[((long long)z * Lc.ny + c.y) * Lc.nx + c.x] = sum;
+  }
+}
+
+// Coarsest level: `sweeps` damped Jacobi sweeps from z = 0, in LDS, one
+// workgroup per system (levels of at most kCoarseMax cells).
+constexpr int kCoarseMax = 8192;
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mg_coarse(const SysDev* __restrict__ sds,
+                                                      int sweeps) {
+  extern __shared__ unsigned char smem[];
+  const SysDev& s = sds[blockIdx.x];
+  if (s.nlev <= 0 || s.sc->done) return;
+  const MgLevel& Lv = s.lev[s.nlev - 1];
+  const LevelView<T> V(Lv);
+  const int n = (int)Lv.n;
+  T* za = (T*)smem;
+  T* zb = za + n;
+  const T* __restrict__ r = (const T*)Lv.r;
+  const T w = (T)kOmega;
+  for (int i = threadIdx.x; i < n; i += kBlock) za[i] = w * V.dinv[i] * r[i];
+  __syncthreads();
+  for (int k = 1; k < sweeps; ++k) {
+    for (int i = threadIdx.x; i < n; i += kBlock) {
+      const int x = i % V.nx, y = (i / V.nx) % V.ny, z = i / (V.nx * V.ny);
+      auto u = [&](long long j) -> T { return za[j]; };
+      zb[i] = za[i] + w * V.dinv[i] * (r[i] - V.lap(x, y, z, u));
+    }
+    __syncthreads();
+    T* t = za;
+    za = zb;
+    zb = t;
+  }
+  T* out = (T*)Lv.z[0];
+  for (int i = threadIdx.x; i < n; i += kBlock) out[i] = za[i];
+}
+
+// (r, z) of the preconditioned CG: beta = (r,z) / (r,z)_old (0 after a start)
+__global__ __launch_bounds__(kBlock) void k_fin_precond(const SysDev* __restrict__ sds) {
+  const SysDev& s = sds[blockIdx.x];
+  Scalars* sc = s.sc;
+  if (s.nlev <= 0 || sc->done) return;
+  double acc[1];
+  reduce_partials<1, 0u>(acc, s.partials, s.nblk_pc);
+  if (threadIdx.x == 0) {
+    const double rz = acc[0];
+    sc->beta = sc->first ? 0.0 : rz / sc->rho;
+    sc->rho = rz;
+    sc->rz = rz;
+    sc->first = 0;
+    if (!isfinite(rz)) {
+      sc->status = 6;
+      sc->done = 1;
+    }
+  }
+}
+
+// Coarse operator of level l+1 from level l (one thread per coarse cell):
+// coarse face = scale * (sum of the 4 fine faces it covers), m = scale * sum
+// of the 8 children; then D^-1 of the level.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mg_coarsen(MgLevel f, MgLevel c, T scale) {
+  const long long n = c.n;
+  const T* fkx = (const T*)f.kx;
+  const T* fky = (const T*)f.ky;
+  const T* fkz = (const T*)f.kz;
+  const T* fm = (const T*)f.m;
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (long long)gridDim.x * kBlock) {
+    const int x = (int)(i % c.nx);
+    const long long t = i / c.nx;
+    const int y = (int)(t % c.ny);
+    const int z = (int)(t / c.ny);
+    auto F = [&](int fx, int fy, int fz) {
+      return ((long long)fz * f.ny + fy) * f.nx + fx;
+    };
+    T sx = 0, sy = 0, sz = 0, sm = 0;
+    for (int a = 0; a < 2; ++a)
+      for (int bb = 0; bb < 2; ++bb) {
+        sx += fkx[F(2 * x + 1, 2 * y + a, 2 * z + bb)];
+        sy += fky[F(2 * x + a, 2 * y + 1, 2 * z + bb)];
+        sz += fkz[F(2 * x + a, 2 * y + bb, 2 * z + 1)];
+      }
+    if (fm) {
+      for (int d = 0; d < 8; ++d) sm += fm[F(2 * x + (d & 1), 2 * y + ((d >> 1) & 1), 2 * z + (d >> 2))];
+      ((T*)c.m)[i] = scale * sm;
+    }
+    ((T*)c.kx)[i] = scale * sx;
+    ((T*)c.ky)[i] = scale * sy;
+    ((T*)c.kz)[i] = scale * sz;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mg_dinv(MgLevel L) {
+  const T* kx = (const T*)L.kx;
+  const T* ky = (const T*)L.ky;
+  const T* kz = (const T*)L.kz;
+  const T* m = (const T*)L.m;
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < L.n;
+       i += (long long)gridDim.x * kBlock) {
+    const int x = (int)(i % L.nx);
+    const long long t = i / L.nx;
+    const int y = (int)(t % L.ny);
+    const int z = (int)(t / L.ny);
+    const long long pl = (long long)L.nx * L.ny;
+    T d = kx[i] + kx[(long long)z * pl + (long long)y * L.nx + wrapm(x, L.nx)] + ky[i] +
+          ky[(long long)z * pl + (long long)wrapm(y, L.ny) * L.nx + x] + kz[i] +
+          kz[(long long)wrapm(z, L.nz) * pl + (long long)y * L.nx + x];
+    if (m) d += m[i];
+    ((T*)L.dinv)[i] = d != (T)0 ? (T)1 / d : (T)0;
   }
 }
 
@@ -784,7 +1061,7 @@ __global__ __launch_bounds__(kBlock) void k_convert_pcg(
     kx[i] = (T)(-sgn * e[2]);
     ky[i] = (T)(-sgn * e[4]);
     kz[i] = (T)(-sgn * e[6]);
-    S[i] = (T)(1.0 / sqrt(fabs(e[0])));
+    if (S) S[i] = (T)(1.0 / sqrt(fabs(e[0])));
     if (m) {
       const double rs = e[0] + e[1] + e[2] + e[3] + e[4] + e[5] + e[6];
       m[i] = fabs(rs) > 1e-12 * fabs(e[0]) ? (T)(sgn * rs) : (T)0;
@@ -925,12 +1202,60 @@ __global__ __launch_bounds__(kBlock) void k_fill(T* __restrict__ p, long long n,
 #define APH_LAUNCH(kern, grid, stream, ...) \
   hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, stream, __VA_ARGS__)
 
-int iter_kernels(int method) { return method == kConjugate ? 4 : (method == kJacobi ? 2 : 8); }
+int iter_kernels(int method, int pc) {
+  return method == kConjugate ? 4 : (method == kJacobi ? 2 : (pc ? 9 : 8));
+}
+
+// One V-cycle over every system of the launch (levels of systems with fewer
+// levels drop out of the deeper launches), then (r, z) and beta.
+template <typename T>
+static void launch_vcycle(const LaunchSet& L, hipStream_t st) {
+  const int top = L.nlev_max - 1;  // deepest non-coarsest level + 1
+  const int nu = L.nu;
+  for (int l = 0; l < top; ++l) {
+    APH_LAUNCH((k_mg_smooth<T, true, false, false>), L.nlf[l], st, L.sds, L.rec_lf[l],
+               L.b2r_lf[l], l, 0, 0);
+    int cur = 0;
+    for (int k = 1; k < nu; ++k, cur ^= 1) {
+      APH_LAUNCH((k_mg_smooth<T, false, false, false>), L.nlf[l], st, L.sds, L.rec_lf[l],
+                 L.b2r_lf[l], l, cur, cur ^ 1);
+    }
+    APH_LAUNCH(k_mg_restrict<T>, L.nlc[l], st, L.sds, L.rec_lc[l], L.b2r_lc[l], l, cur);
+  }
+  hipLaunchKernelGGL(k_mg_coarse<T>, dim3(L.ns), dim3(kBlock), L.coarse_lds, st, L.sds,
+                     L.coarse_sweeps);
+  for (int l = top - 1; l >= 0; --l) {
+    int cur = (nu - 1) & 1;  // z index after the pre-smoothing
+    const bool last0 = l == 0 && nu == 1;
+    if (last0) {
+      APH_LAUNCH((k_mg_smooth<T, false, true, true>), L.nlf[l], st, L.sds, L.rec_lf[l],
+                 L.b2r_lf[l], l, cur, cur ^ 1);
+    } else {
+      APH_LAUNCH((k_mg_smooth<T, false, true, false>), L.nlf[l], st, L.sds, L.rec_lf[l],
+                 L.b2r_lf[l], l, cur, cur ^ 1);
+    }
+    cur ^= 1;
+    for (int k = 1; k < nu; ++k, cur ^= 1) {
+      if (l == 0 && k == nu - 1) {
+        APH_LAUNCH((k_mg_smooth<T, false, false, true>), L.nlf[l], st, L.sds, L.rec_lf[l],
+                   L.b2r_lf[l], l, cur, cur ^ 1);
+      } else {
+        APH_LAUNCH((k_mg_smooth<T, false, false, false>), L.nlf[l], st, L.sds, L.rec_lf[l],
+                   L.b2r_lf[l], l, cur, cur ^ 1);
+      }
+    }
+  }
+  APH_LAUNCH(k_fin_precond, L.ns, st, L.sds);
+}
 
 template <typename T, int VX>
 static void launch_pcg_kernel(const LaunchSet& L, hipStream_t st, int j) {
   switch (j) {
-    case 0: APH_LAUNCH((k_pcg_spmv<T, VX>), L.nst, st, L.sds, L.rec_st, L.b2r_st); break;
+    case 0:
+      if (L.pc) APH_LAUNCH((k_pcg_spmv<T, VX, false>), L.nst, st, L.sds, L.rec_st, L.b2r_st);
+      else APH_LAUNCH((k_pcg_spmv<T, VX, true>), L.nst, st, L.sds, L.rec_st, L.b2r_st);
+      break;
+    case 8: launch_vcycle<T>(L, st); break;
     case 1: APH_LAUNCH(k_fin_pcg_spmv, L.ns, st, L.sds); break;
     case 2: APH_LAUNCH((k_pcg_update<T, VX>), L.nsm, st, L.sds, L.rec_sm, L.b2r_sm); break;
     case 3: APH_LAUNCH(k_fin_pcg_update, L.ns, st, L.sds); break;
@@ -965,7 +1290,7 @@ hipError_t launch_kernel(int method, int fp32, int vx, const LaunchSet& L, hipSt
 }
 
 hipError_t launch_iter(int method, int fp32, int vx, const LaunchSet& L, hipStream_t st) {
-  for (int j = 0; j < iter_kernels(method); ++j) {
+  for (int j = 0; j < iter_kernels(method, L.pc); ++j) {
     const hipError_t e = launch_kernel(method, fp32, vx, L, st, j);
     if (e != hipSuccess) return e;
   }
@@ -977,13 +1302,31 @@ hipError_t launch_start(int method, int fp32, int vx, const LaunchSet& L, hipStr
     APH_LAUNCH(k_ref_init, L.nst, st, L.sds, L.rec_st, L.b2r_st);
     APH_LAUNCH(k_fin_ref_init, L.ns, st, L.sds);
   } else if (method == kPcg) {
-    // the start is a reliable update (ru = 1): x64 = x0, r^ = S (b - A x0)
-    for (int j = 4; j < 8; ++j) {
+    // the start is a reliable update (ru = 1): x64 = x0, r^ = S (b - A x0),
+    // followed by z = M^-1 r^ with the multigrid preconditioner
+    for (int j = 4; j < iter_kernels(method, L.pc); ++j) {
       const hipError_t e = launch_kernel(method, fp32, vx, L, st, j);
       if (e != hipSuccess) return e;
     }
   }
   return hipGetLastError();
+}
+
+static int grid_for(long long n);
+
+template <typename T>
+static hipError_t mg_setup(const MgLevel* lv, int nlev, hipStream_t st) {
+  for (int l = 0; l < nlev; ++l) {
+    if (l > 0) {
+      APH_LAUNCH(k_mg_coarsen<T>, grid_for(lv[l].n), st, lv[l - 1], lv[l], (T)0.5);
+    }
+    APH_LAUNCH(k_mg_dinv<T>, grid_for(lv[l].n), st, lv[l]);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_mg_setup(int fp32, const MgLevel* levels, int nlev, hipStream_t st) {
+  return fp32 ? mg_setup<float>(levels, nlev, st) : mg_setup<double>(levels, nlev, st);
 }
 
 hipError_t launch_true_res_f7(const LaunchSet& L, hipStream_t st) {

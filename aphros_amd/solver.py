@@ -15,7 +15,7 @@ class Poisson:
 
     def __init__(self, shape, method=L.PCG, precision=L.FP64, resmode=L.RES_REL_TRUE,
                  tol=1e-8, miniter=0, maxiter=100000, h3=None, nullspace=L.NULL_AUTO,
-                 delta=0.0, check_every=0, device=0, mem_budget=0):
+                 delta=0.0, check_every=0, device=0, mem_budget=0, precond=L.PRECOND_JACOBI):
         nz, ny, nx = shape
         self.shape = (nz, ny, nx)
         d = L.Desc()
@@ -25,7 +25,7 @@ class Poisson:
         d.method, d.precision, d.resmode = method, precision, resmode
         d.tol, d.miniter, d.maxiter = tol, miniter, maxiter
         d.nullspace, d.delta, d.check_every = nullspace, delta, check_every
-        d.device, d.mem_budget = device, mem_budget
+        d.device, d.mem_budget, d.precond = device, mem_budget, precond
         self.desc = d
         self.h = C.c_void_p()
         L.check(L.lib().aphpois_create(C.byref(d), C.byref(self.h)))
@@ -68,8 +68,8 @@ class Poisson:
         L.check(L.lib().aphpois_set_profiling(self.h, int(on)))
 
     def kernel_times(self):
-        avg = (C.c_double * 8)()
-        cnt = (C.c_longlong * 8)()
+        avg = (C.c_double * 16)()
+        cnt = (C.c_longlong * 16)()
         nk = C.c_int()
         L.check(L.lib().aphpois_kernel_times(self.h, avg, cnt, C.byref(nk)))
         return [(avg[j], cnt[j]) for j in range(nk.value)]

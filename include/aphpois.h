@@ -61,6 +61,17 @@ enum {
   APHPOIS_RES_REL_TRUE = 2
 };
 
+/* ---- preconditioner of PCG ---- */
+enum {
+  /* diagonal (Jacobi) scaling; one streaming SpMV per iteration */
+  APHPOIS_PRECOND_JACOBI = 0,
+  /* one symmetric multigrid V-cycle per iteration (2x2x2 aggregation with
+   * the sum of the covered face conductances, damped-Jacobi smoothing);
+   * every grid edge must have enough factors of 2 to coarsen to <= 8192
+   * cells (FP64: 4096) */
+  APHPOIS_PRECOND_MG = 1
+};
+
 /* ---- nullspace handling for PCG ---- */
 enum {
   APHPOIS_NULL_AUTO = 0, /* remove the mean when the system is singular */
@@ -82,6 +93,7 @@ typedef struct aphpois_desc {
   int check_every;  /* iterations between host polls, 0 = auto */
   int device;       /* HIP device ordinal */
   long long mem_budget; /* bytes of device memory this handle may use, 0 = no limit */
+  int precond;      /* PCG preconditioner: APHPOIS_PRECOND_* */
 } aphpois_desc;
 
 typedef struct aphpois_info {
@@ -155,9 +167,10 @@ int aphpois_set_profiling(aphpois* h, int on);
 /* Average device time (ms) and count of each iteration kernel over the
  * iterations that ran since profiling was switched on.  Kernel order (each
  * reducing kernel is followed by its finalize kernel): PCG {spmv, fin,
- * update, fin, ru_accum, fin, ru_resid, fin}; CONJUGATE {spmv, fin, update,
- * fin}; JACOBI {sweep, fin}.  avg_ms and count hold 8 entries; *nkernels
- * receives the number used. */
+ * update, fin, ru_accum, fin, ru_resid, fin} and, with the multigrid
+ * preconditioner, {V-cycle} (all its launches together); CONJUGATE {spmv,
+ * fin, update, fin}; JACOBI {sweep, fin}.  avg_ms and count hold 16 entries;
+ * *nkernels receives the number used. */
 int aphpois_kernel_times(aphpois* h, double* avg_ms, long long* count, int* nkernels);
 /* Launch geometry of the last solve: workgroups of the stencil and streaming
  * launches and the x vector width. */
@@ -182,6 +195,7 @@ typedef struct aphpois_batch_desc {
   int device;
   long long mem_budget; /* bytes, 0 = no limit */
   int max_systems;      /* slot capacity, 0 = 1024 */
+  int precond;          /* APHPOIS_PRECOND_* */
 } aphpois_batch_desc;
 
 int aphpois_batch_create(const aphpois_batch_desc* d, aphpois_batch** out);
@@ -227,6 +241,7 @@ typedef struct aphpois_server_desc {
   int device;
   long long mem_budget; /* admission budget in bytes of device memory, 0 = none */
   int max_active;       /* resident systems at most, 0 = 1024 */
+  int precond;          /* APHPOIS_PRECOND_* */
 } aphpois_server_desc;
 
 typedef struct aphpois_request {

@@ -26,9 +26,9 @@ def _sys(n, ns, rmin, rmax, ratio, seed):
     return e.reshape(n, n, n, 8)
 
 
-def _single(E, precision=L.MIXED, x0=None, tol=1e-8):
+def _single(E, precision=L.MIXED, x0=None, tol=1e-8, precond=L.PRECOND_JACOBI):
     s = Poisson(E.shape[:3], method=L.PCG, precision=precision, resmode=L.RES_REL_TRUE,
-                tol=tol, maxiter=50000)
+                tol=tol, maxiter=50000, precond=precond)
     s.set_system(E)
     x, info = s.solve(x0)
     s.close()
@@ -36,14 +36,15 @@ def _single(E, precision=L.MIXED, x0=None, tol=1e-8):
 
 
 @pytest.mark.parametrize("precision", [L.MIXED, L.FP64])
-def test_batch_equals_single(precision):
+@pytest.mark.parametrize("precond", [L.PRECOND_JACOBI, L.PRECOND_MG])
+def test_batch_equals_single(precision, precond):
     systems = [_sys(*p) for p in SYSTEMS]
-    b = Batch(precision=precision, tol=1e-8, maxiter=50000)
+    b = Batch(precision=precision, tol=1e-8, maxiter=50000, precond=precond)
     slots = [b.add(E) for E in systems]
     b.solve()
     for E, sl in zip(systems, slots):
         xb, ib = b.get(sl)
-        xs, is_ = _single(E, precision)
+        xs, is_ = _single(E, precision, precond=precond)
         print(f"n={E.shape[0]} iter={ib['iter']} relres={ib['true_relres']:.2e}", flush=True)
         assert ib["converged"] == 1 and ib["iter"] == is_["iter"]
         assert np.array_equal(xb, xs)

@@ -79,10 +79,11 @@ def test_conjugate_sphere_history(c1):
 
 
 @pytest.mark.parametrize("precision", [L.FP64, L.MIXED])
-def test_pcg_c1_true_residual(c1, precision):
-    _say(f"pcg c1 precision={precision}")
+@pytest.mark.parametrize("precond", [L.PRECOND_JACOBI, L.PRECOND_MG])
+def test_pcg_c1_true_residual(c1, precision, precond):
+    _say(f"pcg c1 precision={precision} precond={precond}")
     s = Poisson(c1.shape[:3], method=L.PCG, precision=precision, resmode=L.RES_REL_TRUE,
-                tol=1e-8, maxiter=20000)
+                tol=1e-8, maxiter=20000, precond=precond)
     s.set_system(c1)
     x, info = s.solve(None)
     assert info["converged"] == 1 and info["status"] == 0
@@ -96,11 +97,12 @@ def test_pcg_c1_true_residual(c1, precision):
     assert err < 1e-5
 
 
-def test_pcg_tlinear_negative_diagonal(tlin32):
+@pytest.mark.parametrize("precond", [L.PRECOND_JACOBI, L.PRECOND_MG])
+def test_pcg_tlinear_negative_diagonal(tlin32, precond):
     """e0 < 0 convention (t.linear / SolvePoisson)."""
     E, ue, h = tlin32
     s = Poisson(E.shape[:3], method=L.PCG, precision=L.MIXED, resmode=L.RES_REL_TRUE,
-                tol=1e-10, maxiter=5000, h3=h ** 3)
+                tol=1e-10, maxiter=5000, h3=h ** 3, precond=precond)
     s.set_system(E)
     x, info = s.solve(None)
     assert info["converged"] == 1
