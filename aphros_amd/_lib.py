@@ -104,6 +104,7 @@ SIGNATURES = {
     "aphpois_launch_info": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                       C.POINTER(C.c_int)]),
     "aphpois_set_tile": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "aphpois_precond_apply": (C.c_int, [_vp, _dp, _dp]),
     "aphpois_batch_set_tile": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int]),
     "aphpois_batch_set_system_tile": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "aphpois_batch_last_seconds": (C.c_double, [_vp]),

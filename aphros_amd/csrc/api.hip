@@ -169,6 +169,12 @@ int aphpois_set_tile(aphpois* h, int lx, int bz, int vx, int sm) {
   return APHPOIS_OK;
 }
 
+int aphpois_precond_apply(aphpois* h, const double* r, double* z) {
+  APH_CHECK_H(h);
+  if (!r || !z) return set_err(APHPOIS_EINVAL, "r or z is NULL");
+  APH_RET(*h, h->eng.precond_apply(h->slot, r, z));
+}
+
 int aphpois_destroy(aphpois* h) {
   delete h;
   return APHPOIS_OK;

@@ -99,6 +99,9 @@ class Engine {
   // b += amp * max|b| * (eta - mean eta), eta uniform in [-1, 1] from the
   // splitmix64 stream `key`.
   int perturb_rhs(int slot, double amp, unsigned long long key);
+  // z = M r with the multigrid preconditioner of the resident system (dense
+  // host arrays, x fastest); leaves any solve state of the slot as it was.
+  int precond_apply(int slot, const double* r, double* z);
   long long bytes() const;
   int nslots() const { return (int)slots_.size(); }
   System* sys(int slot) {
@@ -127,7 +130,10 @@ class Engine {
       prof_n_[j] = 0;
     }
   }
-  int prof_kernels() const { return iter_kernels(method_, pc_ ? 1 : 0); }
+  // iteration kernels, plus (multigrid) [9] level-0 sweep, [10] level-0
+  // restriction, [11] coarser levels per V-cycle, [12] coarsest solve,
+  // [13] [14] [15] level-0 pre-sweep, first and second post-sweep
+  int prof_kernels() const { return pc_ ? 16 : iter_kernels(method_, 0); }
   double prof_avg_ms(int j) const { return prof_n_[j] ? prof_ms_[j] / prof_n_[j] : 0.0; }
   long long prof_count(int j) const { return prof_n_[j]; }
 
@@ -183,6 +189,8 @@ class Engine {
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
   bool profile_ = false;
   std::vector<hipEvent_t> pev_;
+  std::vector<hipEvent_t> pev_mg_;  // V-cycle launches (profiling)
+  std::vector<int> mg_kinds_;
   double prof_ms_[16] = {};
   long long prof_n_[16] = {};
   double last_run_ms_ = 0;

@@ -84,6 +84,7 @@ Engine::~Engine() {
   if (ev0_) (void)hipEventDestroy(ev0_);
   if (ev1_) (void)hipEventDestroy(ev1_);
   for (hipEvent_t e : pev_) (void)hipEventDestroy(e);
+  for (hipEvent_t e : pev_mg_) (void)hipEventDestroy(e);
   if (st_) (void)hipStreamDestroy(st_);
 }
 
@@ -560,6 +561,18 @@ int Engine::build_tables() {
   std::vector<std::vector<int>> blf(kMaxLevels), blc(kMaxLevels);
   int nlev_max = 0;
   long long coarse_max = 0;
+  // x vector width of each level's sweeps: the widest that divides the level's
+  // nx of every system that smooths on that level
+  int vxl[kMaxLevels];
+  for (int l = 0; l < kMaxLevels; ++l) vxl[l] = fp32_ ? 4 : 2;
+  if (pc_) {
+    for (int i : launch_slots_) {
+      const System& s = *slots_[i];
+      for (int l = 0; l + 1 < (int)s.lev.size(); ++l) {
+        while (vxl[l] > 1 && s.lev[l].nx % vxl[l] != 0) vxl[l] /= 2;
+      }
+    }
+  }
   for (int k = 0; k < ns; ++k) {
     System& s = *slots_[launch_slots_[k]];
     // The tiling of a system depends on that system only, so its reduction
@@ -599,7 +612,7 @@ int Engine::build_tables() {
       for (int l = 0; l + 1 < nl; ++l) {
         const MgLevel& F = s.lev[l];
         const MgLevel& C = s.lev[l + 1];
-        Rec f = make_stencil_rec(F.nx, F.ny, F.nz, 1,
+        Rec f = make_stencil_rec(F.nx, F.ny, F.nz, vxl[l],
                                  std::min(3072LL, std::max(1LL, cdiv(F.n, 2048))), none);
         f.sys = k;
         f.blk0 = (int)blf[l].size();
@@ -689,7 +702,7 @@ int Engine::build_tables() {
   L_.ns = ns;
   L_.pc = pc_ ? 1 : 0;
   L_.nlev_max = nlev_max;
-  L_.nu = 2;
+  for (int l = 0; l < kMaxLevels; ++l) L_.vx_l[l] = vxl[l];
   L_.coarse_sweeps = 40;
   L_.coarse_lds = (int)(2 * coarse_max * (fp32_ ? 4 : 8));
   for (int l = 0; l < kMaxLevels; ++l) {
@@ -786,6 +799,57 @@ int Engine::perturb_rhs(int slot, double amp, unsigned long long key) {
   return APHPOIS_OK;
 }
 
+int Engine::precond_apply(int slot, const double* r, double* z) {
+  System* sp = sys(slot);
+  if (!sp) return fail(APHPOIS_EINVAL, "no such slot");
+  if (method_ != kPcg || !pc_) return fail(APHPOIS_EINVAL, "needs method PCG with multigrid");
+  if (!sp->has_system) return fail(APHPOIS_ESTATE, "no system set");
+  System& s = *sp;
+  const long long n = s.n;
+  const int es = fp32_ ? 4 : 8;
+  // only this system in the launch tables
+  std::vector<char> was_active(slots_.size(), 0);
+  for (size_t i = 0; i < slots_.size(); ++i) {
+    if (slots_[i]) {
+      was_active[i] = slots_[i]->active;
+      slots_[i]->active = (int)i == slot;
+    }
+  }
+  int rc = build_tables();
+  auto restore = [&]() {
+    for (size_t i = 0; i < slots_.size(); ++i)
+      if (slots_[i]) slots_[i]->active = was_active[i];
+    dirty_ = true;
+  };
+  if (rc != APHPOIS_OK) {
+    restore();
+    return rc;
+  }
+  const double* d = nullptr;
+  rc = upload_strided(r, false, 0, s.nx, (long long)s.nx * s.ny, s.nx, s.ny, s.nz, 1, &d);
+  Scalars sc{};
+  sc.pc = 1;
+  sc.first = 1;
+  sc.method = method_;
+  hipError_t e = hipSuccess;
+  if (rc == APHPOIS_OK) {
+    e = launch_cast(d, 8, const_cast<void*>(s.lev[0].r), es, n, st_);
+    if (e == hipSuccess) e = hipStreamSynchronize(st_);
+    if (e == hipSuccess) e = hipMemcpy(d_sc_ + slot, &sc, sizeof(Scalars), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_kernel(method_, fp32_, vx_, L_, st_, 8);
+    // z of level 0 back through the staging buffer (free again after the upload)
+    if (e == hipSuccess) e = launch_cast(s.lev[0].z[1], es, staging_, 8, n, st_);
+    if (e == hipSuccess) e = hipMemcpyAsync(z, staging_, n * 8, hipMemcpyDeviceToHost, st_);
+    if (e == hipSuccess) e = hipStreamSynchronize(st_);
+    // the device scalars of the slot back to their last pulled state
+    if (e == hipSuccess)
+      e = hipMemcpy(d_sc_ + slot, h_sc_ + slot, sizeof(Scalars), hipMemcpyHostToDevice);
+  }
+  restore();
+  if (rc != APHPOIS_OK) return rc;
+  return hip(e, "precond_apply");
+}
+
 int Engine::step(int max_replays, std::vector<int>* finished) {
   finished->clear();
   if (method_ != kPcg) return fail(APHPOIS_EINVAL, "step() needs method PCG");
@@ -840,10 +904,23 @@ int Engine::run() {
       // same kernels as the graph, launched one by one with an event before
       // each kernel (events cannot be timed inside a replayed graph)
       const int nk = iter_kernels(method_, pc_ ? 1 : 0);
+      const int nv = pc_ ? vcycle_launches(L_) + 1 : 0;
+      while ((int)pev_mg_.size() < graph_k_ * nv) {
+        hipEvent_t e;
+        APH_HIP(hipEventCreate(&e));
+        pev_mg_.push_back(e);
+      }
+      mg_kinds_.resize(nv);
       for (int sl = 0; sl < graph_k_; ++sl) {
         for (int j = 0; j < nk; ++j) {
           APH_HIP(hipEventRecord(pev_[(size_t)sl * nk + j], st_));
-          APH_HIP(launch_kernel(method_, fp32_, vx_, L_, st_, j));
+          if (j == 8 && pc_) {
+            int nl = 0;
+            APH_HIP(launch_vcycle_profiled(fp32_, L_, st_, &pev_mg_[(size_t)sl * nv],
+                                           mg_kinds_.data(), &nl));
+          } else {
+            APH_HIP(launch_kernel(method_, fp32_, vx_, L_, st_, j));
+          }
         }
       }
       APH_HIP(hipEventRecord(pev_[(size_t)graph_k_ * nk], st_));
@@ -869,6 +946,41 @@ int Engine::run() {
                                       pev_[(size_t)sl * nk + j + 1]));
           prof_ms_[j] += ms;
           prof_n_[j] += 1;
+        }
+        if (pc_) {
+          // V-cycle launches by kind: [9] level-0 sweep (per launch, all
+          // three), [10] level-0 restriction, [11] coarser levels (per
+          // V-cycle), [12] coarsest solve, [13] [14] [15] level-0 pre-sweep,
+          // first and second post-sweep
+          const int nv = vcycle_launches(L_) + 1;
+          double coarse = 0;
+          for (int i = 0; i + 1 < nv; ++i) {
+            float ms = 0;
+            APH_HIP(hipEventElapsedTime(&ms, pev_mg_[(size_t)sl * nv + i],
+                                        pev_mg_[(size_t)sl * nv + i + 1]));
+            const int kd = mg_kinds_[i];
+            int slot2 = -1, slot1 = -1;
+            if (kd == 0 || kd == 5 || kd == 6) {
+              slot1 = 9;
+              slot2 = kd == 0 ? 13 : (kd == 5 ? 14 : 15);
+            } else if (kd == 1) {
+              slot1 = 10;
+            } else if (kd == 3) {
+              slot1 = 12;
+            } else if (kd == 2) {
+              coarse += ms;
+            }
+            if (slot1 >= 0) {
+              prof_ms_[slot1] += ms;
+              prof_n_[slot1] += 1;
+            }
+            if (slot2 >= 0) {
+              prof_ms_[slot2] += ms;
+              prof_n_[slot2] += 1;
+            }
+          }
+          prof_ms_[11] += coarse;
+          prof_n_[11] += 1;
         }
       }
     }

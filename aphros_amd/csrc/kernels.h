@@ -21,9 +21,9 @@ struct LaunchSet {
   // over the systems that coarsen below l
   int pc = 0;
   int nlev_max = 0;
-  int nu = 2;          // smoothing sweeps before and after the coarse correction
   int coarse_sweeps = 40;
   int coarse_lds = 0;  // dynamic LDS bytes of the coarsest-level kernel
+  int vx_l[kMaxLevels] = {};  // cells per thread in x of the level-l sweeps
   const Rec* rec_lf[kMaxLevels] = {};
   const int* b2r_lf[kMaxLevels] = {};
   int nlf[kMaxLevels] = {};
@@ -40,6 +40,11 @@ int iter_kernels(int method, int pc = 0);
 // Coarse operators and inverse diagonals of levels 0..nlev-1 (level 0 holds
 // the fine conductances already).
 hipError_t launch_mg_setup(int fp32, const MgLevel* levels, int nlev, hipStream_t st);
+// The V-cycle with an event recorded before each launch (ev needs
+// vcycle_launches(L) + 1 entries; kinds receives the launch kinds).
+hipError_t launch_vcycle_profiled(int fp32, const LaunchSet& L, hipStream_t st, hipEvent_t* ev,
+                                  int* kinds, int* nlaunch);
+int vcycle_launches(const LaunchSet& L);
 hipError_t launch_kernel(int method, int fp32, int vx, const LaunchSet& L, hipStream_t st,
                          int j);
 hipError_t launch_iter(int method, int fp32, int vx, const LaunchSet& L, hipStream_t st);
@@ -63,6 +68,9 @@ hipError_t launch_scatter(const double* src, int nx, int ny, int nz, double* dst
 hipError_t launch_sumsq(const double* v, long long n, double* partials, unsigned* counter,
                         double* out, hipStream_t st);
 hipError_t launch_fill(void* p, long long n, int elem_bytes, hipStream_t st);
+// dst = (type of dst) src, element sizes 8 -> 4, 4 -> 8 or 8 -> 8 bytes
+hipError_t launch_cast(const void* src, int src_bytes, void* dst, int dst_bytes, long long n,
+                       hipStream_t st);
 // b += amp * max|b| * (eta - mean eta), eta from the splitmix64 stream `key`;
 // partials needs 2 * 1024 + 2 doubles
 hipError_t launch_perturb(double* b, long long n, unsigned long long key, double amp,

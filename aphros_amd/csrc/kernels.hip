@@ -503,70 +503,177 @@ __device__ __forceinline__ Cell cell_of(const Rec& rc, int lb, int nx, int ny, i
   return c;
 }
 
-// Smoothing sweep at level l.  FIRST: z_out = w D^-1 r (zero initial guess);
-// else z_out = z_in + w D^-1 (r - L z_in).  PROLONG: z_in is first corrected
-// with the coarse correction zc (injection from level l+1).  DOT: partials of
-// (r, z_out) for the preconditioned CG (level 0, last sweep).
-template <typename T, bool FIRST, bool PROLONG, bool DOT>
-__global__ __launch_bounds__(kBlock) void k_mg_smooth(const SysDev* __restrict__ sds,
-                                                      const Rec* __restrict__ recs,
-                                                      const int* __restrict__ b2r, int l,
-                                                      int zin, int zout) {
+// Smoothing sweep (the bandwidth-bound levels): each thread owns VX
+// consecutive cells of a row and marches through z with u of planes z-1, z,
+// z+1 in registers; the tile's u plane is shared through LDS for the x and y
+// neighbours (edge threads recompute out-of-tile neighbours from memory).
+//   MODE 0: u = z_in
+//   MODE 1: u = w D^-1 r (the zero-guess first sweep, fused)
+//   MODE 2: u = z_in + P z_c (coarse correction, fused)
+// and z_out = u + w D^-1 (r - L u); DOT: partials of (r, z_out).
+// L0: the launch on level 0 (a separate instance, so that profiles keep the
+// bandwidth-bound fine-grid sweeps apart from the coarse levels).
+template <typename T, int VX, int MODE, bool DOT, bool L0>
+__global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ sds,
+                                                     const Rec* __restrict__ recs,
+                                                     const int* __restrict__ b2r, int l_arg,
+                                                     int zin, int zout) {
+  const int l = L0 ? 0 : l_arg;
+  using V = vec_t<T, VX>;
+  __shared__ T su[2][kBlock * VX];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   const Rec rc = recs[b2r[b]];
   const SysDev& s = sds[rc.sys];
   if (s.sc->done) return;
   const MgLevel& Lv = s.lev[l];
-  const LevelView<T> V(Lv);
+  const int nx = Lv.nx, ny = Lv.ny, nz = Lv.nz;
   const int lb = b - rc.blk0;
-  const Cell c = cell_of(rc, lb, V.nx, V.ny, V.nz);
+  // tile of the level grid
+  const int tix = lb % rc.tiles_x;
+  const int t2 = lb / rc.tiles_x;
+  const int tiy = t2 % rc.tiles_y;
+  const int tiz = t2 / rc.tiles_y;
+  const int lx = rc.lx;
+  const int ty = kBlock / lx;
+  const int ix = threadIdx.x % lx, iy = threadIdx.x / lx;
+  const int x0 = (tix * lx + ix) * VX;
+  const int y = tiy * ty + iy;
+  const int za = tiz * rc.bz, zb = min(za + rc.bz, nz);
+  const bool active = iy < ty && x0 < nx && y < ny;
+  const int tx = lx * VX;
+  const int lofs = iy * tx + ix * VX;
+  const bool xm_in = ix > 0;
+  const bool xp_in = ix + 1 < lx && x0 + VX < nx;
+  const bool ym_in = iy > 0;
+  const bool yp_in = iy + 1 < ty && y + 1 < ny;
+  const T* __restrict__ kx = (const T*)Lv.kx;
+  const T* __restrict__ ky = (const T*)Lv.ky;
+  const T* __restrict__ kz = (const T*)Lv.kz;
+  const T* __restrict__ mm = (const T*)Lv.m;
+  const T* __restrict__ dinv = (const T*)Lv.dinv;
   const T* __restrict__ r = (const T*)Lv.r;
   const T* __restrict__ zi = (const T*)Lv.z[zin];
   T* __restrict__ zo = (T*)Lv.z[zout];
-  const T w = (T)kOmega;
-  double acc[1] = {0.0};
-  if (c.active) {
-    const MgLevel* Lc = PROLONG ? &s.lev[l + 1] : nullptr;
-    // the coarse correction of the coarsest level sits in z[0], of the
-    // others in z[1] (after their post-smoothing)
-    const T* __restrict__ zc =
-        PROLONG ? (const T*)Lc->z[l + 1 == s.nlev - 1 ? 0 : 1] : nullptr;
-    const int cnx = PROLONG ? Lc->nx : 0, cny = PROLONG ? Lc->ny : 0;
-    auto u = [&](long long i) -> T {
-      if constexpr (FIRST) {
-        return (T)0;
-      } else if constexpr (PROLONG) {
-        const int x = (int)(i % V.nx);
-        const long long t = i / V.nx;
-        const int y = (int)(t % V.ny);
-        const int z = (int)(t / V.ny);
-        return zi[i] + zc[((long long)(z >> 1) * cny + (y >> 1)) * cnx + (x >> 1)];
-      } else {
-        return zi[i];
-      }
-    };
-    for (int z = c.za; z < c.zb; ++z) {
-      const long long i = V.at(c.x, c.y, z);
-      T v;
-      if constexpr (FIRST) {
-        v = w * V.dinv[i] * r[i];
-      } else {
-        v = u(i) + w * V.dinv[i] * (r[i] - V.lap(c.x, c.y, z, u));
-      }
-      zo[i] = v;
-      if constexpr (DOT) acc[0] += (double)r[i] * (double)v;
-    }
+  const T* __restrict__ zc = nullptr;
+  int cnx = 0, cny = 0;
+  if constexpr (MODE == 2) {
+    const MgLevel& Lc = s.lev[l + 1];
+    zc = (const T*)Lc.z[l + 1 == s.nlev - 1 ? 0 : 1];
+    cnx = Lc.nx;
+    cny = Lc.ny;
   }
-  if constexpr (DOT) block_partial<1, 0u>(acc, s.partials, rc.nblk, lb);
+  const T w = (T)kOmega;
+  const long long pl = (long long)nx * ny;
+  const long long row = (long long)y * nx;
+  const int ym = wrapm(y, ny), yp = wrapp(y, ny);
+  const int xm = x0 == 0 ? nx - 1 : x0 - 1;
+  const int xp = x0 + VX == nx ? 0 : x0 + VX;
+  // u at (x, yy, zz) for VX cells / for one cell
+  auto uv = [&](int xx, int yy, int zz) -> V {
+    const long long i = (long long)zz * pl + (long long)yy * nx + xx;
+    if constexpr (MODE == 1) {
+      return w * ldv<T, VX>(dinv, i) * ldv<T, VX>(r, i);
+    } else if constexpr (MODE == 2) {
+      V u = ldv<T, VX>(zi, i);
+      const long long cb = ((long long)(zz >> 1) * cny + (yy >> 1)) * cnx;
+#pragma unroll
+      for (int v = 0; v < VX; ++v) u[v] += zc[cb + ((xx + v) >> 1)];
+      return u;
+    } else {
+      return ldv<T, VX>(zi, i);
+    }
+  };
+  auto u1 = [&](int xx, int yy, int zz) -> T {
+    const long long i = (long long)zz * pl + (long long)yy * nx + xx;
+    if constexpr (MODE == 1) {
+      return w * dinv[i] * r[i];
+    } else if constexpr (MODE == 2) {
+      return zi[i] + zc[((long long)(zz >> 1) * cny + (yy >> 1)) * cnx + (xx >> 1)];
+    } else {
+      return zi[i];
+    }
+  };
+  V um = (T)0, uc = (T)0, kzm = (T)0;
+  if (active) {
+    um = uv(x0, y, wrapm(za, nz));
+    kzm = ldv<T, VX>(kz, (long long)wrapm(za, nz) * pl + row + x0);
+    uc = uv(x0, y, za);
+#pragma unroll
+    for (int v = 0; v < VX; ++v) su[0][lofs + v] = uc[v];
+  }
+  __syncthreads();
+  double dot = 0.0;
+  for (int z = za; z < zb; ++z) {
+    const int buf = (z - za) & 1;
+    if (active) {
+      const long long base = (long long)z * pl;
+      const long long ic = base + row + x0;
+      const V un = uv(x0, y, wrapp(z, nz));
+      if (z + 1 < zb) {
+#pragma unroll
+        for (int v = 0; v < VX; ++v) su[buf ^ 1][lofs + v] = un[v];
+      }
+      const T* sp = su[buf];
+      const V kzc = ldv<T, VX>(kz, ic);
+      V lap = kzc * (uc - un) + kzm * (uc - um);
+      kzm = kzc;
+      {
+        V uy;
+        if (ym_in) {
+#pragma unroll
+          for (int v = 0; v < VX; ++v) uy[v] = sp[lofs - tx + v];
+        } else {
+          uy = uv(x0, ym, z);
+        }
+        lap += ldv<T, VX>(ky, base + (long long)ym * nx + x0) * (uc - uy);
+        if (yp_in) {
+#pragma unroll
+          for (int v = 0; v < VX; ++v) uy[v] = sp[lofs + tx + v];
+        } else {
+          uy = uv(x0, yp, z);
+        }
+        lap += ldv<T, VX>(ky, ic) * (uc - uy);
+      }
+      {
+        const T uxm = xm_in ? sp[lofs - 1] : u1(xm, y, z);
+        const T uxp = xp_in ? sp[lofs + VX] : u1(xp, y, z);
+        const V kxc = ldv<T, VX>(kx, ic);
+        const T kxm = kx[base + row + xm];
+#pragma unroll
+        for (int v = 0; v < VX; ++v) {
+          const T up = v + 1 < VX ? uc[(v + 1) % VX] : uxp;
+          const T uw = v > 0 ? uc[(v + VX - 1) % VX] : uxm;
+          const T kw = v > 0 ? kxc[(v + VX - 1) % VX] : kxm;
+          lap[v] += kxc[v] * (uc[v] - up) + kw * (uc[v] - uw);
+        }
+      }
+      if (mm) lap += ldv<T, VX>(mm, ic) * uc;
+      const V rv = ldv<T, VX>(r, ic);
+      const V out = uc + w * ldv<T, VX>(dinv, ic) * (rv - lap);
+      *reinterpret_cast<V*>(zo + ic) = out;
+      if constexpr (DOT) {
+#pragma unroll
+        for (int v = 0; v < VX; ++v) dot += (double)rv[v] * (double)out[v];
+      }
+      um = uc;
+      uc = un;
+    }
+    __syncthreads();
+  }
+  if constexpr (DOT) {
+    double acc[1] = {dot};
+    block_partial<1, 0u>(acc, s.partials, rc.nblk, lb);
+  }
 }
 
 // Residual of level l restricted to level l+1: r_{l+1}[C] = sum over the
 // 2x2x2 children c of (r - L z)_c.  One thread per coarse cell.
-template <typename T>
+template <typename T, bool L0>
 __global__ __launch_bounds__(kBlock) void k_mg_restrict(const SysDev* __restrict__ sds,
                                                         const Rec* __restrict__ recs,
-                                                        const int* __restrict__ b2r, int l,
+                                                        const int* __restrict__ b2r, int l_arg,
                                                         int zin) {
+  const int l = L0 ? 0 : l_arg;
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   const Rec rc = recs[b2r[b]];
   const SysDev& s = sds[rc.sys];
@@ -1208,45 +1315,86 @@ int iter_kernels(int method, int pc) {
 
 // One V-cycle over every system of the launch (levels of systems with fewer
 // levels drop out of the deeper launches), then (r, z) and beta.
-template <typename T>
-static void launch_vcycle(const LaunchSet& L, hipStream_t st) {
-  const int top = L.nlev_max - 1;  // deepest non-coarsest level + 1
-  const int nu = L.nu;
-  for (int l = 0; l < top; ++l) {
-    APH_LAUNCH((k_mg_smooth<T, true, false, false>), L.nlf[l], st, L.sds, L.rec_lf[l],
-               L.b2r_lf[l], l, 0, 0);
-    int cur = 0;
-    for (int k = 1; k < nu; ++k, cur ^= 1) {
-      APH_LAUNCH((k_mg_smooth<T, false, false, false>), L.nlf[l], st, L.sds, L.rec_lf[l],
-                 L.b2r_lf[l], l, cur, cur ^ 1);
-    }
-    APH_LAUNCH(k_mg_restrict<T>, L.nlc[l], st, L.sds, L.rec_lc[l], L.b2r_lc[l], l, cur);
+template <typename T, int MODE, bool DOT, bool L0>
+static void launch_sweep_l(const LaunchSet& L, hipStream_t st, int l, int zin, int zout) {
+  switch (L.vx_l[l]) {
+    case 4:
+      APH_LAUNCH((k_mg_sweep<T, 4, MODE, DOT, L0>), L.nlf[l], st, L.sds, L.rec_lf[l],
+                 L.b2r_lf[l], l, zin, zout);
+      break;
+    case 2:
+      APH_LAUNCH((k_mg_sweep<T, 2, MODE, DOT, L0>), L.nlf[l], st, L.sds, L.rec_lf[l],
+                 L.b2r_lf[l], l, zin, zout);
+      break;
+    default:
+      APH_LAUNCH((k_mg_sweep<T, 1, MODE, DOT, L0>), L.nlf[l], st, L.sds, L.rec_lf[l],
+                 L.b2r_lf[l], l, zin, zout);
+      break;
   }
+}
+
+template <typename T, int MODE, bool DOT>
+static void launch_sweep(const LaunchSet& L, hipStream_t st, int l, int zin, int zout) {
+  if (l == 0) launch_sweep_l<T, MODE, DOT, true>(L, st, l, zin, zout);
+  else launch_sweep_l<T, MODE, DOT, false>(L, st, l, zin, zout);
+}
+
+// V(2,2) cycle.  Per level above the coarsest: the zero-guess sweep fused with
+// the second pre-smoothing sweep (-> z[1]), restriction of r - L z[1], and
+// after the coarse correction the prolongation fused with the first
+// post-smoothing sweep (z[1] -> z[0]) and one more sweep (z[0] -> z[1]; on
+// level 0 it also forms (r, z)).  The correction of every level ends in z[1],
+// of the coarsest in z[0].  `ev` (profiling) records an event before every
+// launch and after the last; `kinds` receives 0 = level-0 pre-smoothing
+// sweep, 1 = level-0 restriction, 2 = coarser-level launch, 3 = coarsest
+// solve, 4 = finalize, 5 and 6 = level-0 first and second post-smoothing
+// sweeps.
+template <typename T>
+static int launch_vcycle(const LaunchSet& L, hipStream_t st, hipEvent_t* ev = nullptr,
+                         int* kinds = nullptr) {
+  int k = 0;
+  auto mark = [&](int kind) {
+    if (ev) {
+      (void)hipEventRecord(ev[k], st);
+      kinds[k] = kind;
+    }
+    ++k;
+  };
+  const int top = L.nlev_max - 1;  // deepest non-coarsest level + 1
+  for (int l = 0; l < top; ++l) {
+    mark(l == 0 ? 0 : 2);
+    launch_sweep<T, 1, false>(L, st, l, 0, 1);
+    mark(l == 0 ? 1 : 2);
+    if (l == 0) {
+      APH_LAUNCH((k_mg_restrict<T, true>), L.nlc[l], st, L.sds, L.rec_lc[l], L.b2r_lc[l], l, 1);
+    } else {
+      APH_LAUNCH((k_mg_restrict<T, false>), L.nlc[l], st, L.sds, L.rec_lc[l], L.b2r_lc[l], l, 1);
+    }
+  }
+  mark(3);
   hipLaunchKernelGGL(k_mg_coarse<T>, dim3(L.ns), dim3(kBlock), L.coarse_lds, st, L.sds,
                      L.coarse_sweeps);
   for (int l = top - 1; l >= 0; --l) {
-    int cur = (nu - 1) & 1;  // z index after the pre-smoothing
-    const bool last0 = l == 0 && nu == 1;
-    if (last0) {
-      APH_LAUNCH((k_mg_smooth<T, false, true, true>), L.nlf[l], st, L.sds, L.rec_lf[l],
-                 L.b2r_lf[l], l, cur, cur ^ 1);
-    } else {
-      APH_LAUNCH((k_mg_smooth<T, false, true, false>), L.nlf[l], st, L.sds, L.rec_lf[l],
-                 L.b2r_lf[l], l, cur, cur ^ 1);
-    }
-    cur ^= 1;
-    for (int k = 1; k < nu; ++k, cur ^= 1) {
-      if (l == 0 && k == nu - 1) {
-        APH_LAUNCH((k_mg_smooth<T, false, false, true>), L.nlf[l], st, L.sds, L.rec_lf[l],
-                   L.b2r_lf[l], l, cur, cur ^ 1);
-      } else {
-        APH_LAUNCH((k_mg_smooth<T, false, false, false>), L.nlf[l], st, L.sds, L.rec_lf[l],
-                   L.b2r_lf[l], l, cur, cur ^ 1);
-      }
-    }
+    mark(l == 0 ? 5 : 2);
+    launch_sweep<T, 2, false>(L, st, l, 1, 0);
+    mark(l == 0 ? 6 : 2);
+    if (l == 0) launch_sweep<T, 0, true>(L, st, l, 0, 1);
+    else launch_sweep<T, 0, false>(L, st, l, 0, 1);
   }
+  mark(4);
   APH_LAUNCH(k_fin_precond, L.ns, st, L.sds);
+  if (ev) (void)hipEventRecord(ev[k], st);
+  return k;
 }
+
+hipError_t launch_vcycle_profiled(int fp32, const LaunchSet& L, hipStream_t st, hipEvent_t* ev,
+                                  int* kinds, int* nlaunch) {
+  *nlaunch = fp32 ? launch_vcycle<float>(L, st, ev, kinds)
+                  : launch_vcycle<double>(L, st, ev, kinds);
+  return hipGetLastError();
+}
+
+int vcycle_launches(const LaunchSet& L) { return 4 * (L.nlev_max - 1) + 2; }
 
 template <typename T, int VX>
 static void launch_pcg_kernel(const LaunchSet& L, hipStream_t st, int j) {
@@ -1399,6 +1547,30 @@ hipError_t launch_sumsq(const double* v, long long n, double* partials, unsigned
   if (g > 1024) g = 1024;
   APH_LAUNCH(k_sumsq, g, st, v, n, partials, counter, out);
   APH_LAUNCH(k_fin_sumsq, 1, st, partials, g, out);
+  return hipGetLastError();
+}
+
+template <typename Ti, typename To>
+__global__ __launch_bounds__(kBlock) void k_cast(const Ti* __restrict__ src, To* __restrict__ dst,
+                                                 long long n) {
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (long long)gridDim.x * kBlock) {
+    dst[i] = (To)src[i];
+  }
+}
+
+hipError_t launch_cast(const void* src, int src_bytes, void* dst, int dst_bytes, long long n,
+                       hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  if (src_bytes == 8 && dst_bytes == 4) {
+    APH_LAUNCH((k_cast<double, float>), grid_for(n), st, (const double*)src, (float*)dst, n);
+  } else if (src_bytes == 4 && dst_bytes == 8) {
+    APH_LAUNCH((k_cast<float, double>), grid_for(n), st, (const float*)src, (double*)dst, n);
+  } else if (src_bytes == 8 && dst_bytes == 8) {
+    return hipMemcpyAsync(dst, src, n * 8, hipMemcpyDeviceToDevice, st);
+  } else {
+    return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

@@ -82,6 +82,16 @@ class Poisson:
     def set_tile(self, lx=0, bz=0, vx=0, sm=0):
         L.check(L.lib().aphpois_set_tile(self.h, lx, bz, vx, sm))
 
+    def precond_apply(self, r):
+        """z = M r with the multigrid preconditioner (PCG with precond MG);
+        r in the positive definite sign convention, shape (nz, ny, nx)."""
+        r = np.ascontiguousarray(r, np.float64)
+        if r.size != int(np.prod(self.shape)):
+            raise ValueError("r has the wrong size")
+        z = np.empty_like(r)
+        L.check(L.lib().aphpois_precond_apply(self.h, L.dptr(r), L.dptr(z)))
+        return z
+
     def device_bytes(self):
         return L.lib().aphpois_device_bytes(self.h)
 

@@ -24,13 +24,48 @@ sys.path.insert(0, ROOT)
 METRIC = "Poisson solves/s @512^3 to 1e-8 (mixed vs FP64 GPU); p50/p99 served latency"
 HBM_PEAK_GBS = 8000.0
 
-# algorithmic bytes per cell of the dominant kernel (PCG spmv with the fused
-# p update): read r, p_in, S, kx, ky, kz; write p_out, q  -> 8 words per cell
-SPMV_WORDS = 8
+# Algorithmic HBM words per cell of each iteration kernel (element size of the
+# engine's precision; +1 word each for the mass term of a non-singular system).
+# slot: index in Poisson.kernel_times(); name: the kernel instance in profiles.
+KERNELS = {
+    # p = z + beta p fused: read z, p_in, kx, ky, kz; write p_out, q
+    "spmv": {"slot": 0, "words": 7.0, "m": 1, "name": "k_pcg_spmv<T, VX, false>"},
+    # read p, xh, r, q; write xh, r
+    "update": {"slot": 2, "words": 6.0, "m": 0, "name": "k_pcg_update<T, VX>"},
+    # u = w D^-1 r fused: read r, dinv, kx, ky, kz; write z
+    "mg_presweep_l0": {"slot": 13, "words": 6.0, "m": 1,
+                       "name": "k_mg_sweep<T, VX, 1, false, true>"},
+    # read z_in, z_coarse (1/8), kx, ky, kz, dinv, r; write z
+    "mg_postsweep1_l0": {"slot": 14, "words": 7.125, "m": 1,
+                         "name": "k_mg_sweep<T, VX, 2, false, true>"},
+    # read z_in, kx, ky, kz, dinv, r; write z; (r, z) partials
+    "mg_postsweep2_l0": {"slot": 15, "words": 7.0, "m": 1,
+                         "name": "k_mg_sweep<T, VX, 0, true, true>"},
+    # read r, z, kx, ky, kz; write r_coarse (1/8)
+    "mg_restrict_l0": {"slot": 10, "words": 5.125, "m": 1, "name": "k_mg_restrict<T, true>"},
+}
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def kernel_table(kt, ncell, elem, has_m, per_iter_launches):
+    """Per-kernel average duration and achieved algorithmic bandwidth."""
+    out = {}
+    for key, k in KERNELS.items():
+        if k["slot"] >= len(kt):
+            continue
+        ms, cnt = kt[k["slot"]]
+        if cnt == 0 or ms <= 0:
+            continue
+        words = k["words"] + (k["m"] if has_m else 0)
+        b = words * elem * ncell
+        out[key] = {"kernel": k["name"], "avg_ms": round(ms, 5), "launches": cnt,
+                    "alg_bytes_per_cell": words * elem, "alg_bytes_per_launch": int(b),
+                    "GBps": round(b / (ms * 1e-3) / 1e9, 1),
+                    "ms_per_iter": round(ms * per_iter_launches.get(key, 1), 5)}
+    return out
 
 
 def main():
@@ -39,15 +74,16 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c3_512")
+    ap.add_argument("--precond", choices=["mg", "jacobi"], default="mg")
     ap.add_argument("--fp64-steps", type=int, default=1)
-    ap.add_argument("--cpu-iters", type=int, default=8,
+    ap.add_argument("--cpu-iters", type=int, default=15,
                     help="iterations of the CPU oracle timed for cpu_baseline (0 = skip)")
     ap.add_argument("--no-profile", action="store_true",
                     help="no per-kernel events in the timed region")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the systems of a batch shard by whole system, see DESIGN.md)", file=sys.stderr)
+              "(a batch of systems would shard by whole system across GPUs)", file=sys.stderr)
         return 2
 
     from aphros_amd import _lib as L
@@ -55,6 +91,7 @@ def main():
     from aphros_amd.devmem import DeviceBuffer
     from aphros_amd.solver import Poisson
 
+    precond = L.PRECOND_MG if args.precond == "mg" else L.PRECOND_JACOBI
     cfg = synth.CONFIGS[args.config]
     n = cfg["n"]
     t0 = time.time()
@@ -62,17 +99,22 @@ def main():
     log(f"[bench] generated {args.config} ({n}^3) in {time.time() - t0:.1f}s")
     ncell = n ** 3
     xbuf = DeviceBuffer(ncell * 8)
+    has_m = False  # the periodic pressure system: row sums vanish
 
     def run_form(precision, steps, warmup, profile):
         s = Poisson((n, n, n), method=L.PCG, precision=precision, resmode=L.RES_REL_TRUE,
-                    tol=1e-8, maxiter=200000)
+                    tol=1e-8, maxiter=200000, precond=precond)
         t = time.time()
         s.set_system(expr)
         log(f"[bench] precision={precision} set_system {time.time() - t:.2f}s "
             f"device_bytes={s.device_bytes() / 1e9:.2f} GB")
+        plain = []
         for i in range(warmup):
+            t = time.perf_counter()
             info = s.solve_device(xbuf.ptr)
-            log(f"[bench]   warmup {i}: iter={info['iter']} relres={info['true_relres']:.3e}")
+            plain.append(time.perf_counter() - t)
+            log(f"[bench]   warmup {i}: {plain[-1]:.3f}s iter={info['iter']} "
+                f"relres={info['true_relres']:.3e}")
         if profile:
             s.set_profiling(True)
         lat, infos = [], []
@@ -84,39 +126,46 @@ def main():
             log(f"[bench]   step {i}: {lat[-1]:.3f}s iter={info['iter']} "
                 f"ru={info['refinements']} relres={info['true_relres']:.3e} "
                 f"converged={info['converged']}")
-            if not info["converged"]:
-                raise RuntimeError("solve did not converge")
+            if not info["converged"] or info["true_relres"] > 1e-8:
+                raise RuntimeError("solve did not converge to 1e-8")
         kt = s.kernel_times() if profile else None
         li = s.launch_info()
         s.close()
-        return lat, infos, kt, li
+        return lat, infos, kt, li, plain
 
-    lat, infos, kt, li = run_form(L.MIXED, args.steps, args.warmup, not args.no_profile)
+    lat, infos, kt, li, plain = run_form(L.MIXED, args.steps, args.warmup,
+                                         not args.no_profile)
     total = sum(lat)
     value = args.steps / total
     iters = [i["iter"] for i in infos]
 
     fp64 = None
     if args.fp64_steps > 0:
-        lat64, inf64, kt64, _ = run_form(L.FP64, args.fp64_steps, 0, False)
-        fp64 = {"solves_per_s": args.fp64_steps / sum(lat64),
+        lat64, inf64, _, _, _ = run_form(L.FP64, args.fp64_steps, 0, False)
+        fp64 = {"solves_per_s": round(args.fp64_steps / sum(lat64), 4),
                 "iters": [i["iter"] for i in inf64],
-                "s_per_solve": sum(lat64) / args.fp64_steps}
+                "s_per_solve": round(sum(lat64) / args.fp64_steps, 4)}
 
-    roofline = None
+    roofline, kernels = None, None
     if kt:
-        spmv_ms, spmv_n = kt[0]
-        upd_ms, upd_n = kt[2]
-        bytes_per_launch = SPMV_WORDS * 4 * ncell
-        achieved = bytes_per_launch / (spmv_ms * 1e-3) / 1e9
-        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "traffic": None,
-                    "kernel": "k_pcg_spmv<float,4>",
-                    "alg_bytes_per_launch": bytes_per_launch,
-                    "avg_ms": round(spmv_ms, 5), "launches": spmv_n,
-                    "update_kernel_avg_ms": round(upd_ms, 5),
-                    "update_kernel_GBps": round(6 * 4 * ncell / (upd_ms * 1e-3) / 1e9, 1)}
+        per_iter = {"mg_presweep_l0": 1, "mg_postsweep1_l0": 1, "mg_postsweep2_l0": 1}
+        kernels = kernel_table(kt, ncell, 4, has_m, per_iter)
+        if kernels:
+            # the dominant kernel: the most device time per iteration
+            key = max(kernels, key=lambda k: kernels[k]["ms_per_iter"])
+            k = kernels[key]
+            achieved = k["GBps"]
+            roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                        "traffic": None, "kernel": k["kernel"] + " [T=float, VX=4]",
+                        "alg_bytes_per_launch": k["alg_bytes_per_launch"],
+                        "avg_ms": k["avg_ms"], "launches": k["launches"]}
+        if precond == L.PRECOND_MG and len(kt) > 12:
+            kernels["vcycle_total_ms"] = round(kt[8][0], 5)
+            kernels["mg_coarse_levels_ms_per_vcycle"] = round(kt[11][0], 5)
+            kernels["mg_coarsest_ms"] = round(kt[12][0], 5)
+        ms_it = 1e3 * total / max(1, sum(iters))
+        kernels["ms_per_iter_wall"] = round(ms_it, 5)
 
     cpu = None
     if args.cpu_iters > 0:
@@ -124,15 +173,21 @@ def main():
         e4 = expr.reshape(n, n, n, 8)
         ref_iters = fp64["iters"][0] if fp64 else iters[0]
         t = time.perf_counter()
-        _, it, _ = O.pcg_jacobi(e4, tol=0.0, maxiter=-args.cpu_iters, remove_mean=False)
+        if precond == L.PRECOND_MG:
+            _, it, _, setup = O.mgpcg(e4, tol=0.0, maxiter=-args.cpu_iters, remove_mean=False)
+            name = "multigrid PCG"
+        else:
+            _, it, _ = O.pcg_jacobi(e4, tol=0.0, maxiter=-args.cpu_iters, remove_mean=False)
+            setup, name = 0.0, "Jacobi-PCG"
         dt = time.perf_counter() - t
-        s_per_iter = dt / it
-        cpu = {"value": 1.0 / (s_per_iter * ref_iters), "unit": "solves/s",
+        s_per_iter = (dt - setup) / it
+        cpu = {"value": round(1.0 / (setup + s_per_iter * ref_iters), 6), "unit": "solves/s",
                "cores": O.num_threads(), "kind": "port",
-               "sample": f"{it} FP64 Jacobi-PCG iterations of the oracle on the {n}^3 system "
-                         f"({dt:.1f}s, {s_per_iter:.3f} s/iter), scaled by the {ref_iters} "
-                         f"iterations the FP64 GPU solve needs"}
+               "sample": f"{it} FP64 {name} iterations of the oracle (OpenMP) on the {n}^3 "
+                         f"system ({dt:.1f}s: setup {setup:.1f}s, {s_per_iter:.3f} s/iter), "
+                         f"scaled to the {ref_iters} iterations the FP64 GPU solve needs"}
 
+    pc_name = "multigrid V(2,2)" if precond == L.PRECOND_MG else "Jacobi"
     out = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -149,15 +204,18 @@ def main():
         "config": {"workload": f"{args.config}: {n}^3 periodic, {cfg['nspheres']} spheres "
                                f"r in [{cfg['rmin']},{cfg['rmax']}], density ratio "
                                f"{cfg['ratio']:g}, seed {cfg['seed']}; mixed-precision "
-                               f"Jacobi-PCG (FP32 inner, FP64 reliable updates) to a 1e-8 "
-                               f"true relative residual, zero initial guess",
+                               f"{pc_name}-preconditioned CG (FP32 inner, FP64 reliable "
+                               f"updates) to a 1e-8 true relative residual, zero initial "
+                               f"guess",
                    "cells": ncell, "iters": iters,
-                   "p50_s": float(np.percentile(lat, 50)),
-                   "p99_s": float(np.percentile(lat, 99)),
+                   "p50_s": round(float(np.percentile(lat, 50)), 4),
+                   "p99_s": round(float(np.percentile(lat, 99)), 4),
+                   "unprofiled_warmup_s": [round(v, 4) for v in plain],
                    "launch": li},
         "fp64": fp64,
         "mixed_speedup_vs_fp64": round(value / fp64["solves_per_s"], 3) if fp64 else None,
         "roofline": roofline,
+        "kernels": kernels,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -168,9 +168,11 @@ int aphpois_set_profiling(aphpois* h, int on);
  * iterations that ran since profiling was switched on.  Kernel order (each
  * reducing kernel is followed by its finalize kernel): PCG {spmv, fin,
  * update, fin, ru_accum, fin, ru_resid, fin} and, with the multigrid
- * preconditioner, {V-cycle} (all its launches together); CONJUGATE {spmv,
- * fin, update, fin}; JACOBI {sweep, fin}.  avg_ms and count hold 16 entries;
- * *nkernels receives the number used. */
+ * preconditioner, {V-cycle (all its launches together), level-0 sweep (per
+ * launch), level-0 restriction, coarser levels (per V-cycle), coarsest solve,
+ * level-0 pre-sweep, level-0 first post-sweep, level-0 second post-sweep};
+ * CONJUGATE {spmv, fin, update, fin}; JACOBI {sweep, fin}.  avg_ms and count
+ * hold 16 entries; *nkernels receives the number used. */
 int aphpois_kernel_times(aphpois* h, double* avg_ms, long long* count, int* nkernels);
 /* Launch geometry of the last solve: workgroups of the stencil and streaming
  * launches and the x vector width. */
@@ -180,6 +182,12 @@ int aphpois_launch_info(aphpois* h, int* stencil_blocks, int* stream_blocks, int
  * thread in x (1, 2 or 4; 4 only in MIXED; must divide nx), sm cells per
  * thread of the streaming kernels. */
 int aphpois_set_tile(aphpois* h, int lx, int bz, int vx, int sm);
+/* z = M r: one application of the multigrid preconditioner (method PCG,
+ * precond MG) of the resident system, in the handle's precision; r and z are
+ * dense host arrays of nx*ny*nz doubles, x fastest, in the sign convention of
+ * the positive definite form s*A (s = sign of e0).  A diagnostic: it lets the
+ * V-cycle be checked against an independent implementation. */
+int aphpois_precond_apply(aphpois* h, const double* r, double* z);
 
 /* ---------------- batched solves (many independent systems) -------------- */
 

@@ -23,6 +23,9 @@
  *                      preconditioned CPU CG, this is the FP64 yardstick for
  *                      the preconditioned GPU forms.
  *   orc_true_relres    ||b - A x||_2 / ||b||_2 with b = -e7, in FP64.
+ *   orc_mg_apply,      the multigrid V-cycle preconditioner and CG with it in
+ *   orc_mgpcg          FP64, the yardstick for the GPU's multigrid PCG (the
+ *                      reference's symmetric plugin has no preconditioner).
  *
  * Grid: nx*ny*nz cells, x fastest, fully periodic (the reference test and
  * default config are periodic: src/distr/distrbasic.cpp:66-83).  Neighbour
@@ -374,6 +377,353 @@ int orc_pcg_jacobi(const double* expr, int nx, int ny, int nz, const double* x0,
   }
   if (relres_out) *relres_out = fixed ? relres : orc_true_relres(expr, nx, ny, nz, x);
   free(r); free(z); free(p); free(q);
+  return iter;
+}
+
+/*
+ * Multigrid-preconditioned CG in FP64: the yardstick for the GPU's multigrid
+ * PCG (the reference has no such CPU solver).  The preconditioner is one
+ * symmetric V-cycle, defined as follows.
+ *
+ * Operator in conductance form.  With s = sign(e0), the system s*A x = s*b
+ * has (s*A u)_c = sum_f k_f (u_c - u_nb) + m_c u_c, k_f = -s * e_plus(c) on
+ * the face to the plus neighbour (symmetry gives the minus faces) and
+ * m_c = s * (row sum) (0 when |row sum| <= 1e-12 |e0|).
+ *
+ * Levels.  Level 0 is the grid; level l+1 halves every edge of level l while
+ * all edges are even and the coarse grid keeps at least 8 cells (at most 12
+ * levels; at least 2, the coarsest at most 4096 cells).  A coarse face
+ * conductance is 0.5 * (sum of the 4 fine faces it covers), the coarse mass
+ * 0.5 * (sum over the 8 children); D = diagonal of the level operator.
+ *
+ * V-cycle z = M r on level l (w = 0.8):
+ *   l < top: u = w D^-1 r; z = u + w D^-1 (r - L u);
+ *            r_{l+1} = sum over the 2x2x2 children of (r - L z);
+ *            z_{l+1} = M r_{l+1};
+ *            u = z + P z_{l+1} (P = injection to the children);
+ *            z = u + w D^-1 (r - L u); z = z + w D^-1 (r - L z)
+ *   top:     u = w D^-1 r, then `sweeps - 1` more damped Jacobi sweeps
+ * Restriction is the transpose of the injection and the pre and post sweeps
+ * mirror each other, so M is symmetric.
+ *
+ * CG: textbook PCG with M, stopped on the recursive residual
+ * ||r|| <= target ||b|| and confirmed on the true residual (restart with a
+ * tighter target if needed), like orc_pcg_jacobi.  maxiter < 0 runs exactly
+ * -maxiter iterations (timing sample).  *setup_s receives the seconds of the
+ * level setup.  Returns the iteration count or -1 (allocation), -2 (grid
+ * cannot be coarsened).
+ */
+typedef struct {
+  int nx, ny, nz;
+  size_t n;
+  double *kx, *ky, *kz, *m, *dinv, *r, *z, *t;
+} mglev_t;
+
+static const double kMgOmega = 0.8;
+
+static double wall_seconds(void) {
+#ifdef _OPENMP
+  return omp_get_wtime();
+#else
+  return (double)clock() / CLOCKS_PER_SEC;
+#endif
+}
+
+/* (L u)_c of level L at (x, y, z) */
+static inline double mg_lap(const mglev_t* L, const double* u, int x, int y, int z) {
+  const grid_t g = {L->nx, L->ny, L->nz};
+  const size_t c = idx(&g, x, y, z);
+  const size_t xm = idx(&g, wrap(x - 1, L->nx), y, z), xp = idx(&g, wrap(x + 1, L->nx), y, z);
+  const size_t ym = idx(&g, x, wrap(y - 1, L->ny), z), yp = idx(&g, x, wrap(y + 1, L->ny), z);
+  const size_t zm = idx(&g, x, y, wrap(z - 1, L->nz)), zp = idx(&g, x, y, wrap(z + 1, L->nz));
+  const double uc = u[c];
+  double s = L->kx[c] * (uc - u[xp]) + L->kx[xm] * (uc - u[xm]) + L->ky[c] * (uc - u[yp]) +
+             L->ky[ym] * (uc - u[ym]) + L->kz[c] * (uc - u[zp]) + L->kz[zm] * (uc - u[zm]);
+  if (L->m) s += L->m[c] * uc;
+  return s;
+}
+
+/* out = u + w D^-1 (r - L u) */
+static void mg_sweep(const mglev_t* L, const double* u, double* out) {
+#pragma omp parallel for schedule(static)
+  for (int z = 0; z < L->nz; ++z)
+    for (int y = 0; y < L->ny; ++y)
+      for (int x = 0; x < L->nx; ++x) {
+        const size_t c = (size_t)x + (size_t)L->nx * ((size_t)y + (size_t)L->ny * z);
+        out[c] = u[c] + kMgOmega * L->dinv[c] * (L->r[c] - mg_lap(L, u, x, y, z));
+      }
+}
+
+static void mg_vcycle(mglev_t* lev, int l, int top, int sweeps) {
+  mglev_t* L = &lev[l];
+  const long long n = (long long)L->n;
+#pragma omp parallel for schedule(static)
+  for (long long c = 0; c < n; ++c) L->t[c] = kMgOmega * L->dinv[c] * L->r[c];
+  if (l == top) {
+    for (int k = 1; k < sweeps; ++k) {
+      mg_sweep(L, L->t, L->z);
+      double* tmp = L->t;
+      L->t = L->z;
+      L->z = tmp;
+    }
+    double* tmp = L->t;
+    L->t = L->z;
+    L->z = tmp;
+    return;
+  }
+  mg_sweep(L, L->t, L->z);
+  mglev_t* C = &lev[l + 1];
+#pragma omp parallel for schedule(static)
+  for (int z = 0; z < C->nz; ++z)
+    for (int y = 0; y < C->ny; ++y)
+      for (int x = 0; x < C->nx; ++x) {
+        double sum = 0;
+        for (int d = 0; d < 8; ++d) {
+          const int fx = 2 * x + (d & 1), fy = 2 * y + ((d >> 1) & 1), fz = 2 * z + (d >> 2);
+          const size_t f = (size_t)fx + (size_t)L->nx * ((size_t)fy + (size_t)L->ny * fz);
+          sum += L->r[f] - mg_lap(L, L->z, fx, fy, fz);
+        }
+        C->r[(size_t)x + (size_t)C->nx * ((size_t)y + (size_t)C->ny * z)] = sum;
+      }
+  mg_vcycle(lev, l + 1, top, sweeps);
+#pragma omp parallel for schedule(static)
+  for (int z = 0; z < L->nz; ++z)
+    for (int y = 0; y < L->ny; ++y)
+      for (int x = 0; x < L->nx; ++x) {
+        const size_t c = (size_t)x + (size_t)L->nx * ((size_t)y + (size_t)L->ny * z);
+        const size_t cc =
+            (size_t)(x >> 1) + (size_t)C->nx * ((size_t)(y >> 1) + (size_t)C->ny * (z >> 1));
+        L->t[c] = L->z[c] + C->z[cc];
+      }
+  mg_sweep(L, L->t, L->z);
+  mg_sweep(L, L->z, L->t);
+  double* tmp = L->t;
+  L->t = L->z;
+  L->z = tmp;
+}
+
+static void mg_free(mglev_t* lev, int nlev) {
+  for (int l = 0; l < nlev; ++l) {
+    free(lev[l].kx); free(lev[l].ky); free(lev[l].kz); free(lev[l].m);
+    free(lev[l].dinv); free(lev[l].r); free(lev[l].z); free(lev[l].t);
+  }
+}
+
+/* Levels of the grid (conductances, masses, D^-1); returns nlev or -1/-2. */
+static int mg_setup(const double* expr, int nx, int ny, int nz, mglev_t* lev) {
+  int nlev = 0;
+  int lx = nx, ly = ny, lz = nz;
+  for (;;) {
+    mglev_t* L = &lev[nlev++];
+    memset(L, 0, sizeof(*L));
+    L->nx = lx; L->ny = ly; L->nz = lz;
+    L->n = (size_t)lx * ly * lz;
+    if (nlev >= 12 || ((lx | ly | lz) & 1) || (long long)(lx / 2) * (ly / 2) * (lz / 2) < 8)
+      break;
+    lx /= 2; ly /= 2; lz /= 2;
+  }
+  if (nlev < 2 || lev[nlev - 1].n > 4096) return -2;
+  const size_t n0 = lev[0].n;
+  double sgn = 1;
+  for (size_t c = 0; c < n0; ++c)
+    if (expr[c * 8] != 0) {
+      sgn = expr[c * 8] < 0 ? -1 : 1;
+      break;
+    }
+  int has_m = 0;
+  for (size_t c = 0; c < n0 && !has_m; ++c) {
+    const double* e = expr + c * 8;
+    const double rs = e[0] + e[1] + e[2] + e[3] + e[4] + e[5] + e[6];
+    if (fabs(rs) > 1e-12 * fabs(e[0])) has_m = 1;
+  }
+  for (int l = 0; l < nlev; ++l) {
+    mglev_t* L = &lev[l];
+    const size_t b = L->n * sizeof(double);
+    L->kx = (double*)malloc(b); L->ky = (double*)malloc(b); L->kz = (double*)malloc(b);
+    L->m = has_m ? (double*)malloc(b) : NULL;
+    L->dinv = (double*)malloc(b); L->r = (double*)malloc(b);
+    L->z = (double*)malloc(b); L->t = (double*)malloc(b);
+    if (!L->kx || !L->ky || !L->kz || (has_m && !L->m) || !L->dinv || !L->r || !L->z || !L->t) {
+      mg_free(lev, nlev);
+      return -1;
+    }
+  }
+#pragma omp parallel for schedule(static)
+  for (long long c = 0; c < (long long)n0; ++c) {
+    const double* e = expr + c * 8;
+    lev[0].kx[c] = -sgn * e[2];
+    lev[0].ky[c] = -sgn * e[4];
+    lev[0].kz[c] = -sgn * e[6];
+    if (has_m) {
+      const double rs = e[0] + e[1] + e[2] + e[3] + e[4] + e[5] + e[6];
+      lev[0].m[c] = fabs(rs) > 1e-12 * fabs(e[0]) ? sgn * rs : 0.0;
+    }
+  }
+  for (int l = 1; l < nlev; ++l) {
+    const mglev_t* F = &lev[l - 1];
+    mglev_t* C = &lev[l];
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < C->nz; ++z)
+      for (int y = 0; y < C->ny; ++y)
+        for (int x = 0; x < C->nx; ++x) {
+          const grid_t g = {F->nx, F->ny, F->nz};
+          double sx = 0, sy = 0, sz = 0, sm = 0;
+          for (int a = 0; a < 2; ++a)
+            for (int bb = 0; bb < 2; ++bb) {
+              sx += F->kx[idx(&g, 2 * x + 1, 2 * y + a, 2 * z + bb)];
+              sy += F->ky[idx(&g, 2 * x + a, 2 * y + 1, 2 * z + bb)];
+              sz += F->kz[idx(&g, 2 * x + a, 2 * y + bb, 2 * z + 1)];
+            }
+          const size_t c = (size_t)x + (size_t)C->nx * ((size_t)y + (size_t)C->ny * z);
+          if (has_m) {
+            for (int d = 0; d < 8; ++d)
+              sm += F->m[idx(&g, 2 * x + (d & 1), 2 * y + ((d >> 1) & 1), 2 * z + (d >> 2))];
+            C->m[c] = 0.5 * sm;
+          }
+          C->kx[c] = 0.5 * sx;
+          C->ky[c] = 0.5 * sy;
+          C->kz[c] = 0.5 * sz;
+        }
+  }
+  for (int l = 0; l < nlev; ++l) {
+    mglev_t* L = &lev[l];
+    const grid_t g = {L->nx, L->ny, L->nz};
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < L->nz; ++z)
+      for (int y = 0; y < L->ny; ++y)
+        for (int x = 0; x < L->nx; ++x) {
+          const size_t c = idx(&g, x, y, z);
+          double d = L->kx[c] + L->kx[idx(&g, wrap(x - 1, L->nx), y, z)] + L->ky[c] +
+                     L->ky[idx(&g, x, wrap(y - 1, L->ny), z)] + L->kz[c] +
+                     L->kz[idx(&g, x, y, wrap(z - 1, L->nz))];
+          if (L->m) d += L->m[c];
+          L->dinv[c] = d != 0 ? 1.0 / d : 0.0;
+        }
+  }
+  return nlev;
+}
+
+/* z = M r for one right-hand side (tests: symmetry of M, action on modes);
+ * returns nlev or -1/-2. */
+int orc_mg_apply(const double* expr, int nx, int ny, int nz, int sweeps, const double* r,
+                 double* z) {
+  mglev_t lev[12];
+  const int nlev = mg_setup(expr, nx, ny, nz, lev);
+  if (nlev < 0) return nlev;
+  memcpy(lev[0].r, r, lev[0].n * sizeof(double));
+  mg_vcycle(lev, 0, nlev - 1, sweeps);
+  memcpy(z, lev[0].z, lev[0].n * sizeof(double));
+  mg_free(lev, nlev);
+  return nlev;
+}
+
+int orc_mgpcg(const double* expr, int nx, int ny, int nz, const double* x0, double* x,
+              double tol, int maxiter, int sweeps, int remove_mean, double* relres_out,
+              double* setup_s) {
+  const double t0 = wall_seconds();
+  mglev_t lev[12];
+  const int nlev = mg_setup(expr, nx, ny, nz, lev);
+  if (nlev < 0) return nlev;
+  if (setup_s) *setup_s = wall_seconds() - t0;
+  mglev_t* L0 = &lev[0];
+  const long long n = (long long)L0->n;
+  double* r = (double*)malloc(n * sizeof(double));
+  double* p = (double*)malloc(n * sizeof(double));
+  double* q = (double*)malloc(n * sizeof(double));
+  if (!r || !p || !q) {
+    free(r); free(p); free(q);
+    mg_free(lev, nlev);
+    return -1;
+  }
+  double sgn = 1;
+  for (long long c = 0; c < n; ++c)
+    if (expr[c * 8] != 0) {
+      sgn = expr[c * 8] < 0 ? -1 : 1;
+      break;
+    }
+  const int fixed = maxiter < 0;
+  const int itmax = fixed ? -maxiter : maxiter;
+  if (x0 && x0 != x) memcpy(x, x0, n * sizeof(double));
+  if (!x0) memset(x, 0, n * sizeof(double));
+  double bb = 0;
+#pragma omp parallel for reduction(+ : bb) schedule(static)
+  for (long long c = 0; c < n; ++c) bb += expr[c * 8 + 7] * expr[c * 8 + 7];
+  const double bnorm = sqrt(bb);
+  double target = tol;
+  int iter = 0;
+  double relres = 1;
+  for (int restart = 0; restart < 8; ++restart) {
+    /* r = s * (b - A x) = -s * (A x + e7) */
+#pragma omp parallel for schedule(static)
+    for (int zz = 0; zz < nz; ++zz)
+      for (int yy = 0; yy < ny; ++yy)
+        for (int xx = 0; xx < nx; ++xx) {
+          const grid_t g = {nx, ny, nz};
+          const size_t c = idx(&g, xx, yy, zz);
+          const double* e = expr + c * 8;
+          size_t nb[6];
+          neighbours(&g, xx, yy, zz, nb);
+          double u = x[c] * e[0] + e[7];
+          for (int k = 0; k < 6; ++k) u += x[nb[k]] * e[1 + k];
+          r[c] = -sgn * u;
+        }
+    memcpy(L0->r, r, n * sizeof(double));
+    mg_vcycle(lev, 0, nlev - 1, sweeps);
+    double rz = 0, rr = 0;
+#pragma omp parallel for reduction(+ : rz, rr) schedule(static)
+    for (long long c = 0; c < n; ++c) {
+      p[c] = L0->z[c];
+      rz += r[c] * L0->z[c];
+      rr += r[c] * r[c];
+    }
+    relres = bnorm > 0 ? sqrt(rr) / bnorm : sqrt(rr);
+    if (!fixed && relres <= tol) break;
+    while (fixed ? iter < itmax : (iter < itmax && sqrt(rr) > target * bnorm)) {
+      double pq = 0;
+#pragma omp parallel for reduction(+ : pq) schedule(static)
+      for (int zz = 0; zz < nz; ++zz)
+        for (int yy = 0; yy < ny; ++yy)
+          for (int xx = 0; xx < nx; ++xx) {
+            const size_t c = (size_t)xx + (size_t)nx * ((size_t)yy + (size_t)ny * zz);
+            const double u = mg_lap(L0, p, xx, yy, zz);
+            q[c] = u;
+            pq += p[c] * u;
+          }
+      const double alpha = rz / pq;
+      rr = 0;
+#pragma omp parallel for reduction(+ : rr) schedule(static)
+      for (long long c = 0; c < n; ++c) {
+        x[c] += alpha * p[c];
+        r[c] -= alpha * q[c];
+        L0->r[c] = r[c];
+        rr += r[c] * r[c];
+      }
+      mg_vcycle(lev, 0, nlev - 1, sweeps);
+      double rz_new = 0;
+#pragma omp parallel for reduction(+ : rz_new) schedule(static)
+      for (long long c = 0; c < n; ++c) rz_new += r[c] * L0->z[c];
+      const double beta = rz_new / rz;
+      rz = rz_new;
+#pragma omp parallel for schedule(static)
+      for (long long c = 0; c < n; ++c) p[c] = L0->z[c] + beta * p[c];
+      ++iter;
+    }
+    if (fixed) break;
+    relres = orc_true_relres(expr, nx, ny, nz, x);
+    if (relres <= tol || iter >= itmax) break;
+    target *= 0.5 * tol / relres;
+  }
+  if (remove_mean) {
+    double s = 0;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+    for (long long c = 0; c < n; ++c) s += x[c];
+    const double mean = s / (double)n;
+#pragma omp parallel for schedule(static)
+    for (long long c = 0; c < n; ++c) x[c] -= mean;
+  }
+  if (relres_out) *relres_out = fixed ? relres : orc_true_relres(expr, nx, ny, nz, x);
+  free(r); free(p); free(q);
+  mg_free(lev, nlev);
   return iter;
 }
 

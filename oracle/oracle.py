@@ -48,6 +48,11 @@ def lib():
         L.orc_pcg_jacobi.argtypes = [_dp, C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_double,
                                      C.c_int, C.c_int, _dp]
         L.orc_pcg_jacobi.restype = C.c_int
+        L.orc_mg_apply.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp]
+        L.orc_mg_apply.restype = C.c_int
+        L.orc_mgpcg.argtypes = [_dp, C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_int,
+                                C.c_int, C.c_int, _dp, _dp]
+        L.orc_mgpcg.restype = C.c_int
         L.orc_num_threads.argtypes = []
         L.orc_num_threads.restype = C.c_int
         _lib = L
@@ -137,6 +142,42 @@ def pcg_jacobi(expr, tol=1e-8, maxiter=100000, remove_mean=True, x0=None):
     if it < 0:
         raise MemoryError("oracle allocation failed")
     return x, it, rr.value
+
+
+MG_SWEEPS = 40  # damped Jacobi sweeps on the coarsest level
+
+
+def mg_apply(expr, r, sweeps=MG_SWEEPS):
+    """One V-cycle z = M r of the multigrid preconditioner; returns (z, nlev)."""
+    expr = np.ascontiguousarray(expr, np.float64)
+    nx, ny, nz = _dims(expr.shape[:3])
+    r = np.ascontiguousarray(r, np.float64)
+    z = np.empty(expr.shape[:3])
+    nlev = lib().orc_mg_apply(_p(expr), nx, ny, nz, sweeps, _p(r), _p(z))
+    if nlev == -2:
+        raise ValueError("grid cannot be coarsened for the multigrid preconditioner")
+    if nlev < 0:
+        raise MemoryError("oracle allocation failed")
+    return z, nlev
+
+
+def mgpcg(expr, tol=1e-8, maxiter=100000, remove_mean=True, x0=None, sweeps=MG_SWEEPS):
+    """FP64 multigrid-preconditioned CG to a true relative residual; returns
+    (x, iter, relres, setup_seconds).  maxiter < 0 runs exactly -maxiter
+    iterations (timing sample)."""
+    expr = np.ascontiguousarray(expr, np.float64)
+    nx, ny, nz = _dims(expr.shape[:3])
+    x = np.empty(expr.shape[:3])
+    x0 = None if x0 is None else np.ascontiguousarray(x0, np.float64)
+    rr = C.c_double()
+    setup = C.c_double()
+    it = lib().orc_mgpcg(_p(expr), nx, ny, nz, _p(x0), _p(x), tol, maxiter, sweeps,
+                         int(remove_mean), C.byref(rr), C.byref(setup))
+    if it == -2:
+        raise ValueError("grid cannot be coarsened for the multigrid preconditioner")
+    if it < 0:
+        raise MemoryError("oracle allocation failed")
+    return x, it, rr.value, setup.value
 
 
 def num_threads():

@@ -110,6 +110,55 @@ def test_pcg_tlinear_negative_diagonal(tlin32, precond):
     assert O.max_diff_mean_removed(x, ue) < 1e-7
 
 
+def _with_mass(e, frac=0.05, seed=4):
+    e = e.copy()
+    rng = np.random.default_rng(seed)
+    e[..., 0] += np.sign(e[..., 0]) * frac * np.abs(e[..., 0]).mean() * rng.random(e.shape[:3])
+    return e
+
+
+@pytest.mark.parametrize("precision,tol", [(L.FP64, 1e-12), (L.MIXED, 1e-4)])
+@pytest.mark.parametrize("case", ["c1", "tlin_negative", "c1_mass"])
+def test_mg_vcycle_matches_oracle(c1, tlin32, precision, tol, case):
+    """One V-cycle on the GPU against the oracle's V-cycle (FP64: rounding
+    only; MIXED: FP32 arithmetic), on the same residual."""
+    e = {"c1": c1, "tlin_negative": tlin32[0], "c1_mass": _with_mass(c1)}[case]
+    _say(f"vcycle {case} precision={precision}")
+    s = Poisson(e.shape[:3], method=L.PCG, precision=precision, resmode=L.RES_REL_TRUE,
+                tol=1e-8, maxiter=20000, precond=L.PRECOND_MG)
+    s.set_system(e)
+    rng = np.random.default_rng(12)
+    r = rng.standard_normal(e.shape[:3])
+    r -= r.mean()
+    z = s.precond_apply(r)
+    zo, nlev = O.mg_apply(e, r)
+    err = np.abs(z - zo).max() / np.abs(zo).max()
+    _say(f"  nlev={nlev} rel_err={err:.3e}")
+    assert err <= tol
+    # the solve state is untouched: a solve after it matches a fresh handle
+    x, info = s.solve(None)
+    s2 = Poisson(e.shape[:3], method=L.PCG, precision=precision, resmode=L.RES_REL_TRUE,
+                 tol=1e-8, maxiter=20000, precond=L.PRECOND_MG)
+    s2.set_system(e)
+    x2, info2 = s2.solve(None)
+    assert info["iter"] == info2["iter"] and np.array_equal(x, x2)
+
+
+def test_mg_fp64_iterations_match_oracle(c1):
+    """FP64 multigrid PCG on the GPU and the oracle's FP64 multigrid PCG (same
+    preconditioner, sums in a different order): equal iteration counts up to
+    the stop test's granularity."""
+    s = Poisson(c1.shape[:3], method=L.PCG, precision=L.FP64, resmode=L.RES_REL_TRUE,
+                tol=1e-8, maxiter=20000, precond=L.PRECOND_MG)
+    s.set_system(c1)
+    x, info = s.solve(None)
+    xo, ito, rro, _ = O.mgpcg(c1, tol=1e-8)
+    _say(f"  gpu iter={info['iter']} oracle iter={ito}")
+    assert abs(info["iter"] - ito) <= 2
+    err = np.abs(x - xo).max() / np.abs(xo).max()
+    assert err < 1e-6
+
+
 def test_warm_start_alias_and_setconf(c1):
     s = Poisson(c1.shape[:3], method=L.PCG, precision=L.MIXED, resmode=L.RES_REL_TRUE,
                 tol=1e-6, maxiter=20000)
