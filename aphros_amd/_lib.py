@@ -105,6 +105,12 @@ SIGNATURES = {
                                       C.POINTER(C.c_int)]),
     "aphpois_set_tile": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int]),
     "aphpois_precond_apply": (C.c_int, [_vp, _dp, _dp]),
+    "aphpois_assemble": (C.c_int, [_vp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_int]),
+    "aphpois_assemble_device": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_double,
+                                          C.c_int]),
+    "aphpois_flux_correct": (C.c_int, [_vp, _dp, _dp, _dp, _dp, C.c_double, C.c_double]),
+    "aphpois_flux_correct_device": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_double,
+                                              C.c_double]),
     "aphpois_batch_set_tile": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int]),
     "aphpois_batch_set_system_tile": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "aphpois_batch_last_seconds": (C.c_double, [_vp]),

@@ -169,6 +169,30 @@ int aphpois_set_tile(aphpois* h, int lx, int bz, int vx, int sm) {
   return APHPOIS_OK;
 }
 
+int aphpois_assemble(aphpois* h, const double* rho, const double* ffv, const double* src,
+                     double cell, double dt, int sign) {
+  APH_CHECK_H(h);
+  APH_RET(*h, h->eng.assemble(h->slot, rho, ffv, src, cell, dt, sign, false));
+}
+
+int aphpois_assemble_device(aphpois* h, const double* rho, const double* ffv,
+                            const double* src, double cell, double dt, int sign) {
+  APH_CHECK_H(h);
+  APH_RET(*h, h->eng.assemble(h->slot, rho, ffv, src, cell, dt, sign, true));
+}
+
+int aphpois_flux_correct(aphpois* h, const double* p, const double* rho, const double* ffv,
+                         double* ffv_out, double cell, double dt) {
+  APH_CHECK_H(h);
+  APH_RET(*h, h->eng.flux_correct(h->slot, p, rho, ffv, ffv_out, cell, dt, false));
+}
+
+int aphpois_flux_correct_device(aphpois* h, const double* p, const double* rho,
+                                const double* ffv, double* ffv_out, double cell, double dt) {
+  APH_CHECK_H(h);
+  APH_RET(*h, h->eng.flux_correct(h->slot, p, rho, ffv, ffv_out, cell, dt, true));
+}
+
 int aphpois_precond_apply(aphpois* h, const double* r, double* z) {
   APH_CHECK_H(h);
   if (!r || !z) return set_err(APHPOIS_EINVAL, "r or z is NULL");

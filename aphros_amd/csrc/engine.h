@@ -65,6 +65,14 @@ class Engine {
                  long long lz);
   int set_rhs(int slot, const double* e7, bool on_device, long long start, long long ly,
               long long lz);
+  // The projection's pressure system from density, face volume fluxes and a
+  // volume source (dense fields, x fastest), assembled on the device and made
+  // the resident system as by set_system.
+  int assemble(int slot, const double* rho, const double* ffv, const double* src, double cell,
+               double dt, int sign, bool on_device);
+  // Corrected face fluxes of the projection for the solution p.
+  int flux_correct(int slot, const double* p, const double* rho, const double* ffv, double* out,
+                   double cell, double dt, bool on_device);
   // x0: nullptr (zero guess), or host/device strided field; keep_x: warm start
   // from the resident solution.
   int prepare(int slot, const double* x0, bool on_device, long long start, long long ly,
@@ -149,6 +157,9 @@ class Engine {
   int upload_strided(const double* src, bool on_device, long long start, long long ly,
                      long long lz, int nx, int ny, int nz, int comps, const double** dptr);
   void free_system(System& s);
+  int ensure_staging(long long bytes);
+  // set_system from a dense-strided FP64 Expr already in device memory
+  int set_system_staged(int slot, const double* d, long long ly, long long lz);
 
   int device_ = 0;
   int method_ = kConjugate;

@@ -386,6 +386,94 @@ int Engine::set_system(int slot, const double* expr, bool on_device, long long s
   System& s = *sp;
   const double* d = nullptr;
   APH_TRY(upload_strided(expr, on_device, start, ly, lz, s.nx, s.ny, s.nz, 8, &d));
+  return set_system_staged(slot, d, ly, lz);
+}
+
+int Engine::ensure_staging(long long need) {
+  if (need <= staging_bytes_) return APHPOIS_OK;
+  if (staging_) {
+    APH_HIP(hipStreamSynchronize(st_));
+    APH_HIP(hipFree(staging_));
+    staging_ = nullptr;
+    staging_bytes_ = 0;
+  }
+  if (budget_ > 0 && bytes() + need > budget_)
+    return fail(APHPOIS_EBUDGET, "memory budget exceeded by the staging buffer");
+  APH_HIP(hipMalloc(&staging_, need));
+  staging_bytes_ = need;
+  return APHPOIS_OK;
+}
+
+int Engine::assemble(int slot, const double* rho, const double* ffv, const double* src,
+                     double cell, double dt, int sign, bool on_device) {
+  System* sp = sys(slot);
+  if (!sp) return fail(APHPOIS_EINVAL, "no such slot");
+  if (!rho || !ffv) return fail(APHPOIS_EINVAL, "rho or ffv is NULL");
+  if (!(cell > 0) || !(dt > 0) || (sign != 1 && sign != -1))
+    return fail(APHPOIS_EINVAL, "need cell > 0, dt > 0 and sign +1 or -1");
+  System& s = *sp;
+  const long long n = s.n;
+  APH_TRY(ensure_staging(n * 8 * 8));
+  const double *drho = rho, *dffv = ffv, *dsrc = src;
+  void* tmp = nullptr;
+  if (!on_device) {
+    // inputs through a temporary device buffer (rho, 3 face fields, source)
+    const long long need = n * 8 * (src ? 5 : 4);
+    if (budget_ > 0 && bytes() + need > budget_)
+      return fail(APHPOIS_EBUDGET, "memory budget exceeded by the assembly inputs");
+    APH_HIP(hipMalloc(&tmp, need));
+    double* t = (double*)tmp;
+    hipError_t e = hipMemcpyAsync(t, rho, n * 8, hipMemcpyHostToDevice, st_);
+    if (e == hipSuccess) e = hipMemcpyAsync(t + n, ffv, 3 * n * 8, hipMemcpyHostToDevice, st_);
+    if (e == hipSuccess && src)
+      e = hipMemcpyAsync(t + 4 * n, src, n * 8, hipMemcpyHostToDevice, st_);
+    if (e != hipSuccess) {
+      (void)hipStreamSynchronize(st_);
+      (void)hipFree(tmp);
+      return hip(e, "assemble upload");
+    }
+    drho = t;
+    dffv = t + n;
+    dsrc = src ? t + 4 * n : nullptr;
+  }
+  hipError_t e = launch_assemble(drho, dffv, dsrc, s.nx, s.ny, s.nz, cell, dt, sign,
+                                 (double*)staging_, st_);
+  if (tmp) {
+    const hipError_t e2 = hipStreamSynchronize(st_);
+    (void)hipFree(tmp);
+    if (e == hipSuccess) e = e2;
+  }
+  APH_HIP(e);
+  return set_system_staged(slot, (const double*)staging_, s.nx, (long long)s.nx * s.ny);
+}
+
+int Engine::flux_correct(int slot, const double* p, const double* rho, const double* ffv,
+                         double* out, double cell, double dt, bool on_device) {
+  System* sp = sys(slot);
+  if (!sp) return fail(APHPOIS_EINVAL, "no such slot");
+  if (!p || !rho || !ffv || !out) return fail(APHPOIS_EINVAL, "NULL argument");
+  if (!(cell > 0) || !(dt > 0)) return fail(APHPOIS_EINVAL, "need cell > 0 and dt > 0");
+  const long long n = sp->n;
+  if (on_device) {
+    APH_HIP(launch_flux_correct(p, rho, ffv, out, sp->nx, sp->ny, sp->nz, cell, dt, st_));
+    APH_HIP(hipStreamSynchronize(st_));
+    return APHPOIS_OK;
+  }
+  // p, rho, ffv in; corrected fluxes out (8 n doubles in flight)
+  APH_TRY(ensure_staging(n * 8 * 8));
+  double* t = (double*)staging_;
+  APH_HIP(hipMemcpyAsync(t, p, n * 8, hipMemcpyHostToDevice, st_));
+  APH_HIP(hipMemcpyAsync(t + n, rho, n * 8, hipMemcpyHostToDevice, st_));
+  APH_HIP(hipMemcpyAsync(t + 2 * n, ffv, 3 * n * 8, hipMemcpyHostToDevice, st_));
+  APH_HIP(launch_flux_correct(t, t + n, t + 2 * n, t + 5 * n, sp->nx, sp->ny, sp->nz, cell, dt,
+                              st_));
+  APH_HIP(hipMemcpyAsync(out, t + 5 * n, 3 * n * 8, hipMemcpyDeviceToHost, st_));
+  APH_HIP(hipStreamSynchronize(st_));
+  return APHPOIS_OK;
+}
+
+int Engine::set_system_staged(int slot, const double* d, long long ly, long long lz) {
+  System& s = *sys(slot);
   APH_HIP(hipMemsetAsync(d_flags_, 0, sizeof(unsigned), st_));
   APH_HIP(launch_check(d, 0, ly, lz, s.nx, s.ny, s.nz, d_flags_, st_));
   unsigned flags = 0;
@@ -614,6 +702,13 @@ int Engine::build_tables() {
         const MgLevel& C = s.lev[l + 1];
         Rec f = make_stencil_rec(F.nx, F.ny, F.nz, vxl[l],
                                  std::min(3072LL, std::max(1LL, cdiv(F.n, 2048))), none);
+        // the fused restriction sums plane pairs inside one workgroup: even
+        // plane counts per tile (F.nz is even on every coarsened level)
+        if (f.bz & 1) {
+          f.bz = std::min(f.bz + 1, F.nz);
+          f.tiles_z = (int)cdiv(F.nz, f.bz);
+          f.nblk = f.tiles_x * f.tiles_y * f.tiles_z;
+        }
         f.sys = k;
         f.blk0 = (int)blf[l].size();
         for (int i = 0; i < f.nblk; ++i) blf[l].push_back((int)rlf[l].size());

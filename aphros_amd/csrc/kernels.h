@@ -68,6 +68,15 @@ hipError_t launch_scatter(const double* src, int nx, int ny, int nz, double* dst
 hipError_t launch_sumsq(const double* v, long long n, double* partials, unsigned* counter,
                         double* out, hipStream_t st);
 hipError_t launch_fill(void* p, long long n, int elem_bytes, hipStream_t st);
+// The projection's pressure system (dense FP64 Expr AoS, x fastest) from
+// density, face volume fluxes (3 n, lower faces) and source (may be null).
+hipError_t launch_assemble(const double* rho, const double* ffv, const double* src, int nx,
+                           int ny, int nz, double cell, double dt, int sign, double* expr,
+                           hipStream_t st);
+// Corrected face fluxes (3 n) for the solution p.
+hipError_t launch_flux_correct(const double* p, const double* rho, const double* ffv,
+                               double* out, int nx, int ny, int nz, double cell, double dt,
+                               hipStream_t st);
 // dst = (type of dst) src, element sizes 8 -> 4, 4 -> 8 or 8 -> 8 bytes
 hipError_t launch_cast(const void* src, int src_bytes, void* dst, int dst_bytes, long long n,
                        hipStream_t st);

@@ -511,6 +511,10 @@ __device__ __forceinline__ Cell cell_of(const Rec& rc, int lb, int nx, int ny, i
 //   MODE 1: u = w D^-1 r (the zero-guess first sweep, fused)
 //   MODE 2: u = z_in + P z_c (coarse correction, fused)
 // and z_out = u + w D^-1 (r - L u); DOT: partials of (r, z_out).
+//   MODE 3: u = z_in, no z_out: the residual r - L u restricted to level
+//           l+1 (sum over the 2x2x2 children); x pairs are summed in the
+//           thread (VX >= 2), plane pairs over the z march (even plane count
+//           per tile) and row pairs through LDS.
 // L0: the launch on level 0 (a separate instance, so that profiles keep the
 // bandwidth-bound fine-grid sweeps apart from the coarse levels).
 template <typename T, int VX, int MODE, bool DOT, bool L0>
@@ -520,7 +524,10 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
                                                      int zin, int zout) {
   const int l = L0 ? 0 : l_arg;
   using V = vec_t<T, VX>;
+  static_assert(MODE != 3 || VX >= 2, "the fused restriction pairs cells in the thread");
+  constexpr int NP = VX >= 2 ? VX / 2 : 1;  // coarse cells per thread (MODE 3)
   __shared__ T su[2][kBlock * VX];
+  __shared__ T sacc[MODE == 3 ? kBlock * NP : 1];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   const Rec rc = recs[b2r[b]];
   const SysDev& s = sds[rc.sys];
@@ -555,10 +562,12 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
   const T* __restrict__ zi = (const T*)Lv.z[zin];
   T* __restrict__ zo = (T*)Lv.z[zout];
   const T* __restrict__ zc = nullptr;
+  T* __restrict__ rco = nullptr;
   int cnx = 0, cny = 0;
-  if constexpr (MODE == 2) {
+  if constexpr (MODE == 2 || MODE == 3) {
     const MgLevel& Lc = s.lev[l + 1];
-    zc = (const T*)Lc.z[l + 1 == s.nlev - 1 ? 0 : 1];
+    if constexpr (MODE == 2) zc = (const T*)Lc.z[l + 1 == s.nlev - 1 ? 0 : 1];
+    else rco = (T*)Lc.r;
     cnx = Lc.nx;
     cny = Lc.ny;
   }
@@ -603,6 +612,9 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
   }
   __syncthreads();
   double dot = 0.0;
+  T racc[NP];
+#pragma unroll
+  for (int j = 0; j < NP; ++j) racc[j] = (T)0;
   for (int z = za; z < zb; ++z) {
     const int buf = (z - za) & 1;
     if (active) {
@@ -649,14 +661,39 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
       }
       if (mm) lap += ldv<T, VX>(mm, ic) * uc;
       const V rv = ldv<T, VX>(r, ic);
-      const V out = uc + w * ldv<T, VX>(dinv, ic) * (rv - lap);
-      *reinterpret_cast<V*>(zo + ic) = out;
-      if constexpr (DOT) {
+      if constexpr (MODE == 3) {
+        const V res = rv - lap;
 #pragma unroll
-        for (int v = 0; v < VX; ++v) dot += (double)rv[v] * (double)out[v];
+        for (int j = 0; j < NP; ++j) racc[j] += res[2 * j] + res[2 * j + 1];
+      } else {
+        const V out = uc + w * ldv<T, VX>(dinv, ic) * (rv - lap);
+        *reinterpret_cast<V*>(zo + ic) = out;
+        if constexpr (DOT) {
+#pragma unroll
+          for (int v = 0; v < VX; ++v) dot += (double)rv[v] * (double)out[v];
+        }
       }
       um = uc;
       uc = un;
+    }
+    if constexpr (MODE == 3) {
+      // second plane of a pair (za is even): odd rows hand their sums to
+      // the even row above through LDS; the even row writes the coarse cells
+      if ((z - za) & 1) {
+        if (iy & 1) {
+#pragma unroll
+          for (int j = 0; j < NP; ++j) sacc[threadIdx.x * NP + j] = racc[j];
+        }
+        __syncthreads();
+        if (active && !(iy & 1)) {
+          const long long co =
+              ((long long)(z >> 1) * cny + (y >> 1)) * cnx + (x0 >> 1);
+#pragma unroll
+          for (int j = 0; j < NP; ++j) rco[co + j] = racc[j] + sacc[(threadIdx.x + lx) * NP + j];
+        }
+#pragma unroll
+        for (int j = 0; j < NP; ++j) racc[j] = (T)0;
+      }
     }
     __syncthreads();
   }
@@ -1327,8 +1364,13 @@ static void launch_sweep_l(const LaunchSet& L, hipStream_t st, int l, int zin, i
                  L.b2r_lf[l], l, zin, zout);
       break;
     default:
-      APH_LAUNCH((k_mg_sweep<T, 1, MODE, DOT, L0>), L.nlf[l], st, L.sds, L.rec_lf[l],
-                 L.b2r_lf[l], l, zin, zout);
+      if constexpr (MODE == 3) {
+        APH_LAUNCH((k_mg_restrict<T, L0>), L.nlc[l], st, L.sds, L.rec_lc[l], L.b2r_lc[l], l,
+                   zin);
+      } else {
+        APH_LAUNCH((k_mg_sweep<T, 1, MODE, DOT, L0>), L.nlf[l], st, L.sds, L.rec_lf[l],
+                   L.b2r_lf[l], l, zin, zout);
+      }
       break;
   }
 }
@@ -1365,11 +1407,7 @@ static int launch_vcycle(const LaunchSet& L, hipStream_t st, hipEvent_t* ev = nu
     mark(l == 0 ? 0 : 2);
     launch_sweep<T, 1, false>(L, st, l, 0, 1);
     mark(l == 0 ? 1 : 2);
-    if (l == 0) {
-      APH_LAUNCH((k_mg_restrict<T, true>), L.nlc[l], st, L.sds, L.rec_lc[l], L.b2r_lc[l], l, 1);
-    } else {
-      APH_LAUNCH((k_mg_restrict<T, false>), L.nlc[l], st, L.sds, L.rec_lc[l], L.b2r_lc[l], l, 1);
-    }
+    launch_sweep<T, 3, false>(L, st, l, 1, 1);
   }
   mark(3);
   hipLaunchKernelGGL(k_mg_coarse<T>, dim3(L.ns), dim3(kBlock), L.coarse_lds, st, L.sds,
@@ -1550,6 +1588,101 @@ hipError_t launch_sumsq(const double* v, long long n, double* partials, unsigned
   return hipGetLastError();
 }
 
+// ===========================================================================
+// Projection around the solve (src/solver/proj.ipp:343-403): the pressure
+// system from density, face volume fluxes and source, and the corrected face
+// fluxes for the solution.  FP64 in the reference's operation order, no FMA
+// contraction, so the assembled Expr is bit-identical to the host assembly.
+// Face fields: 3 arrays of n values (x, y, z); entry c of direction d is the
+// lower face of cell c.
+// ===========================================================================
+__global__ __launch_bounds__(kBlock) void k_assemble(
+    const double* __restrict__ rho, const double* __restrict__ ffv,
+    const double* __restrict__ src, int nx, int ny, int nz, double cell, double dt, int sign,
+    double* __restrict__ expr) {
+#pragma clang fp contract(off)
+  const long long n = (long long)nx * ny * nz;
+  const double h = cell;
+  const double vol = h * h * h;
+  const double area = vol / h;
+  const double a = 1.0 / h;
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (long long)gridDim.x * kBlock) {
+    const int x = (int)(i % nx);
+    const long long t = i / nx;
+    const int y = (int)(t % ny);
+    const int z = (int)(t / ny);
+    const long long pl = (long long)nx * ny;
+    const long long nb[6] = {i - x + wrapm(x, nx), i - x + wrapp(x, nx),
+                             i + (long long)(wrapm(y, ny) - y) * nx,
+                             i + (long long)(wrapp(y, ny) - y) * nx,
+                             i + (long long)(wrapm(z, nz) - z) * pl,
+                             i + (long long)(wrapp(z, nz) - z) * pl};
+    double e[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const bool plus = q & 1;
+      const int d = q >> 1;
+      const long long cm = plus ? i : nb[q];
+      const long long cp = plus ? nb[q] : i;
+      const double inv = (1.0 / rho[cp] + 1.0 / rho[cm]) * 0.5;
+      const double dens = 1.0 / inv;
+      const double s = -area / dens * dt;
+      double v0 = -a * s, v1 = a * s, v2 = 0.0 * s;
+      v2 += ffv[(long long)d * n + cp];
+      if (!plus) {
+        v0 = -v0;
+        v1 = -v1;
+        v2 = -v2;
+      }
+      e[0] += plus ? v0 : v1;
+      e[1 + q] += plus ? v1 : v0;
+      e[7] += v2;
+    }
+    if (src) e[7] -= src[i] * vol;
+    double* out = expr + 8 * i;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) out[k] = sign > 0 ? e[k] : -e[k];
+  }
+}
+
+// out[f] = p[cm] * ffe[0] + p[cp] * ffe[1] + ffe[2] (UEB::Eval of an
+// ExprFace, src/solver/approx_eb.h:338-344) with ffe of the assembly.
+__global__ __launch_bounds__(kBlock) void k_flux_correct(
+    const double* __restrict__ p, const double* __restrict__ rho,
+    const double* __restrict__ ffv, double* __restrict__ out, int nx, int ny, int nz,
+    double cell, double dt) {
+#pragma clang fp contract(off)
+  const long long n = (long long)nx * ny * nz;
+  const double h = cell;
+  const double vol = h * h * h;
+  const double area = vol / h;
+  const double a = 1.0 / h;
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (long long)gridDim.x * kBlock) {
+    const int x = (int)(i % nx);
+    const long long t = i / nx;
+    const int y = (int)(t % ny);
+    const int z = (int)(t / ny);
+    const long long pl = (long long)nx * ny;
+    const long long lower[3] = {i - x + wrapm(x, nx), i + (long long)(wrapm(y, ny) - y) * nx,
+                                i + (long long)(wrapm(z, nz) - z) * pl};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const long long cm = lower[d];
+      const double inv = (1.0 / rho[i] + 1.0 / rho[cm]) * 0.5;
+      const double dens = 1.0 / inv;
+      const double s = -area / dens * dt;
+      const double e0 = -a * s, e1 = a * s;
+      double e2 = 0.0 * s;
+      e2 += ffv[(long long)d * n + i];
+      const double t0 = p[cm] * e0;
+      const double t1 = p[i] * e1;
+      out[(long long)d * n + i] = t0 + t1 + e2;
+    }
+  }
+}
+
 template <typename Ti, typename To>
 __global__ __launch_bounds__(kBlock) void k_cast(const Ti* __restrict__ src, To* __restrict__ dst,
                                                  long long n) {
@@ -1571,6 +1704,24 @@ hipError_t launch_cast(const void* src, int src_bytes, void* dst, int dst_bytes,
   } else {
     return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_assemble(const double* rho, const double* ffv, const double* src, int nx,
+                           int ny, int nz, double cell, double dt, int sign, double* expr,
+                           hipStream_t st) {
+  const long long n = (long long)nx * ny * nz;
+  if (n <= 0) return hipSuccess;
+  APH_LAUNCH(k_assemble, grid_for(n), st, rho, ffv, src, nx, ny, nz, cell, dt, sign, expr);
+  return hipGetLastError();
+}
+
+hipError_t launch_flux_correct(const double* p, const double* rho, const double* ffv,
+                               double* out, int nx, int ny, int nz, double cell, double dt,
+                               hipStream_t st) {
+  const long long n = (long long)nx * ny * nz;
+  if (n <= 0) return hipSuccess;
+  APH_LAUNCH(k_flux_correct, grid_for(n), st, p, rho, ffv, out, nx, ny, nz, cell, dt);
   return hipGetLastError();
 }
 

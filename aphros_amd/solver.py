@@ -82,6 +82,32 @@ class Poisson:
     def set_tile(self, lx=0, bz=0, vx=0, sm=0):
         L.check(L.lib().aphpois_set_tile(self.h, lx, bz, vx, sm))
 
+    def assemble(self, rho, ffv, src=None, cell=None, dt=1.0, sign=1):
+        """The projection's pressure system from density (nz, ny, nx), face
+        volume fluxes (3, nz, ny, nx; lower faces) and volume source,
+        assembled on the device and made resident."""
+        nz, ny, nx = self.shape
+        rho = np.ascontiguousarray(rho, np.float64)
+        ffv = np.ascontiguousarray(ffv, np.float64)
+        if rho.size != nx * ny * nz or ffv.size != 3 * rho.size:
+            raise ValueError("rho needs nx*ny*nz values and ffv 3*nx*ny*nz")
+        src = None if src is None else np.ascontiguousarray(src, np.float64)
+        cell = 1.0 / nx if cell is None else cell
+        L.check(L.lib().aphpois_assemble(self.h, L.dptr(rho), L.dptr(ffv), L.dptr(src), cell,
+                                         dt, sign))
+
+    def flux_correct(self, p, rho, ffv, cell=None, dt=1.0):
+        """Corrected face fluxes (3, nz, ny, nx) for the pressure p."""
+        nz, ny, nx = self.shape
+        p = np.ascontiguousarray(p, np.float64)
+        rho = np.ascontiguousarray(rho, np.float64)
+        ffv = np.ascontiguousarray(ffv, np.float64)
+        out = np.empty((3, nz, ny, nx))
+        cell = 1.0 / nx if cell is None else cell
+        L.check(L.lib().aphpois_flux_correct(self.h, L.dptr(p), L.dptr(rho), L.dptr(ffv),
+                                             L.dptr(out), cell, dt))
+        return out
+
     def precond_apply(self, r):
         """z = M r with the multigrid preconditioner (PCG with precond MG);
         r in the positive definite sign convention, shape (nz, ny, nx)."""

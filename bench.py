@@ -68,6 +68,88 @@ def kernel_table(kt, ncell, elem, has_m, per_iter_launches):
     return out
 
 
+def run_batched(args, precond):
+    """Batched workload: independent periodic systems of uneven edges solved
+    together (one set of launches per kernel for all of them), mixed
+    precision to a 1e-8 true relative residual from a zero guess."""
+    from aphros_amd import _lib as L
+    from aphros_amd import synth
+    from aphros_amd.batch import Batch
+    specs = synth.batch_specs(args.batch_systems)
+    b = Batch(precision=L.MIXED, tol=1e-8, precond=precond)
+    t = time.time()
+    slots = []
+    for sp in specs:
+        e, _, _ = synth.spec_system(sp)
+        slots.append((b.add(e), sp["n"]))
+    log(f"[bench] batched: {len(slots)} systems resident in {time.time() - t:.1f}s, "
+        f"device_bytes={b.device_bytes() / 1e9:.2f} GB")
+    b.solve()  # first solve: launch tables and graph capture
+    for sl, n in slots:
+        b.set_rhs(sl, None, np.zeros(n ** 3))
+    t = time.perf_counter()
+    b.solve()
+    dt = time.perf_counter() - t
+    infos = [b.get(sl, want_x=False)[1] for sl, _ in slots]
+    b.close()
+    if not all(i["converged"] for i in infos):
+        raise RuntimeError("batched solve did not converge")
+    cells = sum(n ** 3 for _, n in slots)
+    log(f"[bench] batched: {dt:.3f}s for {len(slots)} solves")
+    return {"systems": len(slots), "cells": cells, "seconds": round(dt, 4),
+            "solves_per_s": round(len(slots) / dt, 3),
+            "iters_min": min(i["iter"] for i in infos),
+            "iters_max": max(i["iter"] for i in infos),
+            "max_true_relres": max(i["true_relres"] for i in infos),
+            "workload": f"{len(slots)} periodic systems, edges from {synth.BATCH_EDGES}, "
+                        f"1-16 spheres, density ratio log-uniform in [10, 1000], seeds 4.."
+                        f"{3 + len(slots)}; mixed precision to 1e-8, zero guess"}
+
+
+def run_served(args, precond):
+    """Served workload: Poisson arrivals of requests, each a sequence of 10
+    correlated right-hand sides (1% perturbation per step, warm started),
+    continuous batching under a 200 GB memory budget."""
+    from aphros_amd import _lib as L
+    from aphros_amd import synth
+    from aphros_amd.serve import Server
+    specs = synth.serve_specs(args.serve_requests, args.serve_rate)
+    systems = [synth.spec_system(sp)[0] for sp in specs]
+    srv = Server(precision=L.MIXED, tol=1e-8, mem_budget=200e9, precond=precond)
+    t0 = time.perf_counter()
+    ids = []
+    for sp, e in zip(specs, systems):
+        delay = sp["arrival"] - (time.perf_counter() - t0)
+        if delay > 0:
+            time.sleep(delay)
+        ids.append(srv.submit(e, nsteps=10, perturb=0.01, seed=sp["seed"]))
+    res = [srv.wait(i) for i in ids]
+    wall = time.perf_counter() - t0
+    srv.close()
+    bad = [r for r in res if r["status"] != 0 or r["steps_done"] != 10]
+    if bad:
+        raise RuntimeError(f"served requests failed: {bad[:2]}")
+    lat = np.array([r["t_done"] - r["t_submit"] for r in res])
+    per_solve = np.array([(r["t_done"] - r["t_admit"]) / r["steps_done"] for r in res])
+    t_first = min(r["t_submit"] for r in res)
+    t_last = max(r["t_done"] for r in res)
+    solves = sum(r["steps_done"] for r in res)
+    log(f"[bench] served: {len(res)} requests, {solves} solves in {t_last - t_first:.2f}s, "
+        f"p50 {np.percentile(lat, 50):.3f}s p99 {np.percentile(lat, 99):.3f}s")
+    return {"requests": len(res), "solves": solves,
+            "solves_per_s": round(solves / (t_last - t_first), 3),
+            "p50_request_s": round(float(np.percentile(lat, 50)), 4),
+            "p99_request_s": round(float(np.percentile(lat, 99)), 4),
+            "p50_solve_s": round(float(np.percentile(per_solve, 50)), 5),
+            "p99_solve_s": round(float(np.percentile(per_solve, 99)), 5),
+            "max_last_relres": max(r["last_relres"] for r in res),
+            "wall_s": round(wall, 3),
+            "workload": f"{len(res)} requests, Poisson arrivals at {args.serve_rate}/s, edges "
+                        f"from {synth.SERVE_EDGES}, 8 spheres, density ratio 1000, 10 "
+                        f"correlated RHS per request (1% perturbation, warm start), 200 GB "
+                        f"budget, mixed precision to 1e-8"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,6 +162,11 @@ def main():
                     help="iterations of the CPU oracle timed for cpu_baseline (0 = skip)")
     ap.add_argument("--no-profile", action="store_true",
                     help="no per-kernel events in the timed region")
+    ap.add_argument("--batch-systems", type=int, default=96,
+                    help="systems of the batched workload (0 = skip)")
+    ap.add_argument("--serve-requests", type=int, default=32,
+                    help="requests of the served workload (0 = skip)")
+    ap.add_argument("--serve-rate", type=float, default=4.0, help="arrivals per second")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -167,6 +254,9 @@ def main():
         ms_it = 1e3 * total / max(1, sum(iters))
         kernels["ms_per_iter_wall"] = round(ms_it, 5)
 
+    batched = run_batched(args, precond) if args.batch_systems > 0 else None
+    served = run_served(args, precond) if args.serve_requests > 0 else None
+
     cpu = None
     if args.cpu_iters > 0:
         from oracle import oracle as O
@@ -216,6 +306,8 @@ def main():
         "mixed_speedup_vs_fp64": round(value / fp64["solves_per_s"], 3) if fp64 else None,
         "roofline": roofline,
         "kernels": kernels,
+        "batched": batched,
+        "served": served,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -154,6 +154,34 @@ long long aphpois_device_bytes(const aphpois* h);
 
 int aphpois_destroy(aphpois* h);
 
+/* ---------------- the projection around the solve ----------------------- */
+
+/* Assembles the projection's pressure system on the device and makes it the
+ * resident system, as aphpois_set_system would with the Expr that
+ * Proj::GetFlux / GetFluxSum build (src/solver/proj.ipp:343-383; harmonic
+ * face density src/solver/approx_eb.h:351-363; AppendExpr
+ * src/geom/mesh.h:575-579), bit for bit: per face f between cells cm < cp,
+ * ExprFace(-1/h, 1/h, 0) * (-area / rho_f * dt) + (0, 0, ffv[f]), summed
+ * over each cell's faces with the outward factor, minus src * volume.
+ * rho: cell density (nx*ny*nz); ffv: face volume fluxes, three arrays of
+ * nx*ny*nz (x, y, z faces; entry c of direction d is the face between c - e_d
+ * and c); src: volume source per cell, or NULL; cell: the cell size h (cubic
+ * cells); sign +1 for the projection's convention (e0 > 0), -1 for the
+ * negated one.  Dense host arrays, x fastest; _device takes device
+ * pointers. */
+int aphpois_assemble(aphpois* h, const double* rho, const double* ffv, const double* src,
+                     double cell, double dt, int sign);
+int aphpois_assemble_device(aphpois* h, const double* rho, const double* ffv,
+                            const double* src, double cell, double dt, int sign);
+/* Corrected face fluxes for the pressure p (Proj::Project stage "fluxes",
+ * src/solver/proj.ipp:399-403; UEB::Eval, src/solver/approx_eb.h:338-344):
+ * ffv_out[f] = p[cm] * ffe[0] + p[cp] * ffe[1] + ffe[2], with ffe of the
+ * assembly above.  Same layouts; ffv_out may not alias ffv. */
+int aphpois_flux_correct(aphpois* h, const double* p, const double* rho, const double* ffv,
+                         double* ffv_out, double cell, double dt);
+int aphpois_flux_correct_device(aphpois* h, const double* p, const double* rho,
+                                const double* ffv, double* ffv_out, double cell, double dt);
+
 /* Message of the last failing call on this thread ("" if none). */
 const char* aphpois_last_error(void);
 

@@ -15,3 +15,18 @@ g++ $FLAGS -o "$OUT/ref_driver.tmp" "$HERE/ref_driver.cpp" \
   "$REF/geom/mesh.cpp" "$REF/util/suspender.cpp" "$REF/parse/vars.cpp" \
   "$REF/util/logger.cpp" "$REF/util/format.cpp"
 mv "$OUT/ref_driver.tmp" "$OUT/ref_driver"
+
+# oracle/_ref/ref_hip_driver: the same harness with the solver created through
+# the reference's module registry, plus the "hip" module of
+# aphros_amd/adapter linked with the product library (a drop-in check: the
+# reference's own stage loop and plugin interface calling libaphpois).
+REPO="$(cd "$HERE/.." && pwd)"
+if [ -f "$REPO/aphros_amd/libaphpois.so" ]; then
+  g++ $FLAGS -DAPH_HIP_MODULE -I"$REPO/include" -o "$OUT/ref_hip_driver.tmp" \
+    "$HERE/ref_driver.cpp" "$REPO/aphros_amd/adapter/linear_hip.cpp" \
+    "$REF/linear/linear.cpp" "$REF/geom/mesh.cpp" "$REF/util/suspender.cpp" \
+    "$REF/parse/vars.cpp" "$REF/util/logger.cpp" "$REF/util/format.cpp" \
+    -L"$REPO/aphros_amd" -laphpois -L/opt/rocm/lib -Wl,-rpath-link,/opt/rocm/lib \
+    -Wl,-rpath,'$ORIGIN/../../aphros_amd'
+  mv "$OUT/ref_hip_driver.tmp" "$OUT/ref_hip_driver"
+fi

@@ -13,6 +13,11 @@
  *                      src/solver/approx_eb.h:351-363,
  *                      src/solver/approx_eb.ipp:835-837,1435-1446,
  *                      src/geom/mesh.h:575-579)
+ *   orc_assemble_flux  the same with face volume fluxes, time step and
+ *                      source (GetFlux / GetFluxSum, src/solver/proj.ipp:343-383,
+ *                      AppendExpr src/geom/mesh.h:575-579)
+ *   orc_flux_correct   corrected face fluxes after the solve (proj.ipp:399-403,
+ *                      UEB::Eval src/solver/approx_eb.h:338-344)
  *   orc_apply          Expr evaluation e0*x + sum_q e[1+q]*x[nb(q)] (+ e7)
  *                      (src/linear/linear.ipp:49-56,65-72,
  *                      src/solver/approx_eb.h:328-336)
@@ -126,6 +131,95 @@ void orc_assemble(int nx, int ny, int nz, double extent, const double* rho,
         }
       }
   (void)n;
+}
+
+/*
+ * The projection's pressure system with face volume fluxes and a time step
+ * (Proj::GetFlux / GetFluxSum, src/solver/proj.ipp:343-383).  Per face f
+ * between cm (lower) and cp (upper):
+ *   ffe = ExprFace(-1/h, 1/h, 0) * s,  s = -area / dens_f * dt
+ *   ffe[2] += ffv[f]                                  (proj.ipp:356-358)
+ * and per cell, for q = 0..5, v = ffe * outward(c, q) appended with
+ *   sum[0] += v[1 - q%2]; sum[1+q] += v[q%2]; sum[7] += v[2]
+ * (src/geom/mesh.h:575-579), then sum[7] -= src * vol (proj.ipp:379).
+ * ffv holds three face fields of nx*ny*nz values (x, y, z faces); entry c of
+ * direction d is the lower face of cell c (between c - e_d and c).
+ * src may be NULL (no source).  sign < 0 negates every coefficient.
+ */
+void orc_assemble_flux(int nx, int ny, int nz, double cell, double dt, const double* rho,
+                       const double* ffv, const double* src, int sign, double* expr) {
+  grid_t g = {nx, ny, nz};
+  const double h = cell;
+  const double vol = h * h * h;
+  const double area = vol / h;
+  const double a = 1.0 / h;
+  const size_t n = (size_t)nx * ny * nz;
+#pragma omp parallel for schedule(static)
+  for (int z = 0; z < nz; ++z)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) {
+        const size_t c = idx(&g, x, y, z);
+        size_t nb[6];
+        neighbours(&g, x, y, z, nb);
+        double e[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int q = 0; q < 6; ++q) {
+          const int plus = q & 1;
+          const int d = q >> 1;
+          const size_t cm = plus ? c : nb[q];
+          const size_t cp = plus ? nb[q] : c;
+          const double inv = (1.0 / rho[cp] + 1.0 / rho[cm]) * 0.5;
+          const double dens = 1.0 / inv;
+          const double s = -area / dens * dt;
+          double v0 = -a * s, v1 = a * s, v2 = 0.0 * s;
+          v2 += ffv[(size_t)d * n + cp];  /* the face's flux: lower face of cp */
+          if (!plus) {
+            v0 = -v0;
+            v1 = -v1;
+            v2 = -v2;
+          }
+          e[0] += plus ? v0 : v1;
+          e[1 + q] += plus ? v1 : v0;
+          e[7] += v2;
+        }
+        if (src) e[7] -= src[c] * vol;
+        for (int k = 0; k < 8; ++k) expr[c * 8 + k] = sign > 0 ? e[k] : -e[k];
+      }
+}
+
+/*
+ * Corrected face fluxes after the pressure solve (Proj::Project "fluxes",
+ * src/solver/proj.ipp:399-403; UEB::Eval of an ExprFace,
+ * src/solver/approx_eb.h:338-344):  out[f] = p[cm] * ffe[0] + p[cp] * ffe[1]
+ * + ffe[2] with ffe of orc_assemble_flux.  Same face-field layout.
+ */
+void orc_flux_correct(int nx, int ny, int nz, double cell, double dt, const double* rho,
+                      const double* ffv, const double* p, double* out) {
+  grid_t g = {nx, ny, nz};
+  const double h = cell;
+  const double vol = h * h * h;
+  const double area = vol / h;
+  const double a = 1.0 / h;
+  const size_t n = (size_t)nx * ny * nz;
+#pragma omp parallel for schedule(static)
+  for (int z = 0; z < nz; ++z)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) {
+        const size_t cp = idx(&g, x, y, z);
+        size_t nb[6];
+        neighbours(&g, x, y, z, nb);
+        for (int d = 0; d < 3; ++d) {
+          const size_t cm = nb[2 * d];
+          const double inv = (1.0 / rho[cp] + 1.0 / rho[cm]) * 0.5;
+          const double dens = 1.0 / inv;
+          const double s = -area / dens * dt;
+          const double e0 = -a * s, e1 = a * s;
+          double e2 = 0.0 * s;
+          e2 += ffv[(size_t)d * n + cp];
+          const double t0 = p[cm] * e0;
+          const double t1 = p[cp] * e1;
+          out[(size_t)d * n + cp] = t0 + t1 + e2;
+        }
+      }
 }
 
 /* y = A x (+ e7 if with_const), reference order: ((e0*x_c [+ e7]) + e1*x_xm) + ... */

@@ -48,6 +48,12 @@ def lib():
         L.orc_pcg_jacobi.argtypes = [_dp, C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_double,
                                      C.c_int, C.c_int, _dp]
         L.orc_pcg_jacobi.restype = C.c_int
+        L.orc_assemble_flux.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, _dp,
+                                         _dp, _dp, C.c_int, _dp]
+        L.orc_assemble_flux.restype = None
+        L.orc_flux_correct.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, _dp,
+                                       _dp, _dp, _dp]
+        L.orc_flux_correct.restype = None
         L.orc_mg_apply.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp]
         L.orc_mg_apply.restype = C.c_int
         L.orc_mgpcg.argtypes = [_dp, C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_int,
@@ -81,6 +87,32 @@ def assemble(rho, f, sign=1, extent=1.0):
     e = np.empty(rho.shape + (8,))
     lib().orc_assemble(nx, ny, nz, extent, _p(rho), _p(f), sign, _p(e))
     return e
+
+
+def assemble_flux(rho, ffv, src=None, cell=None, dt=1.0, sign=1):
+    """Expr (nz,ny,nx,8) of the projection with face volume fluxes ffv
+    (3, nz, ny, nx: x, y, z lower faces of each cell) and volume source."""
+    rho = np.ascontiguousarray(rho, np.float64)
+    nx, ny, nz = _dims(rho.shape)
+    ffv = np.ascontiguousarray(ffv, np.float64)
+    assert ffv.size == 3 * rho.size
+    src = None if src is None else np.ascontiguousarray(src, np.float64)
+    cell = 1.0 / nx if cell is None else cell
+    e = np.empty(rho.shape + (8,))
+    lib().orc_assemble_flux(nx, ny, nz, cell, dt, _p(rho), _p(ffv), _p(src), sign, _p(e))
+    return e
+
+
+def flux_correct(rho, ffv, p, cell=None, dt=1.0):
+    """Corrected face fluxes (3, nz, ny, nx) after the pressure solve."""
+    rho = np.ascontiguousarray(rho, np.float64)
+    nx, ny, nz = _dims(rho.shape)
+    ffv = np.ascontiguousarray(ffv, np.float64)
+    p = np.ascontiguousarray(p, np.float64)
+    cell = 1.0 / nx if cell is None else cell
+    out = np.empty((3,) + rho.shape)
+    lib().orc_flux_correct(nx, ny, nz, cell, dt, _p(rho), _p(ffv), _p(p), _p(out))
+    return out
 
 
 def apply(expr, x, with_const=False):

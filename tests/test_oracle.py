@@ -248,3 +248,71 @@ def test_planted_mistake_is_caught():
     bad[..., 6] = 0.0
     A = _dense(bad)
     assert not np.array_equal(A, A.T)
+
+
+# ----------------------------------------------------- projection with face fluxes
+
+
+def _flux_case(n=12, seed=8):
+    rng = np.random.default_rng(seed)
+    rho = np.where(rng.random((n, n, n)) < 0.4, 1000.0, 1.0)
+    ffv = rng.standard_normal((3, n, n, n)) * 1e-3
+    src = rng.standard_normal((n, n, n))
+    src -= src.mean()  # solvable periodic system
+    return rho, ffv, src
+
+
+def test_assemble_flux_without_fluxes_is_assemble():
+    """Zero fluxes, dt = 1: the pinned assembly bit for bit (both signs)."""
+    rho, _, src = _flux_case()
+    n = rho.shape[0]
+    for sign in (1, -1):
+        e1 = O.assemble(rho, src, sign=sign)
+        e2 = O.assemble_flux(rho, np.zeros((3, n, n, n)), src, cell=1.0 / n, dt=1.0, sign=sign)
+        assert np.array_equal(e1, e2)
+
+
+def test_assemble_flux_constant_term_is_divergence():
+    """e7 = net outward flux - src * vol (the divergence of a uniform flux
+    field vanishes exactly); coefficients scale with dt."""
+    rho, ffv, src = _flux_case()
+    n = rho.shape[0]
+    h = 1.0 / n
+    e = O.assemble_flux(rho, ffv, src, cell=h, dt=0.5)
+    div = (np.roll(ffv[0], -1, axis=2) - ffv[0] + np.roll(ffv[1], -1, axis=1) - ffv[1]
+           + np.roll(ffv[2], -1, axis=0) - ffv[2])
+    assert np.allclose(e[..., 7], div - src * h ** 3, rtol=0, atol=1e-15)
+    uni = np.ones((3, n, n, n)) * np.array([0.3, -0.2, 0.7])[:, None, None, None]
+    e_u = O.assemble_flux(rho, uni, None, cell=h, dt=0.5)
+    assert np.all(np.abs(e_u[..., 7]) <= 1e-16)
+    e1 = O.assemble_flux(rho, ffv, src, cell=h, dt=1.0)
+    assert np.allclose(e[..., :7], 0.5 * e1[..., :7], rtol=1e-15, atol=0)
+
+
+def test_flux_correction_is_the_projection():
+    """The linear system at p is the divergence of the corrected fluxes minus
+    the source: sum_q outward(q) * F_q(p) - src * vol = A p + e7 for any p.
+    A sign or index mistake in either function breaks the identity."""
+    rho, ffv, src = _flux_case()
+    n = rho.shape[0]
+    h = 1.0 / n
+    e = O.assemble_flux(rho, ffv, src, cell=h, dt=0.7)
+    p = np.random.default_rng(2).standard_normal((n, n, n))
+    F = O.flux_correct(rho, ffv, p, cell=h, dt=0.7)
+    div = (np.roll(F[0], -1, axis=2) - F[0] + np.roll(F[1], -1, axis=1) - F[1]
+           + np.roll(F[2], -1, axis=0) - F[2])
+    lhs = div - src * h ** 3
+    rhs = O.apply(e, p, with_const=True)
+    assert np.max(np.abs(lhs - rhs)) <= 1e-12 * np.max(np.abs(rhs))
+    # after a solve the corrected fluxes are divergence free up to the source
+    x, it, rel = O.pcg_jacobi(e, tol=1e-12)
+    F = O.flux_correct(rho, ffv, x, cell=h, dt=0.7)
+    div = (np.roll(F[0], -1, axis=2) - F[0] + np.roll(F[1], -1, axis=1) - F[1]
+           + np.roll(F[2], -1, axis=0) - F[2])
+    assert rel <= 1e-12
+    assert np.max(np.abs(div - src * h ** 3)) <= 1e-9 * np.max(np.abs(src * h ** 3))
+    # planted mistake (the oracle's convention with the faces' roles swapped)
+    Fw = O.flux_correct(rho, ffv, -p, cell=h, dt=0.7)
+    divw = (np.roll(Fw[0], -1, axis=2) - Fw[0] + np.roll(Fw[1], -1, axis=1) - Fw[1]
+            + np.roll(Fw[2], -1, axis=0) - Fw[2])
+    assert np.max(np.abs(divw - src * h ** 3 - O.apply(e, p, with_const=True))) > 1e-6
