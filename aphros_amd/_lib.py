@@ -105,6 +105,12 @@ SIGNATURES = {
                                       C.POINTER(C.c_int)]),
     "aphpois_set_tile": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int]),
     "aphpois_precond_apply": (C.c_int, [_vp, _dp, _dp]),
+    "aphpois_copy_bandwidth": (C.c_int, [C.c_int, _ll, C.c_int, _dp]),
+    "aphpois_system_file_info": (C.c_int, [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                           C.POINTER(C.c_int)]),
+    "aphpois_system_file_read": (C.c_int, [C.c_char_p, _dp, _ll]),
+    "aphpois_system_file_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, _dp]),
+    "aphpois_system_file_error": (C.c_char_p, []),
     "aphpois_assemble": (C.c_int, [_vp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_int]),
     "aphpois_assemble_device": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_double,
                                           C.c_int]),

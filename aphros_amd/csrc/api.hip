@@ -199,6 +199,36 @@ int aphpois_precond_apply(aphpois* h, const double* r, double* z) {
   APH_RET(*h, h->eng.precond_apply(h->slot, r, z));
 }
 
+int aphpois_copy_bandwidth(int device, long long bytes, int reps, double* gbps) {
+  if (bytes < 16 || reps < 1 || !gbps) return set_err(APHPOIS_EINVAL, "bad arguments");
+  bytes &= ~15LL;
+  if (hipSetDevice(device) != hipSuccess) return set_err(APHPOIS_EDEVICE, "no such device");
+  void *a = nullptr, *b = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipMalloc(&a, bytes);
+  if (e == hipSuccess) e = hipMalloc(&b, bytes);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = hipMemsetAsync(a, 0, bytes, st);
+  if (e == hipSuccess) e = aph::launch_copy(a, b, bytes / 16, st);  // warm-up
+  if (e == hipSuccess) e = hipEventRecord(e0, st);
+  for (int r = 0; r < reps && e == hipSuccess; ++r) e = aph::launch_copy(a, b, bytes / 16, st);
+  if (e == hipSuccess) e = hipEventRecord(e1, st);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  if (e == hipSuccess) *gbps = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
+  if (e1) (void)hipEventDestroy(e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (st) (void)hipStreamDestroy(st);
+  if (b) (void)hipFree(b);
+  if (a) (void)hipFree(a);
+  if (e != hipSuccess) return set_err(APHPOIS_EDEVICE, hipGetErrorString(e));
+  return APHPOIS_OK;
+}
+
 int aphpois_destroy(aphpois* h) {
   delete h;
   return APHPOIS_OK;

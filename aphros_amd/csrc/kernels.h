@@ -77,6 +77,8 @@ hipError_t launch_assemble(const double* rho, const double* ffv, const double* s
 hipError_t launch_flux_correct(const double* p, const double* rho, const double* ffv,
                                double* out, int nx, int ny, int nz, double cell, double dt,
                                hipStream_t st);
+// dst = src, n16 16-byte elements (plain streaming copy)
+hipError_t launch_copy(const void* src, void* dst, long long n16, hipStream_t st);
 // dst = (type of dst) src, element sizes 8 -> 4, 4 -> 8 or 8 -> 8 bytes
 hipError_t launch_cast(const void* src, int src_bytes, void* dst, int dst_bytes, long long n,
                        hipStream_t st);

@@ -167,6 +167,10 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=32,
                     help="requests of the served workload (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=4.0, help="arrivals per second")
+    ap.add_argument("--copy-gib", type=float, default=2.0,
+                    help="buffer size of the plain-copy bandwidth measurement (0 = skip)")
+    ap.add_argument("--only-copy", action="store_true",
+                    help="run only the plain-copy measurement (profiler calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -177,6 +181,18 @@ def main():
     from aphros_amd import synth
     from aphros_amd.devmem import DeviceBuffer
     from aphros_amd.solver import Poisson
+
+    copy_gbps = None
+    if args.copy_gib > 0:
+        import ctypes as C
+        g = C.c_double()
+        nbytes = int(args.copy_gib * 2 ** 30)
+        L.check(L.lib().aphpois_copy_bandwidth(0, nbytes, 10, C.byref(g)))
+        copy_gbps = round(g.value, 1)
+        log(f"[bench] plain copy of {nbytes} bytes: {copy_gbps} GB/s (read + write)")
+    if args.only_copy:
+        print(json.dumps({"copy_bytes": int(args.copy_gib * 2 ** 30), "copy_GBps": copy_gbps}))
+        return 0
 
     precond = L.PRECOND_MG if args.precond == "mg" else L.PRECOND_JACOBI
     cfg = synth.CONFIGS[args.config]
@@ -246,7 +262,8 @@ def main():
                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                         "traffic": None, "kernel": k["kernel"] + " [T=float, VX=4]",
                         "alg_bytes_per_launch": k["alg_bytes_per_launch"],
-                        "avg_ms": k["avg_ms"], "launches": k["launches"]}
+                        "avg_ms": k["avg_ms"], "launches": k["launches"],
+                        "plain_copy_GBps": copy_gbps}
         if precond == L.PRECOND_MG and len(kt) > 12:
             kernels["vcycle_total_ms"] = round(kt[8][0], 5)
             kernels["mg_coarse_levels_ms_per_vcycle"] = round(kt[11][0], 5)

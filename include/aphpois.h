@@ -182,6 +182,19 @@ int aphpois_flux_correct(aphpois* h, const double* p, const double* rho, const d
 int aphpois_flux_correct_device(aphpois* h, const double* p, const double* rho,
                                 const double* ffv, double* ffv_out, double cell, double dt);
 
+/* ---------------- system files ------------------------------------------ */
+
+/* The linear system as one array of shape (nz, ny, nx, 8), float64, C order:
+ * the dataset of the reference's --system_in / --system_out
+ * (src/test/linear/main.cpp:181-189; HDF5 there), stored as a NumPy .npy
+ * file (readable by numpy.load without pickles).  No device needed. */
+int aphpois_system_file_info(const char* path, int* nx, int* ny, int* nz);
+/* count must be nx*ny*nz*8. */
+int aphpois_system_file_read(const char* path, double* expr, long long count);
+int aphpois_system_file_write(const char* path, int nx, int ny, int nz, const double* expr);
+/* Message of the last failing system-file call on this thread. */
+const char* aphpois_system_file_error(void);
+
 /* Message of the last failing call on this thread ("" if none). */
 const char* aphpois_last_error(void);
 
@@ -210,6 +223,10 @@ int aphpois_launch_info(aphpois* h, int* stencil_blocks, int* stream_blocks, int
  * thread in x (1, 2 or 4; 4 only in MIXED; must divide nx), sm cells per
  * thread of the streaming kernels. */
 int aphpois_set_tile(aphpois* h, int lx, int bz, int vx, int sm);
+/* Practical HBM bandwidth of a plain streaming copy kernel on `device`:
+ * copies `bytes` (two buffers of that size are allocated and freed) `reps`
+ * times after one warm-up copy; *gbps = 2 * bytes * reps / time. */
+int aphpois_copy_bandwidth(int device, long long bytes, int reps, double* gbps);
 /* z = M r: one application of the multigrid preconditioner (method PCG,
  * precond MG) of the resident system, in the handle's precision; r and z are
  * dense host arrays of nx*ny*nz doubles, x fastest, in the sign convention of
