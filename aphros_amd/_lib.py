@@ -115,6 +115,8 @@ SIGNATURES = {
     "aphpois_assemble_device": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_double,
                                           C.c_int]),
     "aphpois_flux_correct": (C.c_int, [_vp, _dp, _dp, _dp, _dp, C.c_double, C.c_double]),
+    "aphpois_assemble_faces": (C.c_int, [_vp, _dp, _dp, _dp, C.c_double, C.c_int]),
+    "aphpois_flux_correct_faces": (C.c_int, [_vp, _dp, _dp, _dp, _dp, C.c_double]),
     "aphpois_flux_correct_device": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_double,
                                               C.c_double]),
     "aphpois_batch_set_tile": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int]),

@@ -172,25 +172,43 @@ int aphpois_set_tile(aphpois* h, int lx, int bz, int vx, int sm) {
 int aphpois_assemble(aphpois* h, const double* rho, const double* ffv, const double* src,
                      double cell, double dt, int sign) {
   APH_CHECK_H(h);
-  APH_RET(*h, h->eng.assemble(h->slot, rho, ffv, src, cell, dt, sign, false));
+  if (!rho) return set_err(APHPOIS_EINVAL, "rho is NULL");
+  APH_RET(*h, h->eng.assemble(h->slot, rho, nullptr, ffv, src, cell, dt, sign, false));
 }
 
 int aphpois_assemble_device(aphpois* h, const double* rho, const double* ffv,
                             const double* src, double cell, double dt, int sign) {
   APH_CHECK_H(h);
-  APH_RET(*h, h->eng.assemble(h->slot, rho, ffv, src, cell, dt, sign, true));
+  if (!rho) return set_err(APHPOIS_EINVAL, "rho is NULL");
+  APH_RET(*h, h->eng.assemble(h->slot, rho, nullptr, ffv, src, cell, dt, sign, true));
+}
+
+int aphpois_assemble_faces(aphpois* h, const double* fk, const double* ffv, const double* src,
+                           double cell, int sign) {
+  APH_CHECK_H(h);
+  if (!fk) return set_err(APHPOIS_EINVAL, "fk is NULL");
+  APH_RET(*h, h->eng.assemble(h->slot, nullptr, fk, ffv, src, cell, 1.0, sign, false));
 }
 
 int aphpois_flux_correct(aphpois* h, const double* p, const double* rho, const double* ffv,
                          double* ffv_out, double cell, double dt) {
   APH_CHECK_H(h);
-  APH_RET(*h, h->eng.flux_correct(h->slot, p, rho, ffv, ffv_out, cell, dt, false));
+  if (!rho) return set_err(APHPOIS_EINVAL, "rho is NULL");
+  APH_RET(*h, h->eng.flux_correct(h->slot, p, rho, nullptr, ffv, ffv_out, cell, dt, false));
 }
 
 int aphpois_flux_correct_device(aphpois* h, const double* p, const double* rho,
                                 const double* ffv, double* ffv_out, double cell, double dt) {
   APH_CHECK_H(h);
-  APH_RET(*h, h->eng.flux_correct(h->slot, p, rho, ffv, ffv_out, cell, dt, true));
+  if (!rho) return set_err(APHPOIS_EINVAL, "rho is NULL");
+  APH_RET(*h, h->eng.flux_correct(h->slot, p, rho, nullptr, ffv, ffv_out, cell, dt, true));
+}
+
+int aphpois_flux_correct_faces(aphpois* h, const double* p, const double* fk, const double* ffv,
+                               double* ffv_out, double cell) {
+  APH_CHECK_H(h);
+  if (!fk) return set_err(APHPOIS_EINVAL, "fk is NULL");
+  APH_RET(*h, h->eng.flux_correct(h->slot, p, nullptr, fk, ffv, ffv_out, cell, 1.0, false));
 }
 
 int aphpois_precond_apply(aphpois* h, const double* r, double* z) {

@@ -65,14 +65,15 @@ class Engine {
                  long long lz);
   int set_rhs(int slot, const double* e7, bool on_device, long long start, long long ly,
               long long lz);
-  // The projection's pressure system from density, face volume fluxes and a
-  // volume source (dense fields, x fastest), assembled on the device and made
-  // the resident system as by set_system.
-  int assemble(int slot, const double* rho, const double* ffv, const double* src, double cell,
-               double dt, int sign, bool on_device);
+  // The projection's pressure system from density (or face denominators fk;
+  // exactly one of the two), face volume fluxes and a volume source (dense
+  // fields, x fastest), assembled on the device and made the resident system
+  // as by set_system.
+  int assemble(int slot, const double* rho, const double* fk, const double* ffv,
+               const double* src, double cell, double dt, int sign, bool on_device);
   // Corrected face fluxes of the projection for the solution p.
-  int flux_correct(int slot, const double* p, const double* rho, const double* ffv, double* out,
-                   double cell, double dt, bool on_device);
+  int flux_correct(int slot, const double* p, const double* rho, const double* fk,
+                   const double* ffv, double* out, double cell, double dt, bool on_device);
   // x0: nullptr (zero guess), or host/device strided field; keep_x: warm start
   // from the resident solution.
   int prepare(int slot, const double* x0, bool on_device, long long start, long long ly,

@@ -268,27 +268,33 @@ int Engine::add_system(int nx, int ny, int nz, double h3, int* slot) {
     S.dev.z = r;
     S.dev.nlev = 0;
     if (pc_) {
-      // level 0 shares the conductances and the residual of the CG
+      // level 0 shares the residual of the CG and, in FP64, its conductances;
+      // the levels of an FP32 engine store coefficients in bfloat16
+      const int ek = fp32_ ? 2 : 8;
       for (size_t l = 0; l < S.lev.size() && rc == APHPOIS_OK; ++l) {
         MgLevel& L = S.lev[l];
         void *dinv, *z0, *z1;
-        if (l == 0) {
+        if (l == 0 && !fp32_) {
           L.kx = kx;
           L.ky = ky;
           L.kz = kz;
-          L.r = r;
         } else {
-          void *ckx, *cky, *ckz, *cr;
-          A(&ckx, L.n * es);
-          A(&cky, L.n * es);
-          A(&ckz, L.n * es);
-          A(&cr, L.n * es);
+          void *ckx, *cky, *ckz;
+          A(&ckx, L.n * ek);
+          A(&cky, L.n * ek);
+          A(&ckz, L.n * ek);
           L.kx = ckx;
           L.ky = cky;
           L.kz = ckz;
+        }
+        if (l == 0) {
+          L.r = r;
+        } else {
+          void* cr;
+          A(&cr, L.n * es);
           L.r = cr;
         }
-        A(&dinv, L.n * es);
+        A(&dinv, L.n * ek);
         A(&z0, L.n * es);
         A(&z1, L.n * es);
         L.dinv = dinv;
@@ -404,39 +410,44 @@ int Engine::ensure_staging(long long need) {
   return APHPOIS_OK;
 }
 
-int Engine::assemble(int slot, const double* rho, const double* ffv, const double* src,
-                     double cell, double dt, int sign, bool on_device) {
+int Engine::assemble(int slot, const double* rho, const double* fk, const double* ffv,
+                     const double* src, double cell, double dt, int sign, bool on_device) {
   System* sp = sys(slot);
   if (!sp) return fail(APHPOIS_EINVAL, "no such slot");
-  if (!rho || !ffv) return fail(APHPOIS_EINVAL, "rho or ffv is NULL");
+  if ((!rho) == (!fk) || !ffv) return fail(APHPOIS_EINVAL, "need ffv and one of rho, fk");
   if (!(cell > 0) || !(dt > 0) || (sign != 1 && sign != -1))
     return fail(APHPOIS_EINVAL, "need cell > 0, dt > 0 and sign +1 or -1");
   System& s = *sp;
   const long long n = s.n;
   APH_TRY(ensure_staging(n * 8 * 8));
-  const double *drho = rho, *dffv = ffv, *dsrc = src;
+  const double *drho = rho, *dfk = fk, *dffv = ffv, *dsrc = src;
   void* tmp = nullptr;
   if (!on_device) {
-    // inputs through a temporary device buffer (rho, 3 face fields, source)
-    const long long need = n * 8 * (src ? 5 : 4);
+    // inputs through a temporary device buffer: density (n) or face
+    // denominators (3 n), face fluxes (3 n), source (n)
+    const long long nden = rho ? n : 3 * n;
+    const long long need = 8 * (nden + 3 * n + (src ? n : 0));
     if (budget_ > 0 && bytes() + need > budget_)
       return fail(APHPOIS_EBUDGET, "memory budget exceeded by the assembly inputs");
     APH_HIP(hipMalloc(&tmp, need));
     double* t = (double*)tmp;
-    hipError_t e = hipMemcpyAsync(t, rho, n * 8, hipMemcpyHostToDevice, st_);
-    if (e == hipSuccess) e = hipMemcpyAsync(t + n, ffv, 3 * n * 8, hipMemcpyHostToDevice, st_);
+    hipError_t e =
+        hipMemcpyAsync(t, rho ? rho : fk, nden * 8, hipMemcpyHostToDevice, st_);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(t + nden, ffv, 3 * n * 8, hipMemcpyHostToDevice, st_);
     if (e == hipSuccess && src)
-      e = hipMemcpyAsync(t + 4 * n, src, n * 8, hipMemcpyHostToDevice, st_);
+      e = hipMemcpyAsync(t + nden + 3 * n, src, n * 8, hipMemcpyHostToDevice, st_);
     if (e != hipSuccess) {
       (void)hipStreamSynchronize(st_);
       (void)hipFree(tmp);
       return hip(e, "assemble upload");
     }
-    drho = t;
-    dffv = t + n;
-    dsrc = src ? t + 4 * n : nullptr;
+    drho = rho ? t : nullptr;
+    dfk = fk ? t : nullptr;
+    dffv = t + nden;
+    dsrc = src ? t + nden + 3 * n : nullptr;
   }
-  hipError_t e = launch_assemble(drho, dffv, dsrc, s.nx, s.ny, s.nz, cell, dt, sign,
+  hipError_t e = launch_assemble(drho, dfk, dffv, dsrc, s.nx, s.ny, s.nz, cell, dt, sign,
                                  (double*)staging_, st_);
   if (tmp) {
     const hipError_t e2 = hipStreamSynchronize(st_);
@@ -447,27 +458,33 @@ int Engine::assemble(int slot, const double* rho, const double* ffv, const doubl
   return set_system_staged(slot, (const double*)staging_, s.nx, (long long)s.nx * s.ny);
 }
 
-int Engine::flux_correct(int slot, const double* p, const double* rho, const double* ffv,
-                         double* out, double cell, double dt, bool on_device) {
+int Engine::flux_correct(int slot, const double* p, const double* rho, const double* fk,
+                         const double* ffv, double* out, double cell, double dt,
+                         bool on_device) {
   System* sp = sys(slot);
   if (!sp) return fail(APHPOIS_EINVAL, "no such slot");
-  if (!p || !rho || !ffv || !out) return fail(APHPOIS_EINVAL, "NULL argument");
+  if (!p || (!rho) == (!fk) || !ffv || !out)
+    return fail(APHPOIS_EINVAL, "need p, ffv, out and one of rho, fk");
   if (!(cell > 0) || !(dt > 0)) return fail(APHPOIS_EINVAL, "need cell > 0 and dt > 0");
   const long long n = sp->n;
   if (on_device) {
-    APH_HIP(launch_flux_correct(p, rho, ffv, out, sp->nx, sp->ny, sp->nz, cell, dt, st_));
+    APH_HIP(launch_flux_correct(p, rho, fk, ffv, out, sp->nx, sp->ny, sp->nz, cell, dt, st_));
     APH_HIP(hipStreamSynchronize(st_));
     return APHPOIS_OK;
   }
-  // p, rho, ffv in; corrected fluxes out (8 n doubles in flight)
-  APH_TRY(ensure_staging(n * 8 * 8));
+  // p (n), density (n) or face denominators (3 n), fluxes (3 n) in, 3 n out
+  const long long nden = rho ? n : 3 * n;
+  APH_TRY(ensure_staging(8 * (n + nden + 6 * n)));
   double* t = (double*)staging_;
+  double* tden = t + n;
+  double* tffv = tden + nden;
+  double* tout = tffv + 3 * n;
   APH_HIP(hipMemcpyAsync(t, p, n * 8, hipMemcpyHostToDevice, st_));
-  APH_HIP(hipMemcpyAsync(t + n, rho, n * 8, hipMemcpyHostToDevice, st_));
-  APH_HIP(hipMemcpyAsync(t + 2 * n, ffv, 3 * n * 8, hipMemcpyHostToDevice, st_));
-  APH_HIP(launch_flux_correct(t, t + n, t + 2 * n, t + 5 * n, sp->nx, sp->ny, sp->nz, cell, dt,
-                              st_));
-  APH_HIP(hipMemcpyAsync(out, t + 5 * n, 3 * n * 8, hipMemcpyDeviceToHost, st_));
+  APH_HIP(hipMemcpyAsync(tden, rho ? rho : fk, nden * 8, hipMemcpyHostToDevice, st_));
+  APH_HIP(hipMemcpyAsync(tffv, ffv, 3 * n * 8, hipMemcpyHostToDevice, st_));
+  APH_HIP(launch_flux_correct(t, rho ? tden : nullptr, fk ? tden : nullptr, tffv, tout, sp->nx,
+                              sp->ny, sp->nz, cell, dt, st_));
+  APH_HIP(hipMemcpyAsync(out, tout, 3 * n * 8, hipMemcpyDeviceToHost, st_));
   APH_HIP(hipStreamSynchronize(st_));
   return APHPOIS_OK;
 }
@@ -521,7 +538,8 @@ int Engine::set_system_staged(int slot, const double* d, long long ly, long long
       }
       APH_HIP(hipMemcpyAsync(s.d_lev, s.lev.data(), sizeof(MgLevel) * s.lev.size(),
                              hipMemcpyHostToDevice, st_));
-      APH_HIP(launch_mg_setup(fp32_, s.lev.data(), (int)s.lev.size(), st_));
+      APH_HIP(launch_mg_setup(fp32_, s.lev.data(), (int)s.lev.size(), s.dev.kx, s.dev.ky,
+                              s.dev.kz, st_));
     }
   } else {
     if (method_ == kJacobi && (flags & 4u))

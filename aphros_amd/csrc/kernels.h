@@ -37,9 +37,12 @@ struct LaunchSet {
 // the multigrid preconditioner, [8] the V-cycle with its finalize;
 // CONJUGATE: spmv, fin, update, fin; JACOBI: sweep, fin
 int iter_kernels(int method, int pc = 0);
-// Coarse operators and inverse diagonals of levels 0..nlev-1 (level 0 holds
-// the fine conductances already).
-hipError_t launch_mg_setup(int fp32, const MgLevel* levels, int nlev, hipStream_t st);
+// Coefficients of the V-cycle levels: level 0 from the CG's conductances
+// kx, ky, kz (converted to the levels' coefficient type: bfloat16 in FP32
+// engines, where level 0 has arrays of its own; FP64 engines share them),
+// coarse operators, inverse diagonals.
+hipError_t launch_mg_setup(int fp32, const MgLevel* levels, int nlev, const void* kx,
+                           const void* ky, const void* kz, hipStream_t st);
 // The V-cycle with an event recorded before each launch (ev needs
 // vcycle_launches(L) + 1 entries; kinds receives the launch kinds).
 hipError_t launch_vcycle_profiled(int fp32, const LaunchSet& L, hipStream_t st, hipEvent_t* ev,
@@ -69,14 +72,15 @@ hipError_t launch_sumsq(const double* v, long long n, double* partials, unsigned
                         double* out, hipStream_t st);
 hipError_t launch_fill(void* p, long long n, int elem_bytes, hipStream_t st);
 // The projection's pressure system (dense FP64 Expr AoS, x fastest) from
-// density, face volume fluxes (3 n, lower faces) and source (may be null).
-hipError_t launch_assemble(const double* rho, const double* ffv, const double* src, int nx,
-                           int ny, int nz, double cell, double dt, int sign, double* expr,
-                           hipStream_t st);
+// density (or face denominators fk, 3 n; one of rho and fk is null), face
+// volume fluxes (3 n, lower faces) and source (may be null).
+hipError_t launch_assemble(const double* rho, const double* fk, const double* ffv,
+                           const double* src, int nx, int ny, int nz, double cell, double dt,
+                           int sign, double* expr, hipStream_t st);
 // Corrected face fluxes (3 n) for the solution p.
-hipError_t launch_flux_correct(const double* p, const double* rho, const double* ffv,
-                               double* out, int nx, int ny, int nz, double cell, double dt,
-                               hipStream_t st);
+hipError_t launch_flux_correct(const double* p, const double* rho, const double* fk,
+                               const double* ffv, double* out, int nx, int ny, int nz,
+                               double cell, double dt, hipStream_t st);
 // dst = src, n16 16-byte elements (plain streaming copy)
 hipError_t launch_copy(const void* src, void* dst, long long n16, hipStream_t st);
 // dst = (type of dst) src, element sizes 8 -> 4, 4 -> 8 or 8 -> 8 bytes

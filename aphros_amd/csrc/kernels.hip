@@ -10,6 +10,8 @@
 //
 // Stencil kernels march a (lx*VX) x (256/lx) tile of (x, y) columns through
 // bz planes of z, keeping the z-1, z, z+1 values of the column in registers.
+#include <type_traits>
+
 #include "kernels.h"
 #include "reduce.h"
 
@@ -32,6 +34,72 @@ __device__ __forceinline__ void std_(T* __restrict__ p, long long i, vec_t<doubl
 template <typename T, int VX>
 __device__ __forceinline__ vec_t<T, VX> ldv(const T* __restrict__ p, long long i) {
   return *reinterpret_cast<const vec_t<T, VX>*>(p + i);
+}
+
+// Coefficient storage of the multigrid levels: bfloat16 (the upper 16 bits of
+// an IEEE float) in FP32 engines, T itself in FP64 engines.  The V-cycle only
+// has to be a fixed symmetric positive definite operator, so 8-bit mantissas
+// in its coefficients cost nothing in convergence and halve their traffic.
+struct bf16s {
+  unsigned short b;
+};
+template <typename T>
+struct CoefOf {
+  using type = T;
+};
+template <>
+struct CoefOf<float> {
+  using type = bf16s;
+};
+
+__device__ __forceinline__ float bf2f(unsigned short b) {
+  return __uint_as_float((unsigned)b << 16);
+}
+// round to nearest even (finite values)
+__device__ __forceinline__ unsigned short f2bf(float f) {
+  unsigned u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+
+template <typename T, int VX, typename K>
+__device__ __forceinline__ vec_t<T, VX> ldk(const K* __restrict__ p, long long i) {
+  if constexpr (std::is_same<K, bf16s>::value) {
+    vec_t<T, VX> r;
+    if constexpr (VX == 4) {
+      const uint2 w = *reinterpret_cast<const uint2*>(p + i);
+      r[0] = __uint_as_float(w.x << 16);
+      r[1] = __uint_as_float(w.x & 0xffff0000u);
+      r[2] = __uint_as_float(w.y << 16);
+      r[3] = __uint_as_float(w.y & 0xffff0000u);
+    } else if constexpr (VX == 2) {
+      const unsigned w = *reinterpret_cast<const unsigned*>(p + i);
+      r[0] = __uint_as_float(w << 16);
+      r[1] = __uint_as_float(w & 0xffff0000u);
+    } else {
+      r[0] = bf2f(p[i].b);
+    }
+    return r;
+  } else {
+    return ldv<T, VX>(p, i);
+  }
+}
+
+template <typename T, typename K>
+__device__ __forceinline__ T ldk1(const K* __restrict__ p, long long i) {
+  if constexpr (std::is_same<K, bf16s>::value) return bf2f(p[i].b);
+  else return p[i];
+}
+
+template <typename T, typename K>
+__device__ __forceinline__ void stk(K* __restrict__ p, long long i, T v) {
+  if constexpr (std::is_same<K, bf16s>::value) p[i].b = f2bf(v);
+  else p[i] = v;
+}
+
+template <typename A, int VX, typename V>
+__device__ __forceinline__ vec_t<A, VX> cva(V v) {
+  return __builtin_convertvector(v, vec_t<A, VX>);
 }
 
 template <int VX, typename V>
@@ -93,8 +161,11 @@ template <typename T, int VX, bool SC>
 __global__ __launch_bounds__(kBlock) void k_pcg_spmv(const SysDev* __restrict__ sds,
                                                      const Rec* __restrict__ recs,
                                                      const int* __restrict__ b2r) {
-  using D = vec_t<double, VX>;
-  __shared__ double su[2][kBlock * VX];
+  // arithmetic of the operator: FP64 for the Jacobi-scaled form, T with the
+  // multigrid preconditioner (whose V-cycle absorbs FP32 rounding anyway)
+  using A = typename std::conditional<SC, double, T>::type;
+  using D = vec_t<A, VX>;
+  __shared__ A su[2][kBlock * VX];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   const Rec rc = recs[b2r[b]];
   const SysDev& s = sds[rc.sys];
@@ -142,15 +213,15 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv(const SysDev* __restrict__ 
   };
   auto unew = [&](long long i) -> D {
     if constexpr (SC) return ldd<T, VX>(S, i) * cvt<VX>(pnew_st(i));
-    return cvt<VX>(pnew_st(i));
+    return __builtin_convertvector(pnew_st(i), D);
   };
-  auto unew1 = [&](long long i) -> double {
+  auto unew1 = [&](long long i) -> A {
     const double p = (double)(T)((double)r[i] + beta * (double)pin[i]);
     if constexpr (SC) return (double)S[i] * p;
-    return p;
+    return (A)p;
   };
-  // registers: u of planes z-1, z, z+1 in FP64; p, S and kz kept as stored (T)
-  D um = 0.0, uc = 0.0;
+  // registers: u of planes z-1, z, z+1 in A; p, S and kz kept as stored (T)
+  D um = (A)0, uc = (A)0;
   V pc = (T)0, Sc = (T)0, kzm = (T)0;
   if (t.active) {
     const int zm0 = t.za == 0 ? nz - 1 : t.za - 1;
@@ -160,7 +231,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv(const SysDev* __restrict__ 
     i = (long long)t.za * pl + row + t.x0;
     pc = pnew_st(i);
     Sc = sld(i);
-    uc = cvt<VX>(Sc) * cvt<VX>(pc);
+    uc = cva<A, VX>(Sc) * cva<A, VX>(pc);
 #pragma unroll
     for (int v = 0; v < VX; ++v) su[0][lofs + v] = uc[v];
   }
@@ -174,14 +245,14 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv(const SysDev* __restrict__ 
       const long long in = (long long)wrapp(z, nz) * pl + row + t.x0;
       const V pn = pnew_st(in);
       const V Sn = sld(in);
-      const D un = cvt<VX>(Sn) * cvt<VX>(pn);
+      const D un = cva<A, VX>(Sn) * cva<A, VX>(pn);
       if (z + 1 < t.zb) {
 #pragma unroll
         for (int v = 0; v < VX; ++v) su[buf ^ 1][lofs + v] = un[v];
       }
-      const double* sp = su[buf];
+      const A* sp = su[buf];
       // z and y couplings first (frees registers early)
-      D l = cvt<VX>(ldv<T, VX>(kz, ic)) * (uc - un) + cvt<VX>(kzm) * (uc - um);
+      D l = cva<A, VX>(ldv<T, VX>(kz, ic)) * (uc - un) + cva<A, VX>(kzm) * (uc - um);
       kzm = ldv<T, VX>(kz, ic);
       {
         D uy;
@@ -191,33 +262,33 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv(const SysDev* __restrict__ 
         } else {
           uy = unew(base + rowm + t.x0);
         }
-        l += cvt<VX>(ldv<T, VX>(ky, base + rowm + t.x0)) * (uc - uy);
+        l += cva<A, VX>(ldv<T, VX>(ky, base + rowm + t.x0)) * (uc - uy);
         if (yp_in) {
 #pragma unroll
           for (int v = 0; v < VX; ++v) uy[v] = sp[lofs + tx + v];
         } else {
           uy = unew(base + rowp + t.x0);
         }
-        l += cvt<VX>(ldv<T, VX>(ky, ic)) * (uc - uy);
+        l += cva<A, VX>(ldv<T, VX>(ky, ic)) * (uc - uy);
       }
       {
-        const double uxm = xm_in ? sp[lofs - 1] : unew1(base + row + xm);
-        const double uxp = xp_in ? sp[lofs + VX] : unew1(base + row + xp);
-        const D kxc = cvt<VX>(ldv<T, VX>(kx, ic));
-        const double kxm = (double)kx[base + row + xm];
+        const A uxm = xm_in ? sp[lofs - 1] : unew1(base + row + xm);
+        const A uxp = xp_in ? sp[lofs + VX] : unew1(base + row + xp);
+        const D kxc = cva<A, VX>(ldv<T, VX>(kx, ic));
+        const A kxm = (A)kx[base + row + xm];
 #pragma unroll
         for (int v = 0; v < VX; ++v) {
-          const double up = v + 1 < VX ? uc[(v + 1) % VX] : uxp;
-          const double uw = v > 0 ? uc[(v + VX - 1) % VX] : uxm;
-          const double kw = v > 0 ? kxc[(v + VX - 1) % VX] : kxm;
+          const A up = v + 1 < VX ? uc[(v + 1) % VX] : uxp;
+          const A uw = v > 0 ? uc[(v + VX - 1) % VX] : uxm;
+          const A kw = v > 0 ? kxc[(v + VX - 1) % VX] : kxm;
           l[v] += kxc[v] * (uc[v] - up) + kw * (uc[v] - uw);
         }
       }
-      if (has_m) l += cvt<VX>(ldv<T, VX>(mm, ic)) * uc;
-      const V qv = __builtin_convertvector(cvt<VX>(Sc) * l, V);
+      if (has_m) l += cva<A, VX>(ldv<T, VX>(mm, ic)) * uc;
+      const V qv = __builtin_convertvector(cva<A, VX>(Sc) * l, V);
       *reinterpret_cast<V*>(pout + ic) = pc;
       *reinterpret_cast<V*>(q + ic) = qv;
-      const D pcd = cvt<VX>(pc), qd = cvt<VX>(qv);
+      const vec_t<double, VX> pcd = cvt<VX>(pc), qd = cvt<VX>(qv);
 #pragma unroll
       for (int v = 0; v < VX; ++v) dot += pcd[v] * qd[v];
       um = uc;
@@ -455,16 +526,21 @@ constexpr double kOmega = 0.8;
 
 template <typename T>
 struct LevelView {
+  using K = typename CoefOf<T>::type;
   int nx, ny, nz;
   long long pl;
-  const T* kx;
-  const T* ky;
-  const T* kz;
+  const K* kx_;
+  const K* ky_;
+  const K* kz_;
   const T* m;
-  const T* dinv;
+  const K* dinv_;
   __device__ explicit LevelView(const MgLevel& L)
-      : nx(L.nx), ny(L.ny), nz(L.nz), pl((long long)L.nx * L.ny), kx((const T*)L.kx),
-        ky((const T*)L.ky), kz((const T*)L.kz), m((const T*)L.m), dinv((const T*)L.dinv) {}
+      : nx(L.nx), ny(L.ny), nz(L.nz), pl((long long)L.nx * L.ny), kx_((const K*)L.kx),
+        ky_((const K*)L.ky), kz_((const K*)L.kz), m((const T*)L.m), dinv_((const K*)L.dinv) {}
+  __device__ T kx(long long i) const { return ldk1<T>(kx_, i); }
+  __device__ T ky(long long i) const { return ldk1<T>(ky_, i); }
+  __device__ T kz(long long i) const { return ldk1<T>(kz_, i); }
+  __device__ T dinv(long long i) const { return ldk1<T>(dinv_, i); }
   __device__ long long at(int x, int y, int z) const { return (long long)z * pl + (long long)y * nx + x; }
   // (L u)_c where U(i) returns u at flat index i
   template <class U>
@@ -474,8 +550,8 @@ struct LevelView {
     const long long ym = at(x, wrapm(y, ny), z), yp = at(x, wrapp(y, ny), z);
     const long long zm = at(x, y, wrapm(z, nz)), zp = at(x, y, wrapp(z, nz));
     const T uc = u(c);
-    T s = kx[c] * (uc - u(xp)) + kx[xm] * (uc - u(xm)) + ky[c] * (uc - u(yp)) +
-          ky[ym] * (uc - u(ym)) + kz[c] * (uc - u(zp)) + kz[zm] * (uc - u(zm));
+    T s = kx(c) * (uc - u(xp)) + kx(xm) * (uc - u(xm)) + ky(c) * (uc - u(yp)) +
+          ky(ym) * (uc - u(ym)) + kz(c) * (uc - u(zp)) + kz(zm) * (uc - u(zm));
     if (m) s += m[c] * uc;
     return s;
   }
@@ -553,11 +629,12 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
   const bool xp_in = ix + 1 < lx && x0 + VX < nx;
   const bool ym_in = iy > 0;
   const bool yp_in = iy + 1 < ty && y + 1 < ny;
-  const T* __restrict__ kx = (const T*)Lv.kx;
-  const T* __restrict__ ky = (const T*)Lv.ky;
-  const T* __restrict__ kz = (const T*)Lv.kz;
+  using K = typename CoefOf<T>::type;
+  const K* __restrict__ kx = (const K*)Lv.kx;
+  const K* __restrict__ ky = (const K*)Lv.ky;
+  const K* __restrict__ kz = (const K*)Lv.kz;
   const T* __restrict__ mm = (const T*)Lv.m;
-  const T* __restrict__ dinv = (const T*)Lv.dinv;
+  const K* __restrict__ dinv = (const K*)Lv.dinv;
   const T* __restrict__ r = (const T*)Lv.r;
   const T* __restrict__ zi = (const T*)Lv.z[zin];
   T* __restrict__ zo = (T*)Lv.z[zout];
@@ -581,7 +658,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
   auto uv = [&](int xx, int yy, int zz) -> V {
     const long long i = (long long)zz * pl + (long long)yy * nx + xx;
     if constexpr (MODE == 1) {
-      return w * ldv<T, VX>(dinv, i) * ldv<T, VX>(r, i);
+      return w * ldk<T, VX>(dinv, i) * ldv<T, VX>(r, i);
     } else if constexpr (MODE == 2) {
       V u = ldv<T, VX>(zi, i);
       const long long cb = ((long long)(zz >> 1) * cny + (yy >> 1)) * cnx;
@@ -595,7 +672,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
   auto u1 = [&](int xx, int yy, int zz) -> T {
     const long long i = (long long)zz * pl + (long long)yy * nx + xx;
     if constexpr (MODE == 1) {
-      return w * dinv[i] * r[i];
+      return w * ldk1<T>(dinv, i) * r[i];
     } else if constexpr (MODE == 2) {
       return zi[i] + zc[((long long)(zz >> 1) * cny + (yy >> 1)) * cnx + (xx >> 1)];
     } else {
@@ -603,10 +680,19 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
     }
   };
   V um = (T)0, uc = (T)0, kzm = (T)0;
+  // MODE 1: r and D^-1 of the current plane, loaded one plane ahead with u
+  V rcur = (T)0, dcur = (T)0;
   if (active) {
     um = uv(x0, y, wrapm(za, nz));
-    kzm = ldv<T, VX>(kz, (long long)wrapm(za, nz) * pl + row + x0);
-    uc = uv(x0, y, za);
+    kzm = ldk<T, VX>(kz, (long long)wrapm(za, nz) * pl + row + x0);
+    if constexpr (MODE == 1) {
+      const long long i0 = (long long)za * pl + row + x0;
+      rcur = ldv<T, VX>(r, i0);
+      dcur = ldk<T, VX>(dinv, i0);
+      uc = w * dcur * rcur;
+    } else {
+      uc = uv(x0, y, za);
+    }
 #pragma unroll
     for (int v = 0; v < VX; ++v) su[0][lofs + v] = uc[v];
   }
@@ -620,13 +706,21 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
     if (active) {
       const long long base = (long long)z * pl;
       const long long ic = base + row + x0;
-      const V un = uv(x0, y, wrapp(z, nz));
+      V un, rnext = (T)0, dnext = (T)0;
+      if constexpr (MODE == 1) {
+        const long long in = (long long)wrapp(z, nz) * pl + row + x0;
+        rnext = ldv<T, VX>(r, in);
+        dnext = ldk<T, VX>(dinv, in);
+        un = w * dnext * rnext;
+      } else {
+        un = uv(x0, y, wrapp(z, nz));
+      }
       if (z + 1 < zb) {
 #pragma unroll
         for (int v = 0; v < VX; ++v) su[buf ^ 1][lofs + v] = un[v];
       }
       const T* sp = su[buf];
-      const V kzc = ldv<T, VX>(kz, ic);
+      const V kzc = ldk<T, VX>(kz, ic);
       V lap = kzc * (uc - un) + kzm * (uc - um);
       kzm = kzc;
       {
@@ -637,20 +731,20 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
         } else {
           uy = uv(x0, ym, z);
         }
-        lap += ldv<T, VX>(ky, base + (long long)ym * nx + x0) * (uc - uy);
+        lap += ldk<T, VX>(ky, base + (long long)ym * nx + x0) * (uc - uy);
         if (yp_in) {
 #pragma unroll
           for (int v = 0; v < VX; ++v) uy[v] = sp[lofs + tx + v];
         } else {
           uy = uv(x0, yp, z);
         }
-        lap += ldv<T, VX>(ky, ic) * (uc - uy);
+        lap += ldk<T, VX>(ky, ic) * (uc - uy);
       }
       {
         const T uxm = xm_in ? sp[lofs - 1] : u1(xm, y, z);
         const T uxp = xp_in ? sp[lofs + VX] : u1(xp, y, z);
-        const V kxc = ldv<T, VX>(kx, ic);
-        const T kxm = kx[base + row + xm];
+        const V kxc = ldk<T, VX>(kx, ic);
+        const T kxm = ldk1<T>(kx, base + row + xm);
 #pragma unroll
         for (int v = 0; v < VX; ++v) {
           const T up = v + 1 < VX ? uc[(v + 1) % VX] : uxp;
@@ -660,13 +754,18 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
         }
       }
       if (mm) lap += ldv<T, VX>(mm, ic) * uc;
-      const V rv = ldv<T, VX>(r, ic);
+      V rv;
+      if constexpr (MODE == 1) rv = rcur;
+      else rv = ldv<T, VX>(r, ic);
       if constexpr (MODE == 3) {
         const V res = rv - lap;
 #pragma unroll
         for (int j = 0; j < NP; ++j) racc[j] += res[2 * j] + res[2 * j + 1];
       } else {
-        const V out = uc + w * ldv<T, VX>(dinv, ic) * (rv - lap);
+        V dv;
+        if constexpr (MODE == 1) dv = dcur;
+        else dv = ldk<T, VX>(dinv, ic);
+        const V out = uc + w * dv * (rv - lap);
         *reinterpret_cast<V*>(zo + ic) = out;
         if constexpr (DOT) {
 #pragma unroll
@@ -675,6 +774,10 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
       }
       um = uc;
       uc = un;
+      if constexpr (MODE == 1) {
+        rcur = rnext;
+        dcur = dnext;
+      }
     }
     if constexpr (MODE == 3) {
       // second plane of a pair (za is even): odd rows hand their sums to
@@ -754,13 +857,13 @@ __global__ __launch_bounds__(kBlock) void k_mg_coarse(const SysDev* __restrict__
   T* zb = za + n;
   const T* __restrict__ r = (const T*)Lv.r;
   const T w = (T)kOmega;
-  for (int i = threadIdx.x; i < n; i += kBlock) za[i] = w * V.dinv[i] * r[i];
+  for (int i = threadIdx.x; i < n; i += kBlock) za[i] = w * V.dinv(i) * r[i];
   __syncthreads();
   for (int k = 1; k < sweeps; ++k) {
     for (int i = threadIdx.x; i < n; i += kBlock) {
       const int x = i % V.nx, y = (i / V.nx) % V.ny, z = i / (V.nx * V.ny);
       auto u = [&](long long j) -> T { return za[j]; };
-      zb[i] = za[i] + w * V.dinv[i] * (r[i] - V.lap(x, y, z, u));
+      zb[i] = za[i] + w * V.dinv(i) * (r[i] - V.lap(x, y, z, u));
     }
     __syncthreads();
     T* t = za;
@@ -791,15 +894,31 @@ __global__ __launch_bounds__(kBlock) void k_fin_precond(const SysDev* __restrict
   }
 }
 
+// Level-0 coefficients of the V-cycle from the CG's conductances (their
+// coefficient type; a copy only when it differs from T).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mg_level0(MgLevel L, const T* __restrict__ kx,
+                                                      const T* __restrict__ ky,
+                                                      const T* __restrict__ kz) {
+  using K = typename CoefOf<T>::type;
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < L.n;
+       i += (long long)gridDim.x * kBlock) {
+    stk<T>((K*)L.kx, i, kx[i]);
+    stk<T>((K*)L.ky, i, ky[i]);
+    stk<T>((K*)L.kz, i, kz[i]);
+  }
+}
+
 // Coarse operator of level l+1 from level l (one thread per coarse cell):
 // coarse face = scale * (sum of the 4 fine faces it covers), m = scale * sum
-// of the 8 children; then D^-1 of the level.
+// of the 8 children, in T, stored in the coefficient type.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_mg_coarsen(MgLevel f, MgLevel c, T scale) {
+  using K = typename CoefOf<T>::type;
   const long long n = c.n;
-  const T* fkx = (const T*)f.kx;
-  const T* fky = (const T*)f.ky;
-  const T* fkz = (const T*)f.kz;
+  const K* fkx = (const K*)f.kx;
+  const K* fky = (const K*)f.ky;
+  const K* fkz = (const K*)f.kz;
   const T* fm = (const T*)f.m;
   for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
        i += (long long)gridDim.x * kBlock) {
@@ -813,25 +932,28 @@ __global__ __launch_bounds__(kBlock) void k_mg_coarsen(MgLevel f, MgLevel c, T s
     T sx = 0, sy = 0, sz = 0, sm = 0;
     for (int a = 0; a < 2; ++a)
       for (int bb = 0; bb < 2; ++bb) {
-        sx += fkx[F(2 * x + 1, 2 * y + a, 2 * z + bb)];
-        sy += fky[F(2 * x + a, 2 * y + 1, 2 * z + bb)];
-        sz += fkz[F(2 * x + a, 2 * y + bb, 2 * z + 1)];
+        sx += ldk1<T>(fkx, F(2 * x + 1, 2 * y + a, 2 * z + bb));
+        sy += ldk1<T>(fky, F(2 * x + a, 2 * y + 1, 2 * z + bb));
+        sz += ldk1<T>(fkz, F(2 * x + a, 2 * y + bb, 2 * z + 1));
       }
     if (fm) {
       for (int d = 0; d < 8; ++d) sm += fm[F(2 * x + (d & 1), 2 * y + ((d >> 1) & 1), 2 * z + (d >> 2))];
       ((T*)c.m)[i] = scale * sm;
     }
-    ((T*)c.kx)[i] = scale * sx;
-    ((T*)c.ky)[i] = scale * sy;
-    ((T*)c.kz)[i] = scale * sz;
+    stk<T>((K*)c.kx, i, scale * sx);
+    stk<T>((K*)c.ky, i, scale * sy);
+    stk<T>((K*)c.kz, i, scale * sz);
   }
 }
 
+// D^-1 of a level from its stored coefficients (the diagonal of the operator
+// the sweeps apply), stored in the coefficient type.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_mg_dinv(MgLevel L) {
-  const T* kx = (const T*)L.kx;
-  const T* ky = (const T*)L.ky;
-  const T* kz = (const T*)L.kz;
+  using K = typename CoefOf<T>::type;
+  const K* kx = (const K*)L.kx;
+  const K* ky = (const K*)L.ky;
+  const K* kz = (const K*)L.kz;
   const T* m = (const T*)L.m;
   for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < L.n;
        i += (long long)gridDim.x * kBlock) {
@@ -840,11 +962,11 @@ __global__ __launch_bounds__(kBlock) void k_mg_dinv(MgLevel L) {
     const int y = (int)(t % L.ny);
     const int z = (int)(t / L.ny);
     const long long pl = (long long)L.nx * L.ny;
-    T d = kx[i] + kx[(long long)z * pl + (long long)y * L.nx + wrapm(x, L.nx)] + ky[i] +
-          ky[(long long)z * pl + (long long)wrapm(y, L.ny) * L.nx + x] + kz[i] +
-          kz[(long long)wrapm(z, L.nz) * pl + (long long)y * L.nx + x];
+    T d = ldk1<T>(kx, i) + ldk1<T>(kx, (long long)z * pl + (long long)y * L.nx + wrapm(x, L.nx)) +
+          ldk1<T>(ky, i) + ldk1<T>(ky, (long long)z * pl + (long long)wrapm(y, L.ny) * L.nx + x) +
+          ldk1<T>(kz, i) + ldk1<T>(kz, (long long)wrapm(z, L.nz) * pl + (long long)y * L.nx + x);
     if (m) d += m[i];
-    ((T*)L.dinv)[i] = d != (T)0 ? (T)1 / d : (T)0;
+    stk<T>((K*)L.dinv, i, d != (T)0 ? (T)1 / d : (T)0);
   }
 }
 
@@ -1501,7 +1623,12 @@ hipError_t launch_start(int method, int fp32, int vx, const LaunchSet& L, hipStr
 static int grid_for(long long n);
 
 template <typename T>
-static hipError_t mg_setup(const MgLevel* lv, int nlev, hipStream_t st) {
+static hipError_t mg_setup(const MgLevel* lv, int nlev, const void* kx, const void* ky,
+                           const void* kz, hipStream_t st) {
+  if (!std::is_same<typename CoefOf<T>::type, T>::value) {
+    APH_LAUNCH(k_mg_level0<T>, grid_for(lv[0].n), st, lv[0], (const T*)kx, (const T*)ky,
+               (const T*)kz);
+  }
   for (int l = 0; l < nlev; ++l) {
     if (l > 0) {
       APH_LAUNCH(k_mg_coarsen<T>, grid_for(lv[l].n), st, lv[l - 1], lv[l], (T)0.5);
@@ -1511,8 +1638,10 @@ static hipError_t mg_setup(const MgLevel* lv, int nlev, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_mg_setup(int fp32, const MgLevel* levels, int nlev, hipStream_t st) {
-  return fp32 ? mg_setup<float>(levels, nlev, st) : mg_setup<double>(levels, nlev, st);
+hipError_t launch_mg_setup(int fp32, const MgLevel* levels, int nlev, const void* kx,
+                           const void* ky, const void* kz, hipStream_t st) {
+  return fp32 ? mg_setup<float>(levels, nlev, kx, ky, kz, st)
+              : mg_setup<double>(levels, nlev, kx, ky, kz, st);
 }
 
 hipError_t launch_true_res_f7(const LaunchSet& L, hipStream_t st) {
@@ -1594,10 +1723,22 @@ hipError_t launch_sumsq(const double* v, long long n, double* partials, unsigned
 // fluxes for the solution.  FP64 in the reference's operation order, no FMA
 // contraction, so the assembled Expr is bit-identical to the host assembly.
 // Face fields: 3 arrays of n values (x, y, z); entry c of direction d is the
-// lower face of cell c.
+// lower face of cell c.  The face denominator K_f of s = -area / K_f * dt is
+// the harmonic face density from rho (Proj) or given per face in fk (SIMPLE's
+// momentum coefficient, src/solver/simple.ipp:214-226); one of rho and fk is
+// null.
 // ===========================================================================
+__device__ __forceinline__ double face_den(const double* __restrict__ rho,
+                                           const double* __restrict__ fk, long long n, int d,
+                                           long long cm, long long cp) {
+#pragma clang fp contract(off)
+  if (fk) return fk[(long long)d * n + cp];
+  const double inv = (1.0 / rho[cp] + 1.0 / rho[cm]) * 0.5;
+  return 1.0 / inv;
+}
+
 __global__ __launch_bounds__(kBlock) void k_assemble(
-    const double* __restrict__ rho, const double* __restrict__ ffv,
+    const double* __restrict__ rho, const double* __restrict__ fk, const double* __restrict__ ffv,
     const double* __restrict__ src, int nx, int ny, int nz, double cell, double dt, int sign,
     double* __restrict__ expr) {
 #pragma clang fp contract(off)
@@ -1625,8 +1766,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble(
       const int d = q >> 1;
       const long long cm = plus ? i : nb[q];
       const long long cp = plus ? nb[q] : i;
-      const double inv = (1.0 / rho[cp] + 1.0 / rho[cm]) * 0.5;
-      const double dens = 1.0 / inv;
+      const double dens = face_den(rho, fk, n, d, cm, cp);
       const double s = -area / dens * dt;
       double v0 = -a * s, v1 = a * s, v2 = 0.0 * s;
       v2 += ffv[(long long)d * n + cp];
@@ -1649,7 +1789,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble(
 // out[f] = p[cm] * ffe[0] + p[cp] * ffe[1] + ffe[2] (UEB::Eval of an
 // ExprFace, src/solver/approx_eb.h:338-344) with ffe of the assembly.
 __global__ __launch_bounds__(kBlock) void k_flux_correct(
-    const double* __restrict__ p, const double* __restrict__ rho,
+    const double* __restrict__ p, const double* __restrict__ rho, const double* __restrict__ fk,
     const double* __restrict__ ffv, double* __restrict__ out, int nx, int ny, int nz,
     double cell, double dt) {
 #pragma clang fp contract(off)
@@ -1670,8 +1810,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_correct(
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       const long long cm = lower[d];
-      const double inv = (1.0 / rho[i] + 1.0 / rho[cm]) * 0.5;
-      const double dens = 1.0 / inv;
+      const double dens = face_den(rho, fk, n, d, cm, i);
       const double s = -area / dens * dt;
       const double e0 = -a * s, e1 = a * s;
       double e2 = 0.0 * s;
@@ -1683,20 +1822,27 @@ __global__ __launch_bounds__(kBlock) void k_flux_correct(
   }
 }
 
-// Plain copy (the practical HBM bandwidth a streaming kernel reaches): 16
-// bytes per lane per step, grid-stride.
+// Plain copy (the practical HBM bandwidth a streaming kernel reaches): each
+// lane moves 4 x 16 bytes per step (four loads in flight), grid-stride.
 __global__ __launch_bounds__(kBlock) void k_copy(const float4* __restrict__ src,
                                                  float4* __restrict__ dst, long long n) {
-  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
-       i += (long long)gridDim.x * kBlock) {
-    dst[i] = src[i];
+  const long long stride = (long long)gridDim.x * kBlock;
+  long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    const float4 a = src[i], b = src[i + stride], c = src[i + 2 * stride],
+                 d = src[i + 3 * stride];
+    dst[i] = a;
+    dst[i + stride] = b;
+    dst[i + 2 * stride] = c;
+    dst[i + 3 * stride] = d;
   }
+  for (; i < n; i += stride) dst[i] = src[i];
 }
 
 hipError_t launch_copy(const void* src, void* dst, long long n16, hipStream_t st) {
   if (n16 <= 0) return hipSuccess;
   long long g = (n16 + kBlock - 1) / kBlock;
-  if (g > 8192) g = 8192;
+  if (g > 4096) g = 4096;
   hipLaunchKernelGGL(k_copy, dim3((unsigned)g), dim3(kBlock), 0, st, (const float4*)src,
                      (float4*)dst, n16);
   return hipGetLastError();
@@ -1726,21 +1872,21 @@ hipError_t launch_cast(const void* src, int src_bytes, void* dst, int dst_bytes,
   return hipGetLastError();
 }
 
-hipError_t launch_assemble(const double* rho, const double* ffv, const double* src, int nx,
-                           int ny, int nz, double cell, double dt, int sign, double* expr,
-                           hipStream_t st) {
+hipError_t launch_assemble(const double* rho, const double* fk, const double* ffv,
+                           const double* src, int nx, int ny, int nz, double cell, double dt,
+                           int sign, double* expr, hipStream_t st) {
   const long long n = (long long)nx * ny * nz;
   if (n <= 0) return hipSuccess;
-  APH_LAUNCH(k_assemble, grid_for(n), st, rho, ffv, src, nx, ny, nz, cell, dt, sign, expr);
+  APH_LAUNCH(k_assemble, grid_for(n), st, rho, fk, ffv, src, nx, ny, nz, cell, dt, sign, expr);
   return hipGetLastError();
 }
 
-hipError_t launch_flux_correct(const double* p, const double* rho, const double* ffv,
-                               double* out, int nx, int ny, int nz, double cell, double dt,
-                               hipStream_t st) {
+hipError_t launch_flux_correct(const double* p, const double* rho, const double* fk,
+                               const double* ffv, double* out, int nx, int ny, int nz,
+                               double cell, double dt, hipStream_t st) {
   const long long n = (long long)nx * ny * nz;
   if (n <= 0) return hipSuccess;
-  APH_LAUNCH(k_flux_correct, grid_for(n), st, p, rho, ffv, out, nx, ny, nz, cell, dt);
+  APH_LAUNCH(k_flux_correct, grid_for(n), st, p, rho, fk, ffv, out, nx, ny, nz, cell, dt);
   return hipGetLastError();
 }
 

@@ -96,6 +96,30 @@ class Poisson:
         L.check(L.lib().aphpois_assemble(self.h, L.dptr(rho), L.dptr(ffv), L.dptr(src), cell,
                                          dt, sign))
 
+    def assemble_faces(self, fk, ffv, src=None, cell=None, sign=1):
+        """SIMPLE's pressure-correction system: face denominators fk (3, nz,
+        ny, nx) instead of the density."""
+        nz, ny, nx = self.shape
+        fk = np.ascontiguousarray(fk, np.float64)
+        ffv = np.ascontiguousarray(ffv, np.float64)
+        if fk.size != 3 * nx * ny * nz or ffv.size != fk.size:
+            raise ValueError("fk and ffv need 3*nx*ny*nz values")
+        src = None if src is None else np.ascontiguousarray(src, np.float64)
+        cell = 1.0 / nx if cell is None else cell
+        L.check(L.lib().aphpois_assemble_faces(self.h, L.dptr(fk), L.dptr(ffv), L.dptr(src),
+                                               cell, sign))
+
+    def flux_correct_faces(self, p, fk, ffv, cell=None):
+        nz, ny, nx = self.shape
+        p = np.ascontiguousarray(p, np.float64)
+        fk = np.ascontiguousarray(fk, np.float64)
+        ffv = np.ascontiguousarray(ffv, np.float64)
+        out = np.empty((3, nz, ny, nx))
+        cell = 1.0 / nx if cell is None else cell
+        L.check(L.lib().aphpois_flux_correct_faces(self.h, L.dptr(p), L.dptr(fk), L.dptr(ffv),
+                                                   L.dptr(out), cell))
+        return out
+
     def flux_correct(self, p, rho, ffv, cell=None, dt=1.0):
         """Corrected face fluxes (3, nz, ny, nx) for the pressure p."""
         nz, ny, nx = self.shape

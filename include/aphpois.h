@@ -173,6 +173,13 @@ int aphpois_assemble(aphpois* h, const double* rho, const double* ffv, const dou
                      double cell, double dt, int sign);
 int aphpois_assemble_device(aphpois* h, const double* rho, const double* ffv,
                             const double* src, double cell, double dt, int sign);
+/* The same with the face denominator given per face instead of the harmonic
+ * density: ffe = ExprFace(-1/h, 1/h, 0) * (-area / fk[f]) + (0, 0, ffv[f]),
+ * the pressure-correction system of the SIMPLE solver
+ * (SIMPLE::GetFlux / GetFluxSum, src/solver/simple.ipp:214-240,359-372; fk
+ * is its face momentum coefficient ffk).  fk: three face arrays like ffv. */
+int aphpois_assemble_faces(aphpois* h, const double* fk, const double* ffv, const double* src,
+                           double cell, int sign);
 /* Corrected face fluxes for the pressure p (Proj::Project stage "fluxes",
  * src/solver/proj.ipp:399-403; UEB::Eval, src/solver/approx_eb.h:338-344):
  * ffv_out[f] = p[cm] * ffe[0] + p[cp] * ffe[1] + ffe[2], with ffe of the
@@ -181,6 +188,9 @@ int aphpois_flux_correct(aphpois* h, const double* p, const double* rho, const d
                          double* ffv_out, double cell, double dt);
 int aphpois_flux_correct_device(aphpois* h, const double* p, const double* rho,
                                 const double* ffv, double* ffv_out, double cell, double dt);
+/* The SIMPLE form (src/solver/simple.ipp:384-387): face denominators fk. */
+int aphpois_flux_correct_faces(aphpois* h, const double* p, const double* fk, const double* ffv,
+                               double* ffv_out, double cell);
 
 /* ---------------- system files ------------------------------------------ */
 

@@ -145,9 +145,20 @@ void orc_assemble(int nx, int ny, int nz, double extent, const double* rho,
  * ffv holds three face fields of nx*ny*nz values (x, y, z faces); entry c of
  * direction d is the lower face of cell c (between c - e_d and c).
  * src may be NULL (no source).  sign < 0 negates every coefficient.
+ * With fk (and rho NULL) the face denominator is given per face (same layout
+ * as ffv) instead of the harmonic density: the pressure correction of SIMPLE
+ * (src/solver/simple.ipp:214-226, ffe *= -area / ffk; pass dt = 1).
  */
+static double face_den(const double* rho, const double* fk, size_t n, int d, size_t cm,
+                       size_t cp) {
+  if (fk) return fk[(size_t)d * n + cp];
+  const double inv = (1.0 / rho[cp] + 1.0 / rho[cm]) * 0.5;
+  return 1.0 / inv;
+}
+
 void orc_assemble_flux(int nx, int ny, int nz, double cell, double dt, const double* rho,
-                       const double* ffv, const double* src, int sign, double* expr) {
+                       const double* fk, const double* ffv, const double* src, int sign,
+                       double* expr) {
   grid_t g = {nx, ny, nz};
   const double h = cell;
   const double vol = h * h * h;
@@ -167,8 +178,7 @@ void orc_assemble_flux(int nx, int ny, int nz, double cell, double dt, const dou
           const int d = q >> 1;
           const size_t cm = plus ? c : nb[q];
           const size_t cp = plus ? nb[q] : c;
-          const double inv = (1.0 / rho[cp] + 1.0 / rho[cm]) * 0.5;
-          const double dens = 1.0 / inv;
+          const double dens = face_den(rho, fk, n, d, cm, cp);
           const double s = -area / dens * dt;
           double v0 = -a * s, v1 = a * s, v2 = 0.0 * s;
           v2 += ffv[(size_t)d * n + cp];  /* the face's flux: lower face of cp */
@@ -193,7 +203,7 @@ void orc_assemble_flux(int nx, int ny, int nz, double cell, double dt, const dou
  * + ffe[2] with ffe of orc_assemble_flux.  Same face-field layout.
  */
 void orc_flux_correct(int nx, int ny, int nz, double cell, double dt, const double* rho,
-                      const double* ffv, const double* p, double* out) {
+                      const double* fk, const double* ffv, const double* p, double* out) {
   grid_t g = {nx, ny, nz};
   const double h = cell;
   const double vol = h * h * h;
@@ -209,8 +219,7 @@ void orc_flux_correct(int nx, int ny, int nz, double cell, double dt, const doub
         neighbours(&g, x, y, z, nb);
         for (int d = 0; d < 3; ++d) {
           const size_t cm = nb[2 * d];
-          const double inv = (1.0 / rho[cp] + 1.0 / rho[cm]) * 0.5;
-          const double dens = 1.0 / inv;
+          const double dens = face_den(rho, fk, n, d, cm, cp);
           const double s = -area / dens * dt;
           const double e0 = -a * s, e1 = a * s;
           double e2 = 0.0 * s;
@@ -603,8 +612,24 @@ static void mg_free(mglev_t* lev, int nlev) {
   }
 }
 
-/* Levels of the grid (conductances, masses, D^-1); returns nlev or -1/-2. */
-static int mg_setup(const double* expr, int nx, int ny, int nz, mglev_t* lev) {
+/* v rounded to float, then to bfloat16 (round to nearest even) */
+static double bf16_round(double v) {
+  float f = (float)v;
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  u &= 0xffff0000u;
+  memcpy(&f, &u, 4);
+  return (double)f;
+}
+
+/*
+ * Levels of the grid (conductances, masses, D^-1); returns nlev or -1/-2.
+ * coef16: the coefficient storage of the GPU's FP32 engines: conductances and
+ * D^-1 of every level rounded to bfloat16 (level 0 through the FP32
+ * conductances), masses in FP32, coarse sums and D in FP32 arithmetic.
+ */
+static int mg_setup(const double* expr, int nx, int ny, int nz, int coef16, mglev_t* lev) {
   int nlev = 0;
   int lx = nx, ly = ny, lz = nz;
   for (;;) {
@@ -652,6 +677,12 @@ static int mg_setup(const double* expr, int nx, int ny, int nz, mglev_t* lev) {
       const double rs = e[0] + e[1] + e[2] + e[3] + e[4] + e[5] + e[6];
       lev[0].m[c] = fabs(rs) > 1e-12 * fabs(e[0]) ? sgn * rs : 0.0;
     }
+    if (coef16) {
+      lev[0].kx[c] = bf16_round(lev[0].kx[c]);
+      lev[0].ky[c] = bf16_round(lev[0].ky[c]);
+      lev[0].kz[c] = bf16_round(lev[0].kz[c]);
+      if (has_m) lev[0].m[c] = (double)(float)lev[0].m[c];
+    }
   }
   for (int l = 1; l < nlev; ++l) {
     const mglev_t* F = &lev[l - 1];
@@ -670,13 +701,18 @@ static int mg_setup(const double* expr, int nx, int ny, int nz, mglev_t* lev) {
             }
           const size_t c = (size_t)x + (size_t)C->nx * ((size_t)y + (size_t)C->ny * z);
           if (has_m) {
-            for (int d = 0; d < 8; ++d)
-              sm += F->m[idx(&g, 2 * x + (d & 1), 2 * y + ((d >> 1) & 1), 2 * z + (d >> 2))];
-            C->m[c] = 0.5 * sm;
+            float smf = 0;
+            for (int d = 0; d < 8; ++d) {
+              const double v =
+                  F->m[idx(&g, 2 * x + (d & 1), 2 * y + ((d >> 1) & 1), 2 * z + (d >> 2))];
+              sm += v;
+              smf += (float)v;
+            }
+            C->m[c] = coef16 ? (double)(0.5f * smf) : 0.5 * sm;
           }
-          C->kx[c] = 0.5 * sx;
-          C->ky[c] = 0.5 * sy;
-          C->kz[c] = 0.5 * sz;
+          C->kx[c] = coef16 ? bf16_round(0.5 * sx) : 0.5 * sx;
+          C->ky[c] = coef16 ? bf16_round(0.5 * sy) : 0.5 * sy;
+          C->kz[c] = coef16 ? bf16_round(0.5 * sz) : 0.5 * sz;
         }
   }
   for (int l = 0; l < nlev; ++l) {
@@ -691,18 +727,23 @@ static int mg_setup(const double* expr, int nx, int ny, int nz, mglev_t* lev) {
                      L->ky[idx(&g, x, wrap(y - 1, L->ny), z)] + L->kz[c] +
                      L->kz[idx(&g, x, y, wrap(z - 1, L->nz))];
           if (L->m) d += L->m[c];
-          L->dinv[c] = d != 0 ? 1.0 / d : 0.0;
+          if (coef16) {
+            const float df = (float)d;
+            L->dinv[c] = df != 0 ? bf16_round((double)(1.0f / df)) : 0.0;
+          } else {
+            L->dinv[c] = d != 0 ? 1.0 / d : 0.0;
+          }
         }
   }
   return nlev;
 }
 
 /* z = M r for one right-hand side (tests: symmetry of M, action on modes);
- * returns nlev or -1/-2. */
-int orc_mg_apply(const double* expr, int nx, int ny, int nz, int sweeps, const double* r,
-                 double* z) {
+ * coef16 as in mg_setup; returns nlev or -1/-2. */
+int orc_mg_apply(const double* expr, int nx, int ny, int nz, int sweeps, int coef16,
+                 const double* r, double* z) {
   mglev_t lev[12];
-  const int nlev = mg_setup(expr, nx, ny, nz, lev);
+  const int nlev = mg_setup(expr, nx, ny, nz, coef16, lev);
   if (nlev < 0) return nlev;
   memcpy(lev[0].r, r, lev[0].n * sizeof(double));
   mg_vcycle(lev, 0, nlev - 1, sweeps);
@@ -716,7 +757,7 @@ int orc_mgpcg(const double* expr, int nx, int ny, int nz, const double* x0, doub
               double* setup_s) {
   const double t0 = wall_seconds();
   mglev_t lev[12];
-  const int nlev = mg_setup(expr, nx, ny, nz, lev);
+  const int nlev = mg_setup(expr, nx, ny, nz, 0, lev);
   if (nlev < 0) return nlev;
   if (setup_s) *setup_s = wall_seconds() - t0;
   mglev_t* L0 = &lev[0];

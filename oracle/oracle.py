@@ -49,12 +49,12 @@ def lib():
                                      C.c_int, C.c_int, _dp]
         L.orc_pcg_jacobi.restype = C.c_int
         L.orc_assemble_flux.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, _dp,
-                                         _dp, _dp, C.c_int, _dp]
+                                         _dp, _dp, _dp, C.c_int, _dp]
         L.orc_assemble_flux.restype = None
         L.orc_flux_correct.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, _dp,
-                                       _dp, _dp, _dp]
+                                       _dp, _dp, _dp, _dp]
         L.orc_flux_correct.restype = None
-        L.orc_mg_apply.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp]
+        L.orc_mg_apply.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp]
         L.orc_mg_apply.restype = C.c_int
         L.orc_mgpcg.argtypes = [_dp, C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_int,
                                 C.c_int, C.c_int, _dp, _dp]
@@ -89,29 +89,34 @@ def assemble(rho, f, sign=1, extent=1.0):
     return e
 
 
-def assemble_flux(rho, ffv, src=None, cell=None, dt=1.0, sign=1):
+def assemble_flux(rho, ffv, src=None, cell=None, dt=1.0, sign=1, fk=None):
     """Expr (nz,ny,nx,8) of the projection with face volume fluxes ffv
-    (3, nz, ny, nx: x, y, z lower faces of each cell) and volume source."""
-    rho = np.ascontiguousarray(rho, np.float64)
-    nx, ny, nz = _dims(rho.shape)
+    (3, nz, ny, nx: x, y, z lower faces of each cell) and volume source; with
+    fk (rho None) the face denominators are given (SIMPLE)."""
+    shape = ffv.shape[1:]
+    nx, ny, nz = _dims(shape)
+    rho = None if rho is None else np.ascontiguousarray(rho, np.float64)
+    fk = None if fk is None else np.ascontiguousarray(fk, np.float64)
     ffv = np.ascontiguousarray(ffv, np.float64)
-    assert ffv.size == 3 * rho.size
     src = None if src is None else np.ascontiguousarray(src, np.float64)
     cell = 1.0 / nx if cell is None else cell
-    e = np.empty(rho.shape + (8,))
-    lib().orc_assemble_flux(nx, ny, nz, cell, dt, _p(rho), _p(ffv), _p(src), sign, _p(e))
+    e = np.empty(tuple(shape) + (8,))
+    lib().orc_assemble_flux(nx, ny, nz, cell, dt, _p(rho), _p(fk), _p(ffv), _p(src), sign,
+                            _p(e))
     return e
 
 
-def flux_correct(rho, ffv, p, cell=None, dt=1.0):
+def flux_correct(rho, ffv, p, cell=None, dt=1.0, fk=None):
     """Corrected face fluxes (3, nz, ny, nx) after the pressure solve."""
-    rho = np.ascontiguousarray(rho, np.float64)
-    nx, ny, nz = _dims(rho.shape)
+    shape = ffv.shape[1:]
+    nx, ny, nz = _dims(shape)
+    rho = None if rho is None else np.ascontiguousarray(rho, np.float64)
+    fk = None if fk is None else np.ascontiguousarray(fk, np.float64)
     ffv = np.ascontiguousarray(ffv, np.float64)
     p = np.ascontiguousarray(p, np.float64)
     cell = 1.0 / nx if cell is None else cell
-    out = np.empty((3,) + rho.shape)
-    lib().orc_flux_correct(nx, ny, nz, cell, dt, _p(rho), _p(ffv), _p(p), _p(out))
+    out = np.empty((3,) + tuple(shape))
+    lib().orc_flux_correct(nx, ny, nz, cell, dt, _p(rho), _p(fk), _p(ffv), _p(p), _p(out))
     return out
 
 
@@ -179,13 +184,14 @@ def pcg_jacobi(expr, tol=1e-8, maxiter=100000, remove_mean=True, x0=None):
 MG_SWEEPS = 40  # damped Jacobi sweeps on the coarsest level
 
 
-def mg_apply(expr, r, sweeps=MG_SWEEPS):
-    """One V-cycle z = M r of the multigrid preconditioner; returns (z, nlev)."""
+def mg_apply(expr, r, sweeps=MG_SWEEPS, bf16=False):
+    """One V-cycle z = M r of the multigrid preconditioner; returns (z, nlev).
+    bf16: coefficients stored as the GPU's FP32 engines store them."""
     expr = np.ascontiguousarray(expr, np.float64)
     nx, ny, nz = _dims(expr.shape[:3])
     r = np.ascontiguousarray(r, np.float64)
     z = np.empty(expr.shape[:3])
-    nlev = lib().orc_mg_apply(_p(expr), nx, ny, nz, sweeps, _p(r), _p(z))
+    nlev = lib().orc_mg_apply(_p(expr), nx, ny, nz, sweeps, int(bf16), _p(r), _p(z))
     if nlev == -2:
         raise ValueError("grid cannot be coarsened for the multigrid preconditioner")
     if nlev < 0:

@@ -153,3 +153,20 @@ def test_mgpcg_rejects_odd_grid():
     e = _system(None, 1, shape=(6, 6, 5))
     with pytest.raises(ValueError):
         O.mgpcg(e)
+
+
+def test_vcycle_bf16_coefficients():
+    """The FP32 engines' storage (bfloat16 coefficients): still a symmetric
+    positive definite M, within bfloat16 accuracy of the FP64 V-cycle, and
+    different from it (the rounding is applied)."""
+    e = _system(16, 11)
+    rng = np.random.default_rng(5)
+    r1, r2 = rng.standard_normal((2, 16, 16, 16))
+    z1, _ = O.mg_apply(e, r1, bf16=True)
+    z2, _ = O.mg_apply(e, r2, bf16=True)
+    a, b = np.vdot(z1, r2), np.vdot(r1, z2)
+    assert abs(a - b) <= 1e-12 * np.sqrt(np.vdot(z1, r1) * np.vdot(z2, r2))
+    assert np.vdot(z1, r1) > 0
+    zd, _ = O.mg_apply(e, r1)
+    rel = np.abs(z1 - zd).max() / np.abs(zd).max()
+    assert 1e-5 < rel < 3e-2
