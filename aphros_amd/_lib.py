@@ -28,7 +28,8 @@ class Desc(C.Structure):
                 ("method", C.c_int), ("precision", C.c_int), ("resmode", C.c_int),
                 ("tol", C.c_double), ("miniter", C.c_int), ("maxiter", C.c_int),
                 ("nullspace", C.c_int), ("delta", C.c_double), ("check_every", C.c_int),
-                ("device", C.c_int), ("mem_budget", C.c_longlong), ("precond", C.c_int)]
+                ("device", C.c_int), ("mem_budget", C.c_longlong), ("precond", C.c_int),
+                ("mg_w", C.c_int)]
 
 
 class Info(C.Structure):
@@ -45,13 +46,14 @@ class BatchDesc(C.Structure):
     _fields_ = [("precision", C.c_int), ("resmode", C.c_int), ("tol", C.c_double),
                 ("maxiter", C.c_int), ("delta", C.c_double), ("check_every", C.c_int),
                 ("device", C.c_int), ("mem_budget", C.c_longlong), ("max_systems", C.c_int),
-                ("precond", C.c_int)]
+                ("precond", C.c_int), ("mg_w", C.c_int)]
 
 
 class ServerDesc(C.Structure):
     _fields_ = [("precision", C.c_int), ("tol", C.c_double), ("maxiter", C.c_int),
                 ("delta", C.c_double), ("check_every", C.c_int), ("device", C.c_int),
-                ("mem_budget", C.c_longlong), ("max_active", C.c_int), ("precond", C.c_int)]
+                ("mem_budget", C.c_longlong), ("max_active", C.c_int), ("precond", C.c_int),
+                ("mg_w", C.c_int)]
 
 
 class Request(C.Structure):

@@ -13,7 +13,8 @@ from . import _lib as L
 from . import synth
 from .solver import Poisson
 
-CANDIDATES = [(64, 0), (32, 0), (16, 0), (8, 0), (64, 8), (64, 32), (32, 16), (128, 0)]
+CANDIDATES = [(64, 0), (32, 0), (16, 0), (8, 0), (64, 8), (64, 32), (64, 64), (32, 16),
+              (128, 0)]
 
 
 def cache_path():
@@ -40,11 +41,11 @@ def _save(d):
         pass
 
 
-def tune_edge(n, precision=L.MIXED, iters=150, candidates=None):
+def tune_edge(n, precision=L.MIXED, iters=150, candidates=None, precond=L.PRECOND_JACOBI):
     """Returns ((lx, bz), {candidate: spmv ms})."""
     e, _, _ = synth.system(n, 2, 0.1, 0.2, 100.0, 12345)
     s = Poisson((n, n, n), method=L.PCG, precision=precision, resmode=L.RES_REL_TRUE,
-                tol=1e-30, maxiter=iters)
+                tol=1e-30, maxiter=iters, precond=precond)
     s.set_system(e)
     times = {}
     for lx, bz in candidates or CANDIDATES:
@@ -60,12 +61,12 @@ def tune_edge(n, precision=L.MIXED, iters=150, candidates=None):
     return best, times
 
 
-def tile_for(n, precision=L.MIXED, retune=False):
-    key = f"p{precision}_n{n}"
+def tile_for(n, precision=L.MIXED, retune=False, precond=L.PRECOND_JACOBI):
+    key = f"p{precision}_c{precond}_n{n}"
     d = _load()
     if key in d and not retune:
         return tuple(d[key]["tile"])
-    best, times = tune_edge(n, precision)
+    best, times = tune_edge(n, precision, precond=precond)
     d[key] = {"tile": list(best), "ms": {f"{a}x{b}": t for (a, b), t in times.items()}}
     _save(d)
     return best

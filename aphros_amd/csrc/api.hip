@@ -66,7 +66,7 @@ int aphpois_create(const aphpois_desc* desc, aphpois** out) {
   h->desc = *desc;
   int rc = h->eng.init(desc->device, desc->method, desc->precision, desc->resmode, desc->tol,
                        desc->miniter, desc->maxiter, desc->nullspace, desc->delta,
-                       desc->check_every, desc->mem_budget, 1, desc->precond);
+                       desc->check_every, desc->mem_budget, 1, desc->precond, desc->mg_w);
   if (rc == APHPOIS_OK) {
     rc = h->eng.add_system(desc->nx, desc->ny, desc->nz, desc->h3, &h->slot);
   }
@@ -262,7 +262,7 @@ int aphpois_batch_create(const aphpois_batch_desc* d, aphpois_batch** out) {
   b->desc = *d;
   const int rc = b->eng.init(d->device, APHPOIS_PCG, d->precision, d->resmode, d->tol, 0,
                              d->maxiter, APHPOIS_NULL_AUTO, d->delta, d->check_every,
-                             d->mem_budget, d->max_systems, d->precond);
+                             d->mem_budget, d->max_systems, d->precond, d->mg_w);
   if (rc != APHPOIS_OK) {
     set_err(rc, b->eng.err());
     delete b;

@@ -45,13 +45,18 @@ struct Scalars {
 struct MgLevel {
   int nx, ny, nz, pad;
   long long n;
-  const void* kx;  // plus-face conductances (element type of the engine)
+  // coefficients: FP64 engines: plus-face conductances and D^-1 as separate
+  // arrays (kx, ky, kz, dinv); FP32 engines: all four in bfloat16, packed by
+  // groups of 4 cells (kc: kx[4], ky[4], kz[4], dinv[4] per group)
+  const void* kx;
   const void* ky;
   const void* kz;
-  const void* m;   // mass term or nullptr
+  const void* m;   // mass term (element type of the engine) or nullptr
   const void* dinv;
+  const void* kc;
   void* r;         // right-hand side of the level (level 0: the CG residual)
   void* z[2];      // correction, ping-pong
+  void* zs;        // first of two coarse corrections (levels below a W level)
 };
 
 constexpr int kMaxLevels = 12;
@@ -89,6 +94,7 @@ struct SysDev {
   // preconditioned mode (multigrid): the CG direction is built from z
   // (= M^-1 r) instead of r, and S is 1 (S == nullptr)
   int nlev;        // multigrid levels (0 = Jacobi-scaled CG without z)
+  int mg_w;        // levels l < mg_w apply their coarse correction twice
   int nblk_pc;     // workgroups of the level-0 dot in the last smoothing sweep
   const void* z;   // vector the p update reads (r in Jacobi mode)
   const MgLevel* lev;  // device array of nlev levels

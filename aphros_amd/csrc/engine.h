@@ -55,10 +55,11 @@ class Engine {
   Engine() = default;
   ~Engine();
   // returns APHPOIS_* code; message in err()
-  // precond (method PCG): 0 = Jacobi scaling, 1 = multigrid V-cycle
+  // precond (method PCG): 0 = Jacobi scaling, 1 = multigrid cycle; mg_w:
+  // levels of the cycle that apply their coarse correction twice
   int init(int device, int method, int precision, int resmode, double tol, int miniter,
            int maxiter, int nullspace, double delta, int check_every, long long budget,
-           int capacity, int precond = 0);
+           int capacity, int precond = 0, int mg_w = 0);
   bool multigrid() const { return pc_; }
   int add_system(int nx, int ny, int nz, double h3, int* slot);
   int set_system(int slot, const double* expr, bool on_device, long long start, long long ly,
@@ -166,6 +167,7 @@ class Engine {
   int method_ = kConjugate;
   int fp32_ = 0;
   bool pc_ = false;
+  int mg_w_ = 0;
   int resmode_ = 0;
   double tol_ = 0;
   int miniter_ = 1, maxiter_ = 100;

@@ -90,7 +90,7 @@ Engine::~Engine() {
 
 int Engine::init(int device, int method, int precision, int resmode, double tol, int miniter,
                  int maxiter, int nullspace, double delta, int check_every, long long budget,
-                 int capacity, int precond) {
+                 int capacity, int precond, int mg_w) {
   if (method < 0 || method > 2) return fail(APHPOIS_EINVAL, "unknown method");
   if (precision < 0 || precision > 1) return fail(APHPOIS_EINVAL, "unknown precision");
   if (resmode < 0 || resmode > 2) return fail(APHPOIS_EINVAL, "unknown resmode");
@@ -111,6 +111,8 @@ int Engine::init(int device, int method, int precision, int resmode, double tol,
   if (precond == 1 && method != kPcg)
     return fail(APHPOIS_EINVAL, "the multigrid preconditioner needs method PCG");
   pc_ = precond == 1;
+  if (mg_w < 0 || mg_w > kMaxLevels) return fail(APHPOIS_EINVAL, "mg_w out of range");
+  mg_w_ = mg_w;
   resmode_ = resmode;
   tol_ = tol;
   miniter_ = miniter;
@@ -269,23 +271,32 @@ int Engine::add_system(int nx, int ny, int nz, double h3, int* slot) {
     S.dev.nlev = 0;
     if (pc_) {
       // level 0 shares the residual of the CG and, in FP64, its conductances;
-      // the levels of an FP32 engine store coefficients in bfloat16
-      const int ek = fp32_ ? 2 : 8;
+      // the levels of an FP32 engine store kx, ky, kz, D^-1 in bfloat16,
+      // packed by groups of 4 cells (8 bytes per cell)
       for (size_t l = 0; l < S.lev.size() && rc == APHPOIS_OK; ++l) {
         MgLevel& L = S.lev[l];
-        void *dinv, *z0, *z1;
-        if (l == 0 && !fp32_) {
-          L.kx = kx;
-          L.ky = ky;
-          L.kz = kz;
+        void *z0, *z1;
+        if (fp32_) {
+          void* kc = nullptr;
+          A(&kc, (L.n + 3) / 4 * 32);
+          L.kc = kc;
         } else {
-          void *ckx, *cky, *ckz;
-          A(&ckx, L.n * ek);
-          A(&cky, L.n * ek);
-          A(&ckz, L.n * ek);
-          L.kx = ckx;
-          L.ky = cky;
-          L.kz = ckz;
+          void* dinv = nullptr;
+          if (l == 0) {
+            L.kx = kx;
+            L.ky = ky;
+            L.kz = kz;
+          } else {
+            void *ckx, *cky, *ckz;
+            A(&ckx, L.n * es);
+            A(&cky, L.n * es);
+            A(&ckz, L.n * es);
+            L.kx = ckx;
+            L.ky = cky;
+            L.kz = ckz;
+          }
+          A(&dinv, L.n * es);
+          L.dinv = dinv;
         }
         if (l == 0) {
           L.r = r;
@@ -294,12 +305,19 @@ int Engine::add_system(int nx, int ny, int nz, double h3, int* slot) {
           A(&cr, L.n * es);
           L.r = cr;
         }
-        A(&dinv, L.n * ek);
         A(&z0, L.n * es);
         A(&z1, L.n * es);
-        L.dinv = dinv;
         L.z[0] = z0;
         L.z[1] = z1;
+      }
+      // W levels l < w (their coarse level l + 1 below the coarsest) keep the
+      // first of two coarse corrections in zs of level l + 1
+      const int w = std::max(0, std::min(mg_w_, (int)S.lev.size() - 2));
+      S.dev.mg_w = w;
+      for (int l = 1; l <= w && rc == APHPOIS_OK; ++l) {
+        void* zs = nullptr;
+        A(&zs, S.lev[l].n * es);
+        S.lev[l].zs = zs;
       }
       void* dl = nullptr;
       A(&dl, (long long)sizeof(MgLevel) * (long long)S.lev.size());
@@ -683,7 +701,9 @@ int Engine::build_tables() {
     System& s = *slots_[launch_slots_[k]];
     // The tiling of a system depends on that system only, so its reduction
     // trees (and results) are the same alone and inside a batch.
-    const long long tgt = std::min(3072LL, std::max(1LL, cdiv(s.n, 2048)));
+    // ~one workgroup per 2048 cells, at most 1024 (about one resident wave
+    // set of the 256 CUs: long z marches, few re-read halo planes)
+    const long long tgt = std::min(1024LL, std::max(1LL, cdiv(s.n, 2048)));
     // per-system tile (autotuner) over the engine-wide override
     Tuning tu = tuning_;
     if (s.tune_lx > 0) tu.lx = s.tune_lx;
@@ -718,8 +738,11 @@ int Engine::build_tables() {
       for (int l = 0; l + 1 < nl; ++l) {
         const MgLevel& F = s.lev[l];
         const MgLevel& C = s.lev[l + 1];
-        Rec f = make_stencil_rec(F.nx, F.ny, F.nz, vxl[l],
-                                 std::min(3072LL, std::max(1LL, cdiv(F.n, 2048))), none);
+        // large levels: long z marches as above; small levels (latency
+        // bound): short marches over more workgroups
+        const long long ftgt = F.n >= (1LL << 21) ? std::min(1024LL, cdiv(F.n, 2048))
+                                                  : std::max(1LL, cdiv(F.n, 512));
+        Rec f = make_stencil_rec(F.nx, F.ny, F.nz, vxl[l], ftgt, none);
         // the fused restriction sums plane pairs inside one workgroup: even
         // plane counts per tile (F.nz is even on every coarsened level)
         if (f.bz & 1) {
@@ -815,6 +838,8 @@ int Engine::build_tables() {
   L_.ns = ns;
   L_.pc = pc_ ? 1 : 0;
   L_.nlev_max = nlev_max;
+  L_.mg_w = 0;
+  for (int i : launch_slots_) L_.mg_w = std::max(L_.mg_w, slots_[i]->dev.mg_w);
   for (int l = 0; l < kMaxLevels; ++l) L_.vx_l[l] = vxl[l];
   L_.coarse_sweeps = 40;
   L_.coarse_lds = (int)(2 * coarse_max * (fp32_ ? 4 : 8));

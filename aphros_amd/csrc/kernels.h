@@ -21,6 +21,7 @@ struct LaunchSet {
   // over the systems that coarsen below l
   int pc = 0;
   int nlev_max = 0;
+  int mg_w = 0;  // largest number of W levels over the systems
   int coarse_sweeps = 40;
   int coarse_lds = 0;  // dynamic LDS bytes of the coarsest-level kernel
   int vx_l[kMaxLevels] = {};  // cells per thread in x of the level-l sweeps

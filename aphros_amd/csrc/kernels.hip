@@ -36,22 +36,10 @@ __device__ __forceinline__ vec_t<T, VX> ldv(const T* __restrict__ p, long long i
   return *reinterpret_cast<const vec_t<T, VX>*>(p + i);
 }
 
-// Coefficient storage of the multigrid levels: bfloat16 (the upper 16 bits of
-// an IEEE float) in FP32 engines, T itself in FP64 engines.  The V-cycle only
-// has to be a fixed symmetric positive definite operator, so 8-bit mantissas
-// in its coefficients cost nothing in convergence and halve their traffic.
-struct bf16s {
-  unsigned short b;
-};
-template <typename T>
-struct CoefOf {
-  using type = T;
-};
-template <>
-struct CoefOf<float> {
-  using type = bf16s;
-};
-
+// bfloat16 (the upper 16 bits of an IEEE float): the V-cycle coefficients of
+// FP32 engines.  The V-cycle only has to be a fixed symmetric positive
+// definite operator, so 8-bit mantissas in its coefficients cost nothing in
+// convergence and halve their traffic.
 __device__ __forceinline__ float bf2f(unsigned short b) {
   return __uint_as_float((unsigned)b << 16);
 }
@@ -62,44 +50,101 @@ __device__ __forceinline__ unsigned short f2bf(float f) {
   return (unsigned short)(u >> 16);
 }
 
-template <typename T, int VX, typename K>
-__device__ __forceinline__ vec_t<T, VX> ldk(const K* __restrict__ p, long long i) {
-  if constexpr (std::is_same<K, bf16s>::value) {
-    vec_t<T, VX> r;
+template <typename A, int VX, typename V>
+__device__ __forceinline__ vec_t<A, VX> cva(V v) {
+  return __builtin_convertvector(v, vec_t<A, VX>);
+}
+
+// Coefficients of a multigrid level: component c = 0..3 (kx, ky, kz, D^-1)
+// of the VX cells from flat index i (i % VX == 0).  FP64: separate arrays;
+// FP32: bfloat16 packed by groups of 4 cells (16 values per group: kx[4],
+// ky[4], kz[4], dinv[4]), so a thread with 4 cells reads all four components
+// of its cells with two 16-byte loads.
+template <typename T>
+struct MgCoefs {
+  const T* a[4];
+  __device__ explicit MgCoefs(const MgLevel& L)
+      : a{(const T*)L.kx, (const T*)L.ky, (const T*)L.kz, (const T*)L.dinv} {}
+  template <int VX>
+  __device__ vec_t<T, VX> comp(int c, long long i) const {
+    return ldv<T, VX>(a[c], i);
+  }
+  __device__ T comp1(int c, long long i) const { return a[c][i]; }
+  template <int VX>
+  __device__ void own(long long i, vec_t<T, VX>& kx, vec_t<T, VX>& ky, vec_t<T, VX>& kz,
+                      vec_t<T, VX>& d) const {
+    kx = comp<VX>(0, i);
+    ky = comp<VX>(1, i);
+    kz = comp<VX>(2, i);
+    d = comp<VX>(3, i);
+  }
+};
+
+template <>
+struct MgCoefs<float> {
+  const unsigned short* kc;
+  __device__ explicit MgCoefs(const MgLevel& L) : kc((const unsigned short*)L.kc) {}
+  __device__ static long long at(int c, long long i) { return ((i >> 2) << 4) + 4 * c + (i & 3); }
+  template <int VX>
+  __device__ vec_t<float, VX> comp(int c, long long i) const {
+    vec_t<float, VX> r;
+    const unsigned short* p = kc + at(c, i);
     if constexpr (VX == 4) {
-      const uint2 w = *reinterpret_cast<const uint2*>(p + i);
+      const uint2 w = *reinterpret_cast<const uint2*>(p);
       r[0] = __uint_as_float(w.x << 16);
       r[1] = __uint_as_float(w.x & 0xffff0000u);
       r[2] = __uint_as_float(w.y << 16);
       r[3] = __uint_as_float(w.y & 0xffff0000u);
     } else if constexpr (VX == 2) {
-      const unsigned w = *reinterpret_cast<const unsigned*>(p + i);
+      const unsigned w = *reinterpret_cast<const unsigned*>(p);
       r[0] = __uint_as_float(w << 16);
       r[1] = __uint_as_float(w & 0xffff0000u);
     } else {
-      r[0] = bf2f(p[i].b);
+      r[0] = bf2f(p[0]);
     }
     return r;
-  } else {
-    return ldv<T, VX>(p, i);
   }
-}
+  __device__ float comp1(int c, long long i) const { return bf2f(kc[at(c, i)]); }
+  template <int VX>
+  __device__ void own(long long i, vec_t<float, VX>& kx, vec_t<float, VX>& ky,
+                      vec_t<float, VX>& kz, vec_t<float, VX>& d) const {
+    if constexpr (VX == 4) {
+      const uint4* g = reinterpret_cast<const uint4*>(kc + ((i >> 2) << 4));
+      const uint4 w0 = g[0], w1 = g[1];
+      kx[0] = __uint_as_float(w0.x << 16);
+      kx[1] = __uint_as_float(w0.x & 0xffff0000u);
+      kx[2] = __uint_as_float(w0.y << 16);
+      kx[3] = __uint_as_float(w0.y & 0xffff0000u);
+      ky[0] = __uint_as_float(w0.z << 16);
+      ky[1] = __uint_as_float(w0.z & 0xffff0000u);
+      ky[2] = __uint_as_float(w0.w << 16);
+      ky[3] = __uint_as_float(w0.w & 0xffff0000u);
+      kz[0] = __uint_as_float(w1.x << 16);
+      kz[1] = __uint_as_float(w1.x & 0xffff0000u);
+      kz[2] = __uint_as_float(w1.y << 16);
+      kz[3] = __uint_as_float(w1.y & 0xffff0000u);
+      d[0] = __uint_as_float(w1.z << 16);
+      d[1] = __uint_as_float(w1.z & 0xffff0000u);
+      d[2] = __uint_as_float(w1.w << 16);
+      d[3] = __uint_as_float(w1.w & 0xffff0000u);
+    } else {
+      kx = comp<VX>(0, i);
+      ky = comp<VX>(1, i);
+      kz = comp<VX>(2, i);
+      d = comp<VX>(3, i);
+    }
+  }
+};
 
-template <typename T, typename K>
-__device__ __forceinline__ T ldk1(const K* __restrict__ p, long long i) {
-  if constexpr (std::is_same<K, bf16s>::value) return bf2f(p[i].b);
-  else return p[i];
-}
-
-template <typename T, typename K>
-__device__ __forceinline__ void stk(K* __restrict__ p, long long i, T v) {
-  if constexpr (std::is_same<K, bf16s>::value) p[i].b = f2bf(v);
-  else p[i] = v;
-}
-
-template <typename A, int VX, typename V>
-__device__ __forceinline__ vec_t<A, VX> cva(V v) {
-  return __builtin_convertvector(v, vec_t<A, VX>);
+// stores component c of cell i (setup kernels)
+template <typename T>
+__device__ __forceinline__ void st_coef(const MgLevel& L, int c, long long i, T v) {
+  if constexpr (std::is_same<T, float>::value) {
+    ((unsigned short*)L.kc)[MgCoefs<float>::at(c, i)] = f2bf(v);
+  } else {
+    const void* a[4] = {L.kx, L.ky, L.kz, L.dinv};
+    ((T*)a[c])[i] = v;
+  }
 }
 
 template <int VX, typename V>
@@ -421,18 +466,22 @@ __global__ __launch_bounds__(kBlock) void k_fin_ru_accum(const SysDev* __restric
 
 // Reliable update, part 2: r64 = b - A (x64 - mean) in FP64 with the user's
 // symmetric coefficients; r^ = sign * S * r64; stop test on the true residual;
-// beta from the replaced residual.
+// beta from the replaced residual.  One cell per thread marching through z:
+// x of planes z-1, z, z+1 and e_z of plane z-1 in registers, the x plane of
+// the tile in LDS for the x and y neighbours.
 template <typename T, int VX>
 __global__ __launch_bounds__(kBlock) void k_ru_resid(const SysDev* __restrict__ sds,
                                                      const Rec* __restrict__ recs,
                                                      const int* __restrict__ b2r) {
+  static_assert(VX == 1, "one cell per thread");
+  __shared__ double su[2][kBlock];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   const Rec rc = recs[b2r[b]];
   const SysDev& s = sds[rc.sys];
   Scalars* sc = s.sc;
   if (sc->done || !sc->ru) return;
   const int lb = b - rc.blk0;
-  const Tile t = tile_of<VX>(rc, s, lb);
+  const Tile t = tile_of<1>(rc, s, lb);
   const int nx = s.nx, ny = s.ny, nz = s.nz;
   const double mp = sc->mean_pending;
   const double sgn = s.neg ? -1.0 : 1.0;
@@ -444,35 +493,59 @@ __global__ __launch_bounds__(kBlock) void k_ru_resid(const SysDev* __restrict__ 
   const double* __restrict__ ez = s.s4[3];
   const T* __restrict__ S = (const T*)s.S;
   T* __restrict__ r = (T*)s.r;
+  const int lx = rc.lx;
+  const int ix = threadIdx.x % lx, iy = threadIdx.x / lx;
+  const int ty = kBlock / lx;
+  const bool xm_in = ix > 0;
+  const bool xp_in = ix + 1 < lx && t.x0 + 1 < nx;
+  const bool ym_in = iy > 0;
+  const bool yp_in = iy + 1 < ty && t.y + 1 < ny;
+  const long long pl = (long long)nx * ny;
+  const long long row = (long long)t.y * nx;
+  const long long rowm = (long long)wrapm(t.y, ny) * nx;
+  const long long rowp = (long long)wrapp(t.y, ny) * nx;
+  const int xw = wrapm(t.x0, nx), xe = wrapp(t.x0, nx);
   double acc[3] = {0.0, 0.0, 0.0};
+  double um = 0.0, uc = 0.0, ezm = 0.0;
   if (t.active) {
-    const long long pl = (long long)nx * ny;
-    const long long row = (long long)t.y * nx;
-    const long long rowm = (long long)wrapm(t.y, ny) * nx;
-    const long long rowp = (long long)wrapp(t.y, ny) * nx;
-    for (int z = t.za; z < t.zb; ++z) {
+    const long long im = (long long)wrapm(t.za, nz) * pl + row + t.x0;
+    um = x[im] - mp;
+    ezm = ez[im];
+    uc = x[(long long)t.za * pl + row + t.x0] - mp;
+    su[0][threadIdx.x] = uc;
+  }
+  __syncthreads();
+  for (int z = t.za; z < t.zb; ++z) {
+    const int buf = (z - t.za) & 1;
+    if (t.active) {
       const long long base = (long long)z * pl;
-      const long long bm = (long long)wrapm(z, nz) * pl, bp = (long long)wrapp(z, nz) * pl;
-      for (int v = 0; v < VX; ++v) {
-        const int xx = t.x0 + v;
-        const int xw = wrapm(xx, nx), xe = wrapp(xx, nx);
-        const long long c = base + row + xx;
-        const double xc = x[c] - mp;
-        double ax = e0[c] * xc;
-        ax += ex[base + row + xw] * (x[base + row + xw] - mp);
-        ax += ex[c] * (x[base + row + xe] - mp);
-        ax += ey[base + rowm + xx] * (x[base + rowm + xx] - mp);
-        ax += ey[c] * (x[base + rowp + xx] - mp);
-        ax += ez[bm + row + xx] * (x[bm + row + xx] - mp);
-        ax += ez[c] * (x[bp + row + xx] - mp);
-        const double r64 = bb[c] - ax;
-        const T rh = (T)(sgn * (S ? (double)S[c] : 1.0) * r64);
-        r[c] = rh;
-        acc[0] += (double)rh * (double)rh;
-        acc[1] += r64 * r64;
-        acc[2] = fmax(acc[2], fabs(r64));
-      }
+      const long long c = base + row + t.x0;
+      const double un = x[(long long)wrapp(z, nz) * pl + row + t.x0] - mp;
+      if (z + 1 < t.zb) su[buf ^ 1][threadIdx.x] = un;
+      const double* sp = su[buf];
+      const double uxm = xm_in ? sp[threadIdx.x - 1] : x[base + row + xw] - mp;
+      const double uxp = xp_in ? sp[threadIdx.x + 1] : x[base + row + xe] - mp;
+      const double uym = ym_in ? sp[threadIdx.x - lx] : x[base + rowm + t.x0] - mp;
+      const double uyp = yp_in ? sp[threadIdx.x + lx] : x[base + rowp + t.x0] - mp;
+      const double ezc = ez[c];
+      double ax = e0[c] * uc;
+      ax += ex[base + row + xw] * uxm;
+      ax += ex[c] * uxp;
+      ax += ey[base + rowm + t.x0] * uym;
+      ax += ey[c] * uyp;
+      ax += ezm * um;
+      ax += ezc * un;
+      const double r64 = bb[c] - ax;
+      const T rh = (T)(sgn * (S ? (double)S[c] : 1.0) * r64);
+      r[c] = rh;
+      acc[0] += (double)rh * (double)rh;
+      acc[1] += r64 * r64;
+      acc[2] = fmax(acc[2], fabs(r64));
+      ezm = ezc;
+      um = uc;
+      uc = un;
     }
+    __syncthreads();
   }
   block_partial<3, 4u>(acc, s.partials, rc.nblk, lb);
 }
@@ -522,25 +595,20 @@ __global__ __launch_bounds__(kBlock) void k_fin_ru_resid(const SysDev* __restric
 // injection (its transpose), so the V-cycle is a symmetric linear operator.
 // Arithmetic in the engine's element type T.
 // ===========================================================================
-constexpr double kOmega = 0.8;
+constexpr double kOmega = 0.9;
 
 template <typename T>
 struct LevelView {
-  using K = typename CoefOf<T>::type;
   int nx, ny, nz;
   long long pl;
-  const K* kx_;
-  const K* ky_;
-  const K* kz_;
+  MgCoefs<T> co;
   const T* m;
-  const K* dinv_;
   __device__ explicit LevelView(const MgLevel& L)
-      : nx(L.nx), ny(L.ny), nz(L.nz), pl((long long)L.nx * L.ny), kx_((const K*)L.kx),
-        ky_((const K*)L.ky), kz_((const K*)L.kz), m((const T*)L.m), dinv_((const K*)L.dinv) {}
-  __device__ T kx(long long i) const { return ldk1<T>(kx_, i); }
-  __device__ T ky(long long i) const { return ldk1<T>(ky_, i); }
-  __device__ T kz(long long i) const { return ldk1<T>(kz_, i); }
-  __device__ T dinv(long long i) const { return ldk1<T>(dinv_, i); }
+      : nx(L.nx), ny(L.ny), nz(L.nz), pl((long long)L.nx * L.ny), co(L), m((const T*)L.m) {}
+  __device__ T kx(long long i) const { return co.comp1(0, i); }
+  __device__ T ky(long long i) const { return co.comp1(1, i); }
+  __device__ T kz(long long i) const { return co.comp1(2, i); }
+  __device__ T dinv(long long i) const { return co.comp1(3, i); }
   __device__ long long at(int x, int y, int z) const { return (long long)z * pl + (long long)y * nx + x; }
   // (L u)_c where U(i) returns u at flat index i
   template <class U>
@@ -587,6 +655,8 @@ __device__ __forceinline__ Cell cell_of(const Rec& rc, int lb, int nx, int ny, i
 //   MODE 1: u = w D^-1 r (the zero-guess first sweep, fused)
 //   MODE 2: u = z_in + P z_c (coarse correction, fused)
 // and z_out = u + w D^-1 (r - L u); DOT: partials of (r, z_out).
+//   MODE 4: u = z_in, no z_out: r = r - L u in place and u saved to zs (the
+//           start of the second coarse correction of the W level above)
 //   MODE 3: u = z_in, no z_out: the residual r - L u restricted to level
 //           l+1 (sum over the 2x2x2 children); x pairs are summed in the
 //           thread (VX >= 2), plane pairs over the z march (even plane count
@@ -597,7 +667,7 @@ template <typename T, int VX, int MODE, bool DOT, bool L0>
 __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ sds,
                                                      const Rec* __restrict__ recs,
                                                      const int* __restrict__ b2r, int l_arg,
-                                                     int zin, int zout) {
+                                                     int zin, int zout, int wl) {
   const int l = L0 ? 0 : l_arg;
   using V = vec_t<T, VX>;
   static_assert(MODE != 3 || VX >= 2, "the fused restriction pairs cells in the thread");
@@ -608,6 +678,9 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
   const Rec rc = recs[b2r[b]];
   const SysDev& s = sds[rc.sys];
   if (s.sc->done) return;
+  // a launch of the second coarse correction below W level wl: only the
+  // systems with more than wl W levels take part
+  if (wl >= 0 && wl >= s.mg_w) return;
   const MgLevel& Lv = s.lev[l];
   const int nx = Lv.nx, ny = Lv.ny, nz = Lv.nz;
   const int lb = b - rc.blk0;
@@ -629,24 +702,31 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
   const bool xp_in = ix + 1 < lx && x0 + VX < nx;
   const bool ym_in = iy > 0;
   const bool yp_in = iy + 1 < ty && y + 1 < ny;
-  using K = typename CoefOf<T>::type;
-  const K* __restrict__ kx = (const K*)Lv.kx;
-  const K* __restrict__ ky = (const K*)Lv.ky;
-  const K* __restrict__ kz = (const K*)Lv.kz;
+  const MgCoefs<T> co(Lv);
   const T* __restrict__ mm = (const T*)Lv.m;
-  const K* __restrict__ dinv = (const K*)Lv.dinv;
   const T* __restrict__ r = (const T*)Lv.r;
   const T* __restrict__ zi = (const T*)Lv.z[zin];
   T* __restrict__ zo = (T*)Lv.z[zout];
   const T* __restrict__ zc = nullptr;
+  const T* __restrict__ zsc = nullptr;  // saved first correction (W level)
   T* __restrict__ rco = nullptr;
   int cnx = 0, cny = 0;
   if constexpr (MODE == 2 || MODE == 3) {
     const MgLevel& Lc = s.lev[l + 1];
-    if constexpr (MODE == 2) zc = (const T*)Lc.z[l + 1 == s.nlev - 1 ? 0 : 1];
-    else rco = (T*)Lc.r;
+    if constexpr (MODE == 2) {
+      zc = (const T*)Lc.z[l + 1 == s.nlev - 1 ? 0 : 1];
+      if (l < s.mg_w) zsc = (const T*)Lc.zs;
+    } else {
+      rco = (T*)Lc.r;
+    }
     cnx = Lc.nx;
     cny = Lc.ny;
+  }
+  T* __restrict__ rw = nullptr;   // MODE 4: residual written in place
+  T* __restrict__ zsv = nullptr;  // MODE 4: the correction saved
+  if constexpr (MODE == 4) {
+    rw = (T*)Lv.r;
+    zsv = (T*)Lv.zs;
   }
   const T w = (T)kOmega;
   const long long pl = (long long)nx * ny;
@@ -658,12 +738,15 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
   auto uv = [&](int xx, int yy, int zz) -> V {
     const long long i = (long long)zz * pl + (long long)yy * nx + xx;
     if constexpr (MODE == 1) {
-      return w * ldk<T, VX>(dinv, i) * ldv<T, VX>(r, i);
+      return w * co.template comp<VX>(3, i) * ldv<T, VX>(r, i);
     } else if constexpr (MODE == 2) {
       V u = ldv<T, VX>(zi, i);
       const long long cb = ((long long)(zz >> 1) * cny + (yy >> 1)) * cnx;
 #pragma unroll
-      for (int v = 0; v < VX; ++v) u[v] += zc[cb + ((xx + v) >> 1)];
+      for (int v = 0; v < VX; ++v) {
+        const long long ci = cb + ((xx + v) >> 1);
+        u[v] += zsc ? zc[ci] + zsc[ci] : zc[ci];
+      }
       return u;
     } else {
       return ldv<T, VX>(zi, i);
@@ -672,24 +755,24 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
   auto u1 = [&](int xx, int yy, int zz) -> T {
     const long long i = (long long)zz * pl + (long long)yy * nx + xx;
     if constexpr (MODE == 1) {
-      return w * ldk1<T>(dinv, i) * r[i];
+      return w * co.comp1(3, i) * r[i];
     } else if constexpr (MODE == 2) {
-      return zi[i] + zc[((long long)(zz >> 1) * cny + (yy >> 1)) * cnx + (xx >> 1)];
+      const long long ci = ((long long)(zz >> 1) * cny + (yy >> 1)) * cnx + (xx >> 1);
+      return zi[i] + (zsc ? zc[ci] + zsc[ci] : zc[ci]);
     } else {
       return zi[i];
     }
   };
   V um = (T)0, uc = (T)0, kzm = (T)0;
-  // MODE 1: r and D^-1 of the current plane, loaded one plane ahead with u
-  V rcur = (T)0, dcur = (T)0;
+  // MODE 1: r of the current plane, loaded one plane ahead with u
+  V rcur = (T)0;
   if (active) {
     um = uv(x0, y, wrapm(za, nz));
-    kzm = ldk<T, VX>(kz, (long long)wrapm(za, nz) * pl + row + x0);
+    kzm = co.template comp<VX>(2, (long long)wrapm(za, nz) * pl + row + x0);
     if constexpr (MODE == 1) {
       const long long i0 = (long long)za * pl + row + x0;
       rcur = ldv<T, VX>(r, i0);
-      dcur = ldk<T, VX>(dinv, i0);
-      uc = w * dcur * rcur;
+      uc = w * co.template comp<VX>(3, i0) * rcur;
     } else {
       uc = uv(x0, y, za);
     }
@@ -706,12 +789,11 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
     if (active) {
       const long long base = (long long)z * pl;
       const long long ic = base + row + x0;
-      V un, rnext = (T)0, dnext = (T)0;
+      V un, rnext = (T)0;
       if constexpr (MODE == 1) {
         const long long in = (long long)wrapp(z, nz) * pl + row + x0;
         rnext = ldv<T, VX>(r, in);
-        dnext = ldk<T, VX>(dinv, in);
-        un = w * dnext * rnext;
+        un = w * co.template comp<VX>(3, in) * rnext;
       } else {
         un = uv(x0, y, wrapp(z, nz));
       }
@@ -720,7 +802,8 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
         for (int v = 0; v < VX; ++v) su[buf ^ 1][lofs + v] = un[v];
       }
       const T* sp = su[buf];
-      const V kzc = ldk<T, VX>(kz, ic);
+      V kxc, kyc, kzc, dc;
+      co.template own<VX>(ic, kxc, kyc, kzc, dc);
       V lap = kzc * (uc - un) + kzm * (uc - um);
       kzm = kzc;
       {
@@ -731,20 +814,19 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
         } else {
           uy = uv(x0, ym, z);
         }
-        lap += ldk<T, VX>(ky, base + (long long)ym * nx + x0) * (uc - uy);
+        lap += co.template comp<VX>(1, base + (long long)ym * nx + x0) * (uc - uy);
         if (yp_in) {
 #pragma unroll
           for (int v = 0; v < VX; ++v) uy[v] = sp[lofs + tx + v];
         } else {
           uy = uv(x0, yp, z);
         }
-        lap += ldk<T, VX>(ky, ic) * (uc - uy);
+        lap += kyc * (uc - uy);
       }
       {
         const T uxm = xm_in ? sp[lofs - 1] : u1(xm, y, z);
         const T uxp = xp_in ? sp[lofs + VX] : u1(xp, y, z);
-        const V kxc = ldk<T, VX>(kx, ic);
-        const T kxm = ldk1<T>(kx, base + row + xm);
+        const T kxm = co.comp1(0, base + row + xm);
 #pragma unroll
         for (int v = 0; v < VX; ++v) {
           const T up = v + 1 < VX ? uc[(v + 1) % VX] : uxp;
@@ -761,11 +843,11 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
         const V res = rv - lap;
 #pragma unroll
         for (int j = 0; j < NP; ++j) racc[j] += res[2 * j] + res[2 * j + 1];
+      } else if constexpr (MODE == 4) {
+        *reinterpret_cast<V*>(zsv + ic) = uc;
+        *reinterpret_cast<V*>(rw + ic) = rv - lap;
       } else {
-        V dv;
-        if constexpr (MODE == 1) dv = dcur;
-        else dv = ldk<T, VX>(dinv, ic);
-        const V out = uc + w * dv * (rv - lap);
+        const V out = uc + w * dc * (rv - lap);
         *reinterpret_cast<V*>(zo + ic) = out;
         if constexpr (DOT) {
 #pragma unroll
@@ -774,10 +856,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
       }
       um = uc;
       uc = un;
-      if constexpr (MODE == 1) {
-        rcur = rnext;
-        dcur = dnext;
-      }
+      if constexpr (MODE == 1) rcur = rnext;
     }
     if constexpr (MODE == 3) {
       // second plane of a pair (za is even): odd rows hand their sums to
@@ -812,12 +891,13 @@ template <typename T, bool L0>
 __global__ __launch_bounds__(kBlock) void k_mg_restrict(const SysDev* __restrict__ sds,
                                                         const Rec* __restrict__ recs,
                                                         const int* __restrict__ b2r, int l_arg,
-                                                        int zin) {
+                                                        int zin, int wl) {
   const int l = L0 ? 0 : l_arg;
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   const Rec rc = recs[b2r[b]];
   const SysDev& s = sds[rc.sys];
   if (s.sc->done) return;
+  if (wl >= 0 && wl >= s.mg_w) return;
   const MgLevel& Lf = s.lev[l];
   const MgLevel& Lc = s.lev[l + 1];
   const LevelView<T> F(Lf);
@@ -846,10 +926,11 @@ constexpr int kCoarseMax = 8192;
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_mg_coarse(const SysDev* __restrict__ sds,
-                                                      int sweeps) {
+                                                      int sweeps, int wl) {
   extern __shared__ unsigned char smem[];
   const SysDev& s = sds[blockIdx.x];
   if (s.nlev <= 0 || s.sc->done) return;
+  if (wl >= 0 && wl >= s.mg_w) return;
   const MgLevel& Lv = s.lev[s.nlev - 1];
   const LevelView<T> V(Lv);
   const int n = (int)Lv.n;
@@ -894,31 +975,28 @@ __global__ __launch_bounds__(kBlock) void k_fin_precond(const SysDev* __restrict
   }
 }
 
-// Level-0 coefficients of the V-cycle from the CG's conductances (their
-// coefficient type; a copy only when it differs from T).
+// Level-0 coefficients of the V-cycle from the CG's conductances (a copy
+// only where the levels store another type: the packed bfloat16 of FP32
+// engines).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_mg_level0(MgLevel L, const T* __restrict__ kx,
                                                       const T* __restrict__ ky,
                                                       const T* __restrict__ kz) {
-  using K = typename CoefOf<T>::type;
   for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < L.n;
        i += (long long)gridDim.x * kBlock) {
-    stk<T>((K*)L.kx, i, kx[i]);
-    stk<T>((K*)L.ky, i, ky[i]);
-    stk<T>((K*)L.kz, i, kz[i]);
+    st_coef<T>(L, 0, i, kx[i]);
+    st_coef<T>(L, 1, i, ky[i]);
+    st_coef<T>(L, 2, i, kz[i]);
   }
 }
 
 // Coarse operator of level l+1 from level l (one thread per coarse cell):
 // coarse face = scale * (sum of the 4 fine faces it covers), m = scale * sum
-// of the 8 children, in T, stored in the coefficient type.
+// of the 8 children, in T, stored in the levels' coefficient format.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_mg_coarsen(MgLevel f, MgLevel c, T scale) {
-  using K = typename CoefOf<T>::type;
+  const MgCoefs<T> F(f);
   const long long n = c.n;
-  const K* fkx = (const K*)f.kx;
-  const K* fky = (const K*)f.ky;
-  const K* fkz = (const K*)f.kz;
   const T* fm = (const T*)f.m;
   for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
        i += (long long)gridDim.x * kBlock) {
@@ -926,34 +1004,31 @@ __global__ __launch_bounds__(kBlock) void k_mg_coarsen(MgLevel f, MgLevel c, T s
     const long long t = i / c.nx;
     const int y = (int)(t % c.ny);
     const int z = (int)(t / c.ny);
-    auto F = [&](int fx, int fy, int fz) {
+    auto idx = [&](int fx, int fy, int fz) {
       return ((long long)fz * f.ny + fy) * f.nx + fx;
     };
     T sx = 0, sy = 0, sz = 0, sm = 0;
     for (int a = 0; a < 2; ++a)
       for (int bb = 0; bb < 2; ++bb) {
-        sx += ldk1<T>(fkx, F(2 * x + 1, 2 * y + a, 2 * z + bb));
-        sy += ldk1<T>(fky, F(2 * x + a, 2 * y + 1, 2 * z + bb));
-        sz += ldk1<T>(fkz, F(2 * x + a, 2 * y + bb, 2 * z + 1));
+        sx += F.comp1(0, idx(2 * x + 1, 2 * y + a, 2 * z + bb));
+        sy += F.comp1(1, idx(2 * x + a, 2 * y + 1, 2 * z + bb));
+        sz += F.comp1(2, idx(2 * x + a, 2 * y + bb, 2 * z + 1));
       }
     if (fm) {
-      for (int d = 0; d < 8; ++d) sm += fm[F(2 * x + (d & 1), 2 * y + ((d >> 1) & 1), 2 * z + (d >> 2))];
+      for (int d = 0; d < 8; ++d) sm += fm[idx(2 * x + (d & 1), 2 * y + ((d >> 1) & 1), 2 * z + (d >> 2))];
       ((T*)c.m)[i] = scale * sm;
     }
-    stk<T>((K*)c.kx, i, scale * sx);
-    stk<T>((K*)c.ky, i, scale * sy);
-    stk<T>((K*)c.kz, i, scale * sz);
+    st_coef<T>(c, 0, i, scale * sx);
+    st_coef<T>(c, 1, i, scale * sy);
+    st_coef<T>(c, 2, i, scale * sz);
   }
 }
 
 // D^-1 of a level from its stored coefficients (the diagonal of the operator
-// the sweeps apply), stored in the coefficient type.
+// the sweeps apply).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_mg_dinv(MgLevel L) {
-  using K = typename CoefOf<T>::type;
-  const K* kx = (const K*)L.kx;
-  const K* ky = (const K*)L.ky;
-  const K* kz = (const K*)L.kz;
+  const MgCoefs<T> co(L);
   const T* m = (const T*)L.m;
   for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < L.n;
        i += (long long)gridDim.x * kBlock) {
@@ -962,11 +1037,11 @@ __global__ __launch_bounds__(kBlock) void k_mg_dinv(MgLevel L) {
     const int y = (int)(t % L.ny);
     const int z = (int)(t / L.ny);
     const long long pl = (long long)L.nx * L.ny;
-    T d = ldk1<T>(kx, i) + ldk1<T>(kx, (long long)z * pl + (long long)y * L.nx + wrapm(x, L.nx)) +
-          ldk1<T>(ky, i) + ldk1<T>(ky, (long long)z * pl + (long long)wrapm(y, L.ny) * L.nx + x) +
-          ldk1<T>(kz, i) + ldk1<T>(kz, (long long)wrapm(z, L.nz) * pl + (long long)y * L.nx + x);
+    T d = co.comp1(0, i) + co.comp1(0, (long long)z * pl + (long long)y * L.nx + wrapm(x, L.nx)) +
+          co.comp1(1, i) + co.comp1(1, (long long)z * pl + (long long)wrapm(y, L.ny) * L.nx + x) +
+          co.comp1(2, i) + co.comp1(2, (long long)wrapm(z, L.nz) * pl + (long long)y * L.nx + x);
     if (m) d += m[i];
-    stk<T>((K*)L.dinv, i, d != (T)0 ? (T)1 / d : (T)0);
+    st_coef<T>(L, 3, i, d != (T)0 ? (T)1 / d : (T)0);
   }
 }
 
@@ -1475,44 +1550,76 @@ int iter_kernels(int method, int pc) {
 // One V-cycle over every system of the launch (levels of systems with fewer
 // levels drop out of the deeper launches), then (r, z) and beta.
 template <typename T, int MODE, bool DOT, bool L0>
-static void launch_sweep_l(const LaunchSet& L, hipStream_t st, int l, int zin, int zout) {
+static void launch_sweep_l(const LaunchSet& L, hipStream_t st, int l, int zin, int zout,
+                           int wl) {
   switch (L.vx_l[l]) {
     case 4:
       APH_LAUNCH((k_mg_sweep<T, 4, MODE, DOT, L0>), L.nlf[l], st, L.sds, L.rec_lf[l],
-                 L.b2r_lf[l], l, zin, zout);
+                 L.b2r_lf[l], l, zin, zout, wl);
       break;
     case 2:
       APH_LAUNCH((k_mg_sweep<T, 2, MODE, DOT, L0>), L.nlf[l], st, L.sds, L.rec_lf[l],
-                 L.b2r_lf[l], l, zin, zout);
+                 L.b2r_lf[l], l, zin, zout, wl);
       break;
     default:
       if constexpr (MODE == 3) {
         APH_LAUNCH((k_mg_restrict<T, L0>), L.nlc[l], st, L.sds, L.rec_lc[l], L.b2r_lc[l], l,
-                   zin);
+                   zin, wl);
       } else {
         APH_LAUNCH((k_mg_sweep<T, 1, MODE, DOT, L0>), L.nlf[l], st, L.sds, L.rec_lf[l],
-                   L.b2r_lf[l], l, zin, zout);
+                   L.b2r_lf[l], l, zin, zout, wl);
       }
       break;
   }
 }
 
 template <typename T, int MODE, bool DOT>
-static void launch_sweep(const LaunchSet& L, hipStream_t st, int l, int zin, int zout) {
-  if (l == 0) launch_sweep_l<T, MODE, DOT, true>(L, st, l, zin, zout);
-  else launch_sweep_l<T, MODE, DOT, false>(L, st, l, zin, zout);
+static void launch_sweep(const LaunchSet& L, hipStream_t st, int l, int zin, int zout, int wl) {
+  if (l == 0) launch_sweep_l<T, MODE, DOT, true>(L, st, l, zin, zout, wl);
+  else launch_sweep_l<T, MODE, DOT, false>(L, st, l, zin, zout, wl);
 }
 
-// V(2,2) cycle.  Per level above the coarsest: the zero-guess sweep fused with
-// the second pre-smoothing sweep (-> z[1]), restriction of r - L z[1], and
-// after the coarse correction the prolongation fused with the first
+// The cycle from level l down, recursively.  Per level above the coarsest:
+// the zero-guess sweep fused with the second pre-smoothing sweep (-> z[1]),
+// restriction of r - L z[1], the coarse correction (twice on W levels: the
+// first saved to zs by the in-place residual r -= L z, MODE 4, then a second
+// cycle), the prolongation of the correction(s) fused with the first
 // post-smoothing sweep (z[1] -> z[0]) and one more sweep (z[0] -> z[1]; on
 // level 0 it also forms (r, z)).  The correction of every level ends in z[1],
-// of the coarsest in z[0].  `ev` (profiling) records an event before every
-// launch and after the last; `kinds` receives 0 = level-0 pre-smoothing
+// of the coarsest in z[0].  wl: the innermost W level whose second correction
+// this is (-1: none); each system takes part only while wl < its mg_w.
+// `mark(kind)` precedes every launch (profiling): 0 = level-0 pre-smoothing
 // sweep, 1 = level-0 restriction, 2 = coarser-level launch, 3 = coarsest
-// solve, 4 = finalize, 5 and 6 = level-0 first and second post-smoothing
-// sweeps.
+// solve, 5 and 6 = level-0 first and second post-smoothing sweeps.
+template <typename T, typename Mark>
+static void launch_cycle(const LaunchSet& L, hipStream_t st, int l, int wl, Mark& mark) {
+  const int top = L.nlev_max - 1;
+  if (l == top) {
+    mark(3);
+    hipLaunchKernelGGL(k_mg_coarse<T>, dim3(L.ns), dim3(kBlock), L.coarse_lds, st, L.sds,
+                       L.coarse_sweeps, wl);
+    return;
+  }
+  mark(l == 0 ? 0 : 2);
+  launch_sweep<T, 1, false>(L, st, l, 0, 1, wl);
+  mark(l == 0 ? 1 : 2);
+  launch_sweep<T, 3, false>(L, st, l, 1, 1, wl);
+  launch_cycle<T>(L, st, l + 1, wl, mark);
+  if (l < L.mg_w && l + 1 < top) {
+    mark(2);
+    launch_sweep<T, 4, false>(L, st, l + 1, 1, 1, l);
+    launch_cycle<T>(L, st, l + 1, l, mark);
+  }
+  mark(l == 0 ? 5 : 2);
+  launch_sweep<T, 2, false>(L, st, l, 1, 0, wl);
+  mark(l == 0 ? 6 : 2);
+  if (l == 0) launch_sweep<T, 0, true>(L, st, l, 0, 1, wl);
+  else launch_sweep<T, 0, false>(L, st, l, 0, 1, wl);
+}
+
+// The preconditioner z = M r on level 0 and the (r, z) finalize; `ev`
+// (profiling) records an event before every launch and after the last,
+// `kinds` receives the kinds of launch_cycle and 4 = finalize.
 template <typename T>
 static int launch_vcycle(const LaunchSet& L, hipStream_t st, hipEvent_t* ev = nullptr,
                          int* kinds = nullptr) {
@@ -1524,23 +1631,7 @@ static int launch_vcycle(const LaunchSet& L, hipStream_t st, hipEvent_t* ev = nu
     }
     ++k;
   };
-  const int top = L.nlev_max - 1;  // deepest non-coarsest level + 1
-  for (int l = 0; l < top; ++l) {
-    mark(l == 0 ? 0 : 2);
-    launch_sweep<T, 1, false>(L, st, l, 0, 1);
-    mark(l == 0 ? 1 : 2);
-    launch_sweep<T, 3, false>(L, st, l, 1, 1);
-  }
-  mark(3);
-  hipLaunchKernelGGL(k_mg_coarse<T>, dim3(L.ns), dim3(kBlock), L.coarse_lds, st, L.sds,
-                     L.coarse_sweeps);
-  for (int l = top - 1; l >= 0; --l) {
-    mark(l == 0 ? 5 : 2);
-    launch_sweep<T, 2, false>(L, st, l, 1, 0);
-    mark(l == 0 ? 6 : 2);
-    if (l == 0) launch_sweep<T, 0, true>(L, st, l, 0, 1);
-    else launch_sweep<T, 0, false>(L, st, l, 0, 1);
-  }
+  launch_cycle<T>(L, st, 0, -1, mark);
   mark(4);
   APH_LAUNCH(k_fin_precond, L.ns, st, L.sds);
   if (ev) (void)hipEventRecord(ev[k], st);
@@ -1554,7 +1645,15 @@ hipError_t launch_vcycle_profiled(int fp32, const LaunchSet& L, hipStream_t st, 
   return hipGetLastError();
 }
 
-int vcycle_launches(const LaunchSet& L) { return 4 * (L.nlev_max - 1) + 2; }
+static int cycle_launches(const LaunchSet& L, int l) {
+  const int top = L.nlev_max - 1;
+  if (l >= top) return 1;
+  int n = 4 + cycle_launches(L, l + 1);
+  if (l < L.mg_w && l + 1 < top) n += 1 + cycle_launches(L, l + 1);
+  return n;
+}
+
+int vcycle_launches(const LaunchSet& L) { return cycle_launches(L, 0) + 1; }
 
 template <typename T, int VX>
 static void launch_pcg_kernel(const LaunchSet& L, hipStream_t st, int j) {
@@ -1625,7 +1724,7 @@ static int grid_for(long long n);
 template <typename T>
 static hipError_t mg_setup(const MgLevel* lv, int nlev, const void* kx, const void* ky,
                            const void* kz, hipStream_t st) {
-  if (!std::is_same<typename CoefOf<T>::type, T>::value) {
+  if (std::is_same<T, float>::value) {
     APH_LAUNCH(k_mg_level0<T>, grid_for(lv[0].n), st, lv[0], (const T*)kx, (const T*)ky,
                (const T*)kz);
   }

@@ -232,7 +232,7 @@ int aphpois_server_create(const aphpois_server_desc* d, aphpois_server** out) {
   const int rc = s->eng.init(d->device, APHPOIS_PCG, d->precision, APHPOIS_RES_REL_TRUE, d->tol,
                              0, d->maxiter, APHPOIS_NULL_AUTO, d->delta, d->check_every,
                              d->mem_budget, d->max_active > 0 ? d->max_active : 1024,
-                             d->precond);
+                             d->precond, d->mg_w);
   if (rc != APHPOIS_OK) {
     srv_err(rc, s->eng.err());
     delete s;

@@ -9,9 +9,10 @@ from . import _lib as L
 
 class Server:
     def __init__(self, precision=L.MIXED, tol=1e-8, maxiter=200000, delta=0.0, check_every=0,
-                 device=0, mem_budget=0, max_active=0, precond=L.PRECOND_JACOBI):
+                 device=0, mem_budget=0, max_active=0, precond=L.PRECOND_JACOBI, mg_w=0):
         d = L.ServerDesc()
         d.precond = precond
+        d.mg_w = mg_w
         d.precision, d.tol, d.maxiter, d.delta = precision, tol, maxiter, delta
         d.check_every, d.device, d.mem_budget, d.max_active = (check_every, device,
                                                                 int(mem_budget), max_active)

@@ -15,7 +15,8 @@ class Poisson:
 
     def __init__(self, shape, method=L.PCG, precision=L.FP64, resmode=L.RES_REL_TRUE,
                  tol=1e-8, miniter=0, maxiter=100000, h3=None, nullspace=L.NULL_AUTO,
-                 delta=0.0, check_every=0, device=0, mem_budget=0, precond=L.PRECOND_JACOBI):
+                 delta=0.0, check_every=0, device=0, mem_budget=0, precond=L.PRECOND_JACOBI,
+                 mg_w=0):
         nz, ny, nx = shape
         self.shape = (nz, ny, nx)
         d = L.Desc()
@@ -26,6 +27,7 @@ class Poisson:
         d.tol, d.miniter, d.maxiter = tol, miniter, maxiter
         d.nullspace, d.delta, d.check_every = nullspace, delta, check_every
         d.device, d.mem_budget, d.precond = device, mem_budget, precond
+        d.mg_w = mg_w
         self.desc = d
         self.h = C.c_void_p()
         L.check(L.lib().aphpois_create(C.byref(d), C.byref(self.h)))

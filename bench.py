@@ -24,25 +24,30 @@ sys.path.insert(0, ROOT)
 METRIC = "Poisson solves/s @512^3 to 1e-8 (mixed vs FP64 GPU); p50/p99 served latency"
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic HBM words per cell of each iteration kernel (element size of the
-# engine's precision; +1 word each for the mass term of a non-singular system).
-# slot: index in Poisson.kernel_times(); name: the kernel instance in profiles.
+# Algorithmic HBM traffic per cell of each iteration kernel: `vec` values of
+# the engine's element type T (4 bytes MIXED, 8 FP64) and `coef` values of
+# the V-cycle coefficient type (bfloat16, 2 bytes, in MIXED; T in FP64), each
+# array read or written once per launch; +1 T value for the mass term of a
+# non-singular system.  slot: index in Poisson.kernel_times(); name: the
+# kernel instance in profiles.
 KERNELS = {
-    # p = z + beta p fused: read z, p_in, kx, ky, kz; write p_out, q
-    "spmv": {"slot": 0, "words": 7.0, "m": 1, "name": "k_pcg_spmv<T, VX, false>"},
+    # p = z + beta p fused: read z, p_in, kx, ky, kz (the CG's FP32 operator);
+    # write p_out, q
+    "spmv": {"slot": 0, "vec": 7.0, "coef": 0.0, "m": 1, "name": "k_pcg_spmv<T, VX, false>"},
     # read p, xh, r, q; write xh, r
-    "update": {"slot": 2, "words": 6.0, "m": 0, "name": "k_pcg_update<T, VX>"},
-    # u = w D^-1 r fused: read r, dinv, kx, ky, kz; write z
-    "mg_presweep_l0": {"slot": 13, "words": 6.0, "m": 1,
+    "update": {"slot": 2, "vec": 6.0, "coef": 0.0, "m": 0, "name": "k_pcg_update<T, VX>"},
+    # u = w D^-1 r fused: read r, kx, ky, kz, dinv; write z
+    "mg_presweep_l0": {"slot": 13, "vec": 2.0, "coef": 4.0, "m": 1,
                        "name": "k_mg_sweep<T, VX, 1, false, true>"},
-    # read z_in, z_coarse (1/8), kx, ky, kz, dinv, r; write z
-    "mg_postsweep1_l0": {"slot": 14, "words": 7.125, "m": 1,
+    # read z_in, z_coarse (1/8), r, kx, ky, kz, dinv; write z
+    "mg_postsweep1_l0": {"slot": 14, "vec": 3.125, "coef": 4.0, "m": 1,
                          "name": "k_mg_sweep<T, VX, 2, false, true>"},
-    # read z_in, kx, ky, kz, dinv, r; write z; (r, z) partials
-    "mg_postsweep2_l0": {"slot": 15, "words": 7.0, "m": 1,
+    # read z_in, r, kx, ky, kz, dinv; write z; (r, z) partials
+    "mg_postsweep2_l0": {"slot": 15, "vec": 3.0, "coef": 4.0, "m": 1,
                          "name": "k_mg_sweep<T, VX, 0, true, true>"},
     # read r, z, kx, ky, kz; write r_coarse (1/8)
-    "mg_restrict_l0": {"slot": 10, "words": 5.125, "m": 1, "name": "k_mg_restrict<T, true>"},
+    "mg_restrict_l0": {"slot": 10, "vec": 2.125, "coef": 3.0, "m": 1,
+                       "name": "k_mg_sweep<T, VX, 3, false, true>"},
 }
 
 
@@ -50,22 +55,47 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def kernel_table(kt, ncell, elem, has_m, per_iter_launches):
+# the Jacobi-scaled form of the operator: p = S r + beta p, read r, p_in, S,
+# kx, ky, kz; write p_out, q
+SPMV_JACOBI = {"slot": 0, "vec": 8.0, "coef": 0.0, "m": 1, "name": "k_pcg_spmv<T, VX, true>"}
+
+
+def kernel_table(kt, ncell, elem, coef, has_m, per_iter_launches, jacobi=False):
     """Per-kernel average duration and achieved algorithmic bandwidth."""
     out = {}
-    for key, k in KERNELS.items():
+    table = dict(KERNELS)
+    if jacobi:
+        table = {"spmv": SPMV_JACOBI, "update": KERNELS["update"]}
+    for key, k in table.items():
         if k["slot"] >= len(kt):
             continue
         ms, cnt = kt[k["slot"]]
         if cnt == 0 or ms <= 0:
             continue
-        words = k["words"] + (k["m"] if has_m else 0)
-        b = words * elem * ncell
+        per_cell = (k["vec"] + (k["m"] if has_m else 0)) * elem + k["coef"] * coef
+        b = per_cell * ncell
         out[key] = {"kernel": k["name"], "avg_ms": round(ms, 5), "launches": cnt,
-                    "alg_bytes_per_cell": words * elem, "alg_bytes_per_launch": int(b),
+                    "alg_bytes_per_cell": per_cell, "alg_bytes_per_launch": int(b),
                     "GBps": round(b / (ms * 1e-3) / 1e9, 1),
                     "ms_per_iter": round(ms * per_iter_launches.get(key, 1), 5)}
     return out
+
+
+def measured_traffic(config, prec, precond, kernel, li, L):
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE /
+    WRITE_SIZE counter passes (profiles/traffic_<workload>.json, calibrated
+    against a plain copy), when they cover this workload; else None."""
+    if prec != L.MIXED or precond != L.PRECOND_MG:
+        return None
+    path = os.path.join(ROOT, "profiles", f"traffic_{config}_mixed_mg.json")
+    try:
+        with open(path) as f:
+            table = json.load(f)["kernels"]
+    except (OSError, ValueError, KeyError):
+        return None
+    name = kernel.replace("T, VX", f"float, {li['vx']}")
+    rec = table.get(name)
+    return None if rec is None else int(rec["fetch_bytes"] + rec["write_bytes"])
 
 
 def run_batched(args, precond):
@@ -75,13 +105,22 @@ def run_batched(args, precond):
     from aphros_amd import _lib as L
     from aphros_amd import synth
     from aphros_amd.batch import Batch
+    from aphros_amd import autotune
     specs = synth.batch_specs(args.batch_systems)
+    # tile shapes of the CG stencil per grid edge from the autotuner (timed
+    # once per edge, cached)
+    t = time.time()
+    tiles = {n: autotune.tile_for(n, L.MIXED, precond=precond)
+             for n in sorted({sp["n"] for sp in specs})}
+    log(f"[bench] batched: tiles {tiles} ({time.time() - t:.1f}s)")
     b = Batch(precision=L.MIXED, tol=1e-8, precond=precond)
     t = time.time()
     slots = []
     for sp in specs:
         e, _, _ = synth.spec_system(sp)
-        slots.append((b.add(e), sp["n"]))
+        sl = b.add(e)
+        b.set_system_tile(sl, *tiles[sp["n"]])
+        slots.append((sl, sp["n"]))
     log(f"[bench] batched: {len(slots)} systems resident in {time.time() - t:.1f}s, "
         f"device_bytes={b.device_bytes() / 1e9:.2f} GB")
     b.solve()  # first solve: launch tables and graph capture
@@ -97,6 +136,7 @@ def run_batched(args, precond):
     cells = sum(n ** 3 for _, n in slots)
     log(f"[bench] batched: {dt:.3f}s for {len(slots)} solves")
     return {"systems": len(slots), "cells": cells, "seconds": round(dt, 4),
+            "tiles": {str(n): list(tl) for n, tl in tiles.items()},
             "solves_per_s": round(len(slots) / dt, 3),
             "iters_min": min(i["iter"] for i in infos),
             "iters_max": max(i["iter"] for i in infos),
@@ -157,6 +197,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c3_512")
     ap.add_argument("--precond", choices=["mg", "jacobi"], default="mg")
+    ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed",
+                    help="form of the timed solves (the other one is the comparison)")
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--cpu-iters", type=int, default=15,
                     help="iterations of the CPU oracle timed for cpu_baseline (0 = skip)")
@@ -236,7 +278,8 @@ def main():
         s.close()
         return lat, infos, kt, li, plain
 
-    lat, infos, kt, li, plain = run_form(L.MIXED, args.steps, args.warmup,
+    main_prec = L.MIXED if args.precision == "mixed" else L.FP64
+    lat, infos, kt, li, plain = run_form(main_prec, args.steps, args.warmup,
                                          not args.no_profile)
     total = sum(lat)
     value = args.steps / total
@@ -244,7 +287,8 @@ def main():
 
     fp64 = None
     if args.fp64_steps > 0:
-        lat64, inf64, _, _, _ = run_form(L.FP64, args.fp64_steps, 0, False)
+        lat64, inf64, _, _, _ = run_form(L.FP64 if main_prec == L.MIXED else L.MIXED,
+                                         args.fp64_steps, 0, False)
         fp64 = {"solves_per_s": round(args.fp64_steps / sum(lat64), 4),
                 "iters": [i["iter"] for i in inf64],
                 "s_per_solve": round(sum(lat64) / args.fp64_steps, 4)}
@@ -252,16 +296,21 @@ def main():
     roofline, kernels = None, None
     if kt:
         per_iter = {"mg_presweep_l0": 1, "mg_postsweep1_l0": 1, "mg_postsweep2_l0": 1}
-        kernels = kernel_table(kt, ncell, 4, has_m, per_iter)
+        elem = 4 if main_prec == L.MIXED else 8
+        coef = 2 if precond == L.PRECOND_MG and main_prec == L.MIXED else elem
+        kernels = kernel_table(kt, ncell, elem, coef, has_m, per_iter,
+                               jacobi=precond != L.PRECOND_MG)
         if kernels:
             # the dominant kernel: the most device time per iteration
             key = max(kernels, key=lambda k: kernels[k]["ms_per_iter"])
             k = kernels[key]
             achieved = k["GBps"]
+            traffic = measured_traffic(args.config, main_prec, precond, k["kernel"], li, L)
             roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                        "traffic": None, "kernel": k["kernel"] + " [T=float, VX=4]",
+                        "traffic": traffic, "kernel": k["kernel"],
                         "alg_bytes_per_launch": k["alg_bytes_per_launch"],
+                        "element": "float" if main_prec == L.MIXED else "double",
                         "avg_ms": k["avg_ms"], "launches": k["launches"],
                         "plain_copy_GBps": copy_gbps}
         if precond == L.PRECOND_MG and len(kt) > 12:
@@ -278,7 +327,8 @@ def main():
     if args.cpu_iters > 0:
         from oracle import oracle as O
         e4 = expr.reshape(n, n, n, 8)
-        ref_iters = fp64["iters"][0] if fp64 else iters[0]
+        # iterations of the FP64 GPU solve
+        ref_iters = fp64["iters"][0] if (fp64 and main_prec == L.MIXED) else iters[0]
         t = time.perf_counter()
         if precond == L.PRECOND_MG:
             _, it, _, setup = O.mgpcg(e4, tol=0.0, maxiter=-args.cpu_iters, remove_mean=False)
@@ -295,6 +345,8 @@ def main():
                          f"scaled to the {ref_iters} iterations the FP64 GPU solve needs"}
 
     pc_name = "multigrid V(2,2)" if precond == L.PRECOND_MG else "Jacobi"
+    form = ("mixed-precision (FP32 inner, FP64 reliable updates)" if main_prec == L.MIXED
+            else "FP64")
     out = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -306,21 +358,22 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64",
+        "dtype": "f32+f64" if main_prec == L.MIXED else "f64",
         "data": "synthetic",
         "config": {"workload": f"{args.config}: {n}^3 periodic, {cfg['nspheres']} spheres "
                                f"r in [{cfg['rmin']},{cfg['rmax']}], density ratio "
-                               f"{cfg['ratio']:g}, seed {cfg['seed']}; mixed-precision "
-                               f"{pc_name}-preconditioned CG (FP32 inner, FP64 reliable "
-                               f"updates) to a 1e-8 true relative residual, zero initial "
-                               f"guess",
+                               f"{cfg['ratio']:g}, seed {cfg['seed']}; {form} "
+                               f"{pc_name}-preconditioned CG to a 1e-8 true relative "
+                               f"residual, zero initial guess",
                    "cells": ncell, "iters": iters,
                    "p50_s": round(float(np.percentile(lat, 50)), 4),
                    "p99_s": round(float(np.percentile(lat, 99)), 4),
                    "unprofiled_warmup_s": [round(v, 4) for v in plain],
                    "launch": li},
-        "fp64": fp64,
-        "mixed_speedup_vs_fp64": round(value / fp64["solves_per_s"], 3) if fp64 else None,
+        "fp64" if main_prec == L.MIXED else "mixed": fp64,
+        "mixed_speedup_vs_fp64": (None if not fp64 else
+                                  round(value / fp64["solves_per_s"], 3) if main_prec == L.MIXED
+                                  else round(fp64["solves_per_s"] / value, 3)),
         "roofline": roofline,
         "kernels": kernels,
         "batched": batched,

@@ -94,6 +94,8 @@ typedef struct aphpois_desc {
   int device;       /* HIP device ordinal */
   long long mem_budget; /* bytes of device memory this handle may use, 0 = no limit */
   int precond;      /* PCG preconditioner: APHPOIS_PRECOND_* */
+  int mg_w;         /* multigrid: the top mg_w levels apply their coarse
+                       correction twice (0 = V-cycle); see aphpois_desc_default */
 } aphpois_desc;
 
 typedef struct aphpois_info {
@@ -259,6 +261,7 @@ typedef struct aphpois_batch_desc {
   long long mem_budget; /* bytes, 0 = no limit */
   int max_systems;      /* slot capacity, 0 = 1024 */
   int precond;          /* APHPOIS_PRECOND_* */
+  int mg_w;             /* as in aphpois_desc */
 } aphpois_batch_desc;
 
 int aphpois_batch_create(const aphpois_batch_desc* d, aphpois_batch** out);
@@ -305,6 +308,7 @@ typedef struct aphpois_server_desc {
   long long mem_budget; /* admission budget in bytes of device memory, 0 = none */
   int max_active;       /* resident systems at most, 0 = 1024 */
   int precond;          /* APHPOIS_PRECOND_* */
+  int mg_w;             /* as in aphpois_desc */
 } aphpois_server_desc;
 
 typedef struct aphpois_request {

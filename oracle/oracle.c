@@ -499,7 +499,7 @@ int orc_pcg_jacobi(const double* expr, int nx, int ny, int nz, const double* x0,
  * conductance is 0.5 * (sum of the 4 fine faces it covers), the coarse mass
  * 0.5 * (sum over the 8 children); D = diagonal of the level operator.
  *
- * V-cycle z = M r on level l (w = 0.8):
+ * V-cycle z = M r on level l (w = 0.9):
  *   l < top: u = w D^-1 r; z = u + w D^-1 (r - L u);
  *            r_{l+1} = sum over the 2x2x2 children of (r - L z);
  *            z_{l+1} = M r_{l+1};
@@ -508,6 +508,11 @@ int orc_pcg_jacobi(const double* expr, int nx, int ny, int nz, const double* x0,
  *   top:     u = w D^-1 r, then `sweeps - 1` more damped Jacobi sweeps
  * Restriction is the transpose of the injection and the pre and post sweeps
  * mirror each other, so M is symmetric.
+ * W levels: on the top w levels (l < w, and only where level l+1 is not the
+ * coarsest: w <= nlev - 2) the coarse correction is applied twice: with
+ * zs = z_{l+1}, r_{l+1} -= L_{l+1} zs, z_{l+1} = M r_{l+1}, and the
+ * prolongation adds (z_{l+1} + zs).  Two symmetric corrections in sequence
+ * keep M symmetric.
  *
  * CG: textbook PCG with M, stopped on the recursive residual
  * ||r|| <= target ||b|| and confirmed on the true residual (restart with a
@@ -522,7 +527,7 @@ typedef struct {
   double *kx, *ky, *kz, *m, *dinv, *r, *z, *t;
 } mglev_t;
 
-static const double kMgOmega = 0.8;
+static const double kMgOmega = 0.9;
 
 static double wall_seconds(void) {
 #ifdef _OPENMP
@@ -557,7 +562,7 @@ static void mg_sweep(const mglev_t* L, const double* u, double* out) {
       }
 }
 
-static void mg_vcycle(mglev_t* lev, int l, int top, int sweeps) {
+static void mg_vcycle(mglev_t* lev, int l, int top, int sweeps, int w) {
   mglev_t* L = &lev[l];
   const long long n = (long long)L->n;
 #pragma omp parallel for schedule(static)
@@ -588,7 +593,25 @@ static void mg_vcycle(mglev_t* lev, int l, int top, int sweeps) {
         }
         C->r[(size_t)x + (size_t)C->nx * ((size_t)y + (size_t)C->ny * z)] = sum;
       }
-  mg_vcycle(lev, l + 1, top, sweeps);
+  mg_vcycle(lev, l + 1, top, sweeps, w);
+  const int second = l < w && l + 1 < top;
+  double* zs = NULL;
+  if (second) {
+    const size_t nc = C->n;
+    zs = (double*)malloc(nc * sizeof(double));
+    memcpy(zs, C->z, nc * sizeof(double));
+    double* rn = (double*)malloc(nc * sizeof(double));
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < C->nz; ++z)
+      for (int y = 0; y < C->ny; ++y)
+        for (int x = 0; x < C->nx; ++x) {
+          const size_t c = (size_t)x + (size_t)C->nx * ((size_t)y + (size_t)C->ny * z);
+          rn[c] = C->r[c] - mg_lap(C, zs, x, y, z);
+        }
+    memcpy(C->r, rn, nc * sizeof(double));
+    free(rn);
+    mg_vcycle(lev, l + 1, top, sweeps, w);
+  }
 #pragma omp parallel for schedule(static)
   for (int z = 0; z < L->nz; ++z)
     for (int y = 0; y < L->ny; ++y)
@@ -596,8 +619,9 @@ static void mg_vcycle(mglev_t* lev, int l, int top, int sweeps) {
         const size_t c = (size_t)x + (size_t)L->nx * ((size_t)y + (size_t)L->ny * z);
         const size_t cc =
             (size_t)(x >> 1) + (size_t)C->nx * ((size_t)(y >> 1) + (size_t)C->ny * (z >> 1));
-        L->t[c] = L->z[c] + C->z[cc];
+        L->t[c] = L->z[c] + (zs ? C->z[cc] + zs[cc] : C->z[cc]);
       }
+  free(zs);
   mg_sweep(L, L->t, L->z);
   mg_sweep(L, L->z, L->t);
   double* tmp = L->t;
@@ -740,20 +764,20 @@ static int mg_setup(const double* expr, int nx, int ny, int nz, int coef16, mgle
 
 /* z = M r for one right-hand side (tests: symmetry of M, action on modes);
  * coef16 as in mg_setup; returns nlev or -1/-2. */
-int orc_mg_apply(const double* expr, int nx, int ny, int nz, int sweeps, int coef16,
+int orc_mg_apply(const double* expr, int nx, int ny, int nz, int sweeps, int coef16, int w,
                  const double* r, double* z) {
   mglev_t lev[12];
   const int nlev = mg_setup(expr, nx, ny, nz, coef16, lev);
   if (nlev < 0) return nlev;
   memcpy(lev[0].r, r, lev[0].n * sizeof(double));
-  mg_vcycle(lev, 0, nlev - 1, sweeps);
+  mg_vcycle(lev, 0, nlev - 1, sweeps, w);
   memcpy(z, lev[0].z, lev[0].n * sizeof(double));
   mg_free(lev, nlev);
   return nlev;
 }
 
 int orc_mgpcg(const double* expr, int nx, int ny, int nz, const double* x0, double* x,
-              double tol, int maxiter, int sweeps, int remove_mean, double* relres_out,
+              double tol, int maxiter, int sweeps, int w, int remove_mean, double* relres_out,
               double* setup_s) {
   const double t0 = wall_seconds();
   mglev_t lev[12];
@@ -803,7 +827,7 @@ int orc_mgpcg(const double* expr, int nx, int ny, int nz, const double* x0, doub
           r[c] = -sgn * u;
         }
     memcpy(L0->r, r, n * sizeof(double));
-    mg_vcycle(lev, 0, nlev - 1, sweeps);
+    mg_vcycle(lev, 0, nlev - 1, sweeps, w);
     double rz = 0, rr = 0;
 #pragma omp parallel for reduction(+ : rz, rr) schedule(static)
     for (long long c = 0; c < n; ++c) {
@@ -833,7 +857,7 @@ int orc_mgpcg(const double* expr, int nx, int ny, int nz, const double* x0, doub
         L0->r[c] = r[c];
         rr += r[c] * r[c];
       }
-      mg_vcycle(lev, 0, nlev - 1, sweeps);
+      mg_vcycle(lev, 0, nlev - 1, sweeps, w);
       double rz_new = 0;
 #pragma omp parallel for reduction(+ : rz_new) schedule(static)
       for (long long c = 0; c < n; ++c) rz_new += r[c] * L0->z[c];

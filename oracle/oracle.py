@@ -54,10 +54,11 @@ def lib():
         L.orc_flux_correct.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, _dp,
                                        _dp, _dp, _dp, _dp]
         L.orc_flux_correct.restype = None
-        L.orc_mg_apply.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp]
+        L.orc_mg_apply.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                   _dp, _dp]
         L.orc_mg_apply.restype = C.c_int
         L.orc_mgpcg.argtypes = [_dp, C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_int,
-                                C.c_int, C.c_int, _dp, _dp]
+                                C.c_int, C.c_int, C.c_int, _dp, _dp]
         L.orc_mgpcg.restype = C.c_int
         L.orc_num_threads.argtypes = []
         L.orc_num_threads.restype = C.c_int
@@ -184,14 +185,15 @@ def pcg_jacobi(expr, tol=1e-8, maxiter=100000, remove_mean=True, x0=None):
 MG_SWEEPS = 40  # damped Jacobi sweeps on the coarsest level
 
 
-def mg_apply(expr, r, sweeps=MG_SWEEPS, bf16=False):
-    """One V-cycle z = M r of the multigrid preconditioner; returns (z, nlev).
-    bf16: coefficients stored as the GPU's FP32 engines store them."""
+def mg_apply(expr, r, sweeps=MG_SWEEPS, bf16=False, w=0):
+    """One cycle z = M r of the multigrid preconditioner; returns (z, nlev).
+    bf16: coefficients stored as the GPU's FP32 engines store them; w: levels
+    with two coarse corrections."""
     expr = np.ascontiguousarray(expr, np.float64)
     nx, ny, nz = _dims(expr.shape[:3])
     r = np.ascontiguousarray(r, np.float64)
     z = np.empty(expr.shape[:3])
-    nlev = lib().orc_mg_apply(_p(expr), nx, ny, nz, sweeps, int(bf16), _p(r), _p(z))
+    nlev = lib().orc_mg_apply(_p(expr), nx, ny, nz, sweeps, int(bf16), w, _p(r), _p(z))
     if nlev == -2:
         raise ValueError("grid cannot be coarsened for the multigrid preconditioner")
     if nlev < 0:
@@ -199,7 +201,7 @@ def mg_apply(expr, r, sweeps=MG_SWEEPS, bf16=False):
     return z, nlev
 
 
-def mgpcg(expr, tol=1e-8, maxiter=100000, remove_mean=True, x0=None, sweeps=MG_SWEEPS):
+def mgpcg(expr, tol=1e-8, maxiter=100000, remove_mean=True, x0=None, sweeps=MG_SWEEPS, w=0):
     """FP64 multigrid-preconditioned CG to a true relative residual; returns
     (x, iter, relres, setup_seconds).  maxiter < 0 runs exactly -maxiter
     iterations (timing sample)."""
@@ -209,7 +211,7 @@ def mgpcg(expr, tol=1e-8, maxiter=100000, remove_mean=True, x0=None, sweeps=MG_S
     x0 = None if x0 is None else np.ascontiguousarray(x0, np.float64)
     rr = C.c_double()
     setup = C.c_double()
-    it = lib().orc_mgpcg(_p(expr), nx, ny, nz, _p(x0), _p(x), tol, maxiter, sweeps,
+    it = lib().orc_mgpcg(_p(expr), nx, ny, nz, _p(x0), _p(x), tol, maxiter, sweeps, w,
                          int(remove_mean), C.byref(rr), C.byref(setup))
     if it == -2:
         raise ValueError("grid cannot be coarsened for the multigrid preconditioner")

@@ -26,9 +26,9 @@ def _sys(n, ns, rmin, rmax, ratio, seed):
     return e.reshape(n, n, n, 8)
 
 
-def _single(E, precision=L.MIXED, x0=None, tol=1e-8, precond=L.PRECOND_JACOBI):
+def _single(E, precision=L.MIXED, x0=None, tol=1e-8, precond=L.PRECOND_JACOBI, mg_w=0):
     s = Poisson(E.shape[:3], method=L.PCG, precision=precision, resmode=L.RES_REL_TRUE,
-                tol=tol, maxiter=50000, precond=precond)
+                tol=tol, maxiter=50000, precond=precond, mg_w=mg_w)
     s.set_system(E)
     x, info = s.solve(x0)
     s.close()
@@ -36,15 +36,18 @@ def _single(E, precision=L.MIXED, x0=None, tol=1e-8, precond=L.PRECOND_JACOBI):
 
 
 @pytest.mark.parametrize("precision", [L.MIXED, L.FP64])
-@pytest.mark.parametrize("precond", [L.PRECOND_JACOBI, L.PRECOND_MG])
-def test_batch_equals_single(precision, precond):
+@pytest.mark.parametrize("precond,w", [(L.PRECOND_JACOBI, 0), (L.PRECOND_MG, 0),
+                                       (L.PRECOND_MG, 3)])
+def test_batch_equals_single(precision, precond, w):
+    """Systems of different depths (32^3 and 48^3: 5 levels, 64^3: 6): with
+    W levels each system keeps its own number (mg_w clamped to its depth)."""
     systems = [_sys(*p) for p in SYSTEMS]
-    b = Batch(precision=precision, tol=1e-8, maxiter=50000, precond=precond)
+    b = Batch(precision=precision, tol=1e-8, maxiter=50000, precond=precond, mg_w=w)
     slots = [b.add(E) for E in systems]
     b.solve()
     for E, sl in zip(systems, slots):
         xb, ib = b.get(sl)
-        xs, is_ = _single(E, precision, precond=precond)
+        xs, is_ = _single(E, precision, precond=precond, mg_w=w)
         print(f"n={E.shape[0]} iter={ib['iter']} relres={ib['true_relres']:.2e}", flush=True)
         assert ib["converged"] == 1 and ib["iter"] == is_["iter"]
         assert np.array_equal(xb, xs)

@@ -118,42 +118,46 @@ def _with_mass(e, frac=0.05, seed=4):
 
 
 @pytest.mark.parametrize("precision,tol", [(L.FP64, 1e-12), (L.MIXED, 1e-4)])
-@pytest.mark.parametrize("case", ["c1", "tlin_negative", "c1_mass"])
-def test_mg_vcycle_matches_oracle(c1, tlin32, precision, tol, case):
-    """One V-cycle on the GPU against the oracle's V-cycle (FP64: rounding
-    only; MIXED: FP32 arithmetic), on the same residual."""
+@pytest.mark.parametrize("case,w", [("c1", 0), ("tlin_negative", 0), ("c1_mass", 0), ("c1", 2),
+                                    ("c1_mass", 9)])
+def test_mg_vcycle_matches_oracle(c1, tlin32, precision, tol, case, w):
+    """One cycle on the GPU against the oracle's (FP64: rounding only; MIXED:
+    FP32 arithmetic and bfloat16 coefficients restated by the oracle), on the
+    same residual; w levels with two coarse corrections (9: every level that
+    can, i.e. 4 of the 6 levels of 64^3)."""
     e = {"c1": c1, "tlin_negative": tlin32[0], "c1_mass": _with_mass(c1)}[case]
-    _say(f"vcycle {case} precision={precision}")
+    _say(f"vcycle {case} precision={precision} w={w}")
     s = Poisson(e.shape[:3], method=L.PCG, precision=precision, resmode=L.RES_REL_TRUE,
-                tol=1e-8, maxiter=20000, precond=L.PRECOND_MG)
+                tol=1e-8, maxiter=20000, precond=L.PRECOND_MG, mg_w=w)
     s.set_system(e)
     rng = np.random.default_rng(12)
     r = rng.standard_normal(e.shape[:3])
     r -= r.mean()
     z = s.precond_apply(r)
     # MIXED stores the V-cycle coefficients in bfloat16
-    zo, nlev = O.mg_apply(e, r, bf16=precision == L.MIXED)
+    zo, nlev = O.mg_apply(e, r, bf16=precision == L.MIXED, w=w)
     err = np.abs(z - zo).max() / np.abs(zo).max()
     _say(f"  nlev={nlev} rel_err={err:.3e}")
     assert err <= tol
     # the solve state is untouched: a solve after it matches a fresh handle
     x, info = s.solve(None)
     s2 = Poisson(e.shape[:3], method=L.PCG, precision=precision, resmode=L.RES_REL_TRUE,
-                 tol=1e-8, maxiter=20000, precond=L.PRECOND_MG)
+                 tol=1e-8, maxiter=20000, precond=L.PRECOND_MG, mg_w=w)
     s2.set_system(e)
     x2, info2 = s2.solve(None)
     assert info["iter"] == info2["iter"] and np.array_equal(x, x2)
 
 
-def test_mg_fp64_iterations_match_oracle(c1):
+@pytest.mark.parametrize("w", [0, 2])
+def test_mg_fp64_iterations_match_oracle(c1, w):
     """FP64 multigrid PCG on the GPU and the oracle's FP64 multigrid PCG (same
     preconditioner, sums in a different order): equal iteration counts up to
     the stop test's granularity."""
     s = Poisson(c1.shape[:3], method=L.PCG, precision=L.FP64, resmode=L.RES_REL_TRUE,
-                tol=1e-8, maxiter=20000, precond=L.PRECOND_MG)
+                tol=1e-8, maxiter=20000, precond=L.PRECOND_MG, mg_w=w)
     s.set_system(c1)
     x, info = s.solve(None)
-    xo, ito, rro, _ = O.mgpcg(c1, tol=1e-8)
+    xo, ito, rro, _ = O.mgpcg(c1, tol=1e-8, w=w)
     _say(f"  gpu iter={info['iter']} oracle iter={ito}")
     assert abs(info["iter"] - ito) <= 2
     err = np.abs(x - xo).max() / np.abs(xo).max()

@@ -18,7 +18,7 @@ import pytest
 
 from oracle import oracle as O
 
-OMEGA = 0.8
+OMEGA = 0.9
 
 
 def _system(n, seed, ratio=1000.0, mass=0.0, sign=1, shape=None):
@@ -57,9 +57,10 @@ def _injection(shape, shift=0):
     return P, (cz, cy, cx)
 
 
-def _dense_vcycle(e, sweeps, scale=0.5, shift=0):
-    """M as a dense matrix, from the V-cycle's definition in matrix form
-    (scale, shift: planted mistakes for the check's own test)."""
+def _dense_vcycle(e, sweeps, scale=0.5, shift=0, w=0):
+    """M as a dense matrix, from the cycle's definition in matrix form
+    (scale, shift: planted mistakes for the check's own test; w: levels with
+    two coarse corrections)."""
     A = [_dense(e)]
     shapes = [e.shape[:3]]
     Ps = []
@@ -86,7 +87,12 @@ def _dense_vcycle(e, sweeps, scale=0.5, shift=0):
                 u = smooth(l, u, r)
             return u
         z = smooth(l, u, r)
-        zc = vc(l + 1, Ps[l].T @ (r - A[l] @ z))
+        rc = Ps[l].T @ (r - A[l] @ z)
+        zc = vc(l + 1, rc)
+        if l < w and l + 1 < len(A) - 1:
+            zs = zc
+            zc = vc(l + 1, rc - A[l + 1] @ zs)
+            zc = zc + zs
         z = smooth(l, z + Ps[l] @ zc, r)
         return smooth(l, z, r)
 
@@ -106,6 +112,34 @@ def test_vcycle_matches_dense_definition(shape, mass, sign):
         assert nl == nlev
         ze = (M @ r.ravel()).reshape(shape)
         assert np.max(np.abs(z - ze)) <= 1e-11 * np.max(np.abs(ze))
+
+
+@pytest.mark.parametrize("w", [1, 2])
+def test_w_levels_match_dense_definition(w):
+    """Two coarse corrections on the top w levels (8^3: 3 levels, so w = 2
+    acts like w = 1: level 1's coarse level is the coarsest)."""
+    e = _system(None, 7, shape=(8, 8, 8), mass=0.05)
+    M, nlev = _dense_vcycle(e, 40, w=w)
+    r = np.random.default_rng(3).standard_normal((8, 8, 8))
+    z, nl = O.mg_apply(e, r, w=w)
+    ze = (M @ r.ravel()).reshape(8, 8, 8)
+    assert nl == nlev == 3
+    assert np.max(np.abs(z - ze)) <= 1e-11 * np.max(np.abs(ze))
+    z0, _ = O.mg_apply(e, r, w=0)
+    assert np.max(np.abs(z - z0)) > 1e-4 * np.max(np.abs(ze))  # it does something
+
+
+def test_w_levels_symmetric_and_fewer_iterations():
+    e = _system(32, 11)
+    rng = np.random.default_rng(5)
+    r1, r2 = rng.standard_normal((2, 32, 32, 32))
+    z1, _ = O.mg_apply(e, r1, w=2)
+    z2, _ = O.mg_apply(e, r2, w=2)
+    a, b = np.vdot(z1, r2), np.vdot(r1, z2)
+    assert abs(a - b) <= 1e-12 * np.sqrt(np.vdot(z1, r1) * np.vdot(z2, r2))
+    _, it0, rel0, _ = O.mgpcg(e, tol=1e-10, w=0)
+    _, it2, rel2, _ = O.mgpcg(e, tol=1e-10, w=2)
+    assert rel0 <= 1e-10 and rel2 <= 1e-10 and it2 < it0
 
 
 def test_vcycle_symmetric_positive():
