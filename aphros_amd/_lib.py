@@ -89,6 +89,8 @@ _vp = C.c_void_p
 _ll = C.c_longlong
 
 # name -> (restype, argtypes); the exported surface of include/aphpois.h
+MG_W_DEFAULT = 3  # APHPOIS_MG_W_DEFAULT
+
 SIGNATURES = {
     "aphpois_desc_default": (None, [C.POINTER(Desc)]),
     "aphpois_create": (C.c_int, [C.POINTER(Desc), C.POINTER(_vp)]),

@@ -56,6 +56,7 @@ void aphpois_desc_default(aphpois_desc* d) {
   d->check_every = 0;
   d->device = 0;
   d->mem_budget = 0;
+  d->mg_w = APHPOIS_MG_W_DEFAULT;
 }
 
 int aphpois_create(const aphpois_desc* desc, aphpois** out) {

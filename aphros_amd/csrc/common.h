@@ -95,6 +95,7 @@ struct SysDev {
   // (= M^-1 r) instead of r, and S is 1 (S == nullptr)
   int nlev;        // multigrid levels (0 = Jacobi-scaled CG without z)
   int mg_w;        // levels l < mg_w apply their coarse correction twice
+  int mg_lb;       // first level of the bottom kernel (levels mg_lb..nlev-1)
   int nblk_pc;     // workgroups of the level-0 dot in the last smoothing sweep
   const void* z;   // vector the p update reads (r in Jacobi mode)
   const MgLevel* lev;  // device array of nlev levels

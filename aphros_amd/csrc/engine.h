@@ -141,7 +141,7 @@ class Engine {
     }
   }
   // iteration kernels, plus (multigrid) [9] level-0 sweep, [10] level-0
-  // restriction, [11] coarser levels per V-cycle, [12] coarsest solve,
+  // restriction, [11] coarser levels per cycle, [12] bottom levels,
   // [13] [14] [15] level-0 pre-sweep, first and second post-sweep
   int prof_kernels() const { return pc_ ? 16 : iter_kernels(method_, 0); }
   double prof_avg_ms(int j) const { return prof_n_[j] ? prof_ms_[j] / prof_n_[j] : 0.0; }

@@ -314,6 +314,18 @@ int Engine::add_system(int nx, int ny, int nz, double h3, int* slot) {
       // first of two coarse corrections in zs of level l + 1
       const int w = std::max(0, std::min(mg_w_, (int)S.lev.size() - 2));
       S.dev.mg_w = w;
+      // the bottom kernel takes the levels from the first one below every W
+      // level with at most 4096 (FP32) / 2048 (FP64) cells, or the coarsest
+      // alone: at most 64 KiB of LDS for z and a second buffer per level
+      const long long bcap = fp32_ ? 4096 : 2048;
+      int lb = (int)S.lev.size() - 1;
+      for (int l = std::max(1, w + 1); l + 1 < (int)S.lev.size(); ++l) {
+        if (S.lev[l].n <= bcap) {
+          lb = l;
+          break;
+        }
+      }
+      S.dev.mg_lb = lb;
       for (int l = 1; l <= w && rc == APHPOIS_OK; ++l) {
         void* zs = nullptr;
         A(&zs, S.lev[l].n * es);
@@ -684,7 +696,9 @@ int Engine::build_tables() {
   std::vector<std::vector<Rec>> rlf(kMaxLevels), rlc(kMaxLevels);
   std::vector<std::vector<int>> blf(kMaxLevels), blc(kMaxLevels);
   int nlev_max = 0;
-  long long coarse_max = 0;
+  long long coarse_max = 0;  // LDS values of the bottom kernel
+  int bottom_at[kMaxLevels] = {};
+  int deep = 0;
   // x vector width of each level's sweeps: the widest that divides the level's
   // nx of every system that smooths on that level
   int vxl[kMaxLevels];
@@ -734,15 +748,20 @@ int Engine::build_tables() {
     if (pc_) {
       const int nl = (int)s.lev.size();
       nlev_max = std::max(nlev_max, nl);
-      coarse_max = std::max(coarse_max, s.lev.back().n);
-      for (int l = 0; l + 1 < nl; ++l) {
+      long long bottom = 0;
+      for (int l = s.dev.mg_lb; l < nl; ++l) bottom += 2 * s.lev[l].n;
+      coarse_max = std::max(coarse_max, bottom);
+      bottom_at[s.dev.mg_lb] = 1;
+      deep = std::max(deep, s.dev.mg_lb);
+      for (int l = 0; l < s.dev.mg_lb; ++l) {
         const MgLevel& F = s.lev[l];
         const MgLevel& C = s.lev[l + 1];
         // large levels: long z marches as above; small levels (latency
         // bound): short marches over more workgroups
         const long long ftgt = F.n >= (1LL << 21) ? std::min(1024LL, cdiv(F.n, 2048))
                                                   : std::max(1LL, cdiv(F.n, 512));
-        Rec f = make_stencil_rec(F.nx, F.ny, F.nz, vxl[l], ftgt, none);
+        // level 0 follows the system's stencil tuning (autotuner / set_tile)
+        Rec f = make_stencil_rec(F.nx, F.ny, F.nz, vxl[l], ftgt, l == 0 ? tu : none);
         // the fused restriction sums plane pairs inside one workgroup: even
         // plane counts per tile (F.nz is even on every coarsened level)
         if (f.bz & 1) {
@@ -842,7 +861,9 @@ int Engine::build_tables() {
   for (int i : launch_slots_) L_.mg_w = std::max(L_.mg_w, slots_[i]->dev.mg_w);
   for (int l = 0; l < kMaxLevels; ++l) L_.vx_l[l] = vxl[l];
   L_.coarse_sweeps = 40;
-  L_.coarse_lds = (int)(2 * coarse_max * (fp32_ ? 4 : 8));
+  L_.coarse_lds = (int)(coarse_max * (fp32_ ? 4 : 8));
+  for (int l = 0; l < kMaxLevels; ++l) L_.bottom_at[l] = bottom_at[l];
+  L_.deep = deep;
   for (int l = 0; l < kMaxLevels; ++l) {
     L_.rec_lf[l] = d_rec_ + rec_lf_ofs[l];
     L_.rec_lc[l] = d_rec_ + rec_lc_ofs[l];
@@ -864,17 +885,21 @@ int Engine::capture() {
     APH_HIP(hipGraphDestroy(graph_));
     graph_ = nullptr;
   }
+  const int nk = iter_kernels(method_, pc_ ? 1 : 0);
+  // launches per iteration (the multigrid cycle counts as one kernel in nk)
+  const int nl = nk + (pc_ ? vcycle_launches(L_) - 1 : 0);
   int k = check_every_;
   if (k <= 0) {
+    // about 4 ms of work per replay (iterations past convergence cost a
+    // launch each), at most ~2000 graph nodes
     long long ntot = 0;
     for (int i : launch_slots_) ntot += slots_[i]->n;
-    const double bpc = method_ == kPcg ? (fp32_ ? 60.0 : 120.0) : 150.0;
-    const double t_iter = (double)ntot * bpc / 4e12 + 20e-6;
+    const double bpc = method_ == kPcg ? (fp32_ ? 60.0 : 120.0) * (pc_ ? 2.5 : 1.0) : 150.0;
+    const double t_iter = (double)ntot * bpc / 4e12 + nl * 6e-6;
     k = (int)std::lround(4e-3 / t_iter);
-    k = std::max(2, std::min(64, k));
+    k = std::max(2, std::min({64, k, std::max(2, 2000 / nl)}));
   }
   if (method_ == kJacobi && (k & 1)) ++k;
-  const int nk = iter_kernels(method_, pc_ ? 1 : 0);
   if (profile_) {
     const size_t ne = (size_t)k * nk + 1;
     while (pev_.size() < ne) {
@@ -1086,10 +1111,10 @@ int Engine::run() {
           prof_n_[j] += 1;
         }
         if (pc_) {
-          // V-cycle launches by kind: [9] level-0 sweep (per launch, all
+          // cycle launches by kind: [9] level-0 sweep (per launch, all
           // three), [10] level-0 restriction, [11] coarser levels (per
-          // V-cycle), [12] coarsest solve, [13] [14] [15] level-0 pre-sweep,
-          // first and second post-sweep
+          // cycle), [12] bottom-levels kernel, [13] [14] [15] level-0
+          // pre-sweep, first and second post-sweep
           const int nv = vcycle_launches(L_) + 1;
           double coarse = 0;
           for (int i = 0; i + 1 < nv; ++i) {

@@ -22,8 +22,11 @@ struct LaunchSet {
   int pc = 0;
   int nlev_max = 0;
   int mg_w = 0;  // largest number of W levels over the systems
+  int deep = 0;  // largest bottom level over the systems (no launches below)
+  int bottom_at[kMaxLevels] = {};  // some system's bottom kernel starts here
+  int start_only = 0;  // cycle launches for newly prepared systems only
   int coarse_sweeps = 40;
-  int coarse_lds = 0;  // dynamic LDS bytes of the coarsest-level kernel
+  int coarse_lds = 0;  // dynamic LDS bytes of the bottom-levels kernel
   int vx_l[kMaxLevels] = {};  // cells per thread in x of the level-l sweeps
   const Rec* rec_lf[kMaxLevels] = {};
   const int* b2r_lf[kMaxLevels] = {};

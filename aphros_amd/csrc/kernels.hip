@@ -655,6 +655,10 @@ __device__ __forceinline__ Cell cell_of(const Rec& rc, int lb, int nx, int ny, i
 //   MODE 1: u = w D^-1 r (the zero-guess first sweep, fused)
 //   MODE 2: u = z_in + P z_c (coarse correction, fused)
 // and z_out = u + w D^-1 (r - L u); DOT: partials of (r, z_out).
+//   addzs (MODE 0, the last sweep of a second coarse correction): z_out also
+//           adds zs, the first correction, so that the prolongation above
+//           reads one coarse array
+
 //   MODE 4: u = z_in, no z_out: r = r - L u in place and u saved to zs (the
 //           start of the second coarse correction of the W level above)
 //   MODE 3: u = z_in, no z_out: the residual r - L u restricted to level
@@ -667,7 +671,8 @@ template <typename T, int VX, int MODE, bool DOT, bool L0>
 __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ sds,
                                                      const Rec* __restrict__ recs,
                                                      const int* __restrict__ b2r, int l_arg,
-                                                     int zin, int zout, int wl) {
+                                                     int zin, int zout, int wl, int so,
+                                                     int addzs) {
   const int l = L0 ? 0 : l_arg;
   using V = vec_t<T, VX>;
   static_assert(MODE != 3 || VX >= 2, "the fused restriction pairs cells in the thread");
@@ -681,6 +686,8 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
   // a launch of the second coarse correction below W level wl: only the
   // systems with more than wl W levels take part
   if (wl >= 0 && wl >= s.mg_w) return;
+  // so: the start of newly prepared systems (first == 1); running ones skip
+  if (so && !s.sc->first) return;
   const MgLevel& Lv = s.lev[l];
   const int nx = Lv.nx, ny = Lv.ny, nz = Lv.nz;
   const int lb = b - rc.blk0;
@@ -708,14 +715,12 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
   const T* __restrict__ zi = (const T*)Lv.z[zin];
   T* __restrict__ zo = (T*)Lv.z[zout];
   const T* __restrict__ zc = nullptr;
-  const T* __restrict__ zsc = nullptr;  // saved first correction (W level)
   T* __restrict__ rco = nullptr;
   int cnx = 0, cny = 0;
   if constexpr (MODE == 2 || MODE == 3) {
     const MgLevel& Lc = s.lev[l + 1];
     if constexpr (MODE == 2) {
       zc = (const T*)Lc.z[l + 1 == s.nlev - 1 ? 0 : 1];
-      if (l < s.mg_w) zsc = (const T*)Lc.zs;
     } else {
       rco = (T*)Lc.r;
     }
@@ -724,9 +729,13 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
   }
   T* __restrict__ rw = nullptr;   // MODE 4: residual written in place
   T* __restrict__ zsv = nullptr;  // MODE 4: the correction saved
+  const T* __restrict__ zsa = nullptr;  // MODE 0: the first correction added
   if constexpr (MODE == 4) {
     rw = (T*)Lv.r;
     zsv = (T*)Lv.zs;
+  }
+  if constexpr (MODE == 0) {
+    if (addzs) zsa = (const T*)Lv.zs;
   }
   const T w = (T)kOmega;
   const long long pl = (long long)nx * ny;
@@ -741,11 +750,14 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
       return w * co.template comp<VX>(3, i) * ldv<T, VX>(r, i);
     } else if constexpr (MODE == 2) {
       V u = ldv<T, VX>(zi, i);
-      const long long cb = ((long long)(zz >> 1) * cny + (yy >> 1)) * cnx;
+      const long long cb = ((long long)(zz >> 1) * cny + (yy >> 1)) * cnx + (xx >> 1);
+      if constexpr (VX >= 2) {
+        // xx is a multiple of VX: the VX / 2 coarse cells are contiguous
+        const vec_t<T, VX / 2> c = ldv<T, VX / 2>(zc, cb);
 #pragma unroll
-      for (int v = 0; v < VX; ++v) {
-        const long long ci = cb + ((xx + v) >> 1);
-        u[v] += zsc ? zc[ci] + zsc[ci] : zc[ci];
+        for (int v = 0; v < VX; ++v) u[v] += c[v / 2];
+      } else {
+        u[0] += zc[cb];
       }
       return u;
     } else {
@@ -758,7 +770,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
       return w * co.comp1(3, i) * r[i];
     } else if constexpr (MODE == 2) {
       const long long ci = ((long long)(zz >> 1) * cny + (yy >> 1)) * cnx + (xx >> 1);
-      return zi[i] + (zsc ? zc[ci] + zsc[ci] : zc[ci]);
+      return zi[i] + zc[ci];
     } else {
       return zi[i];
     }
@@ -847,7 +859,10 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
         *reinterpret_cast<V*>(zsv + ic) = uc;
         *reinterpret_cast<V*>(rw + ic) = rv - lap;
       } else {
-        const V out = uc + w * dc * (rv - lap);
+        V out = uc + w * dc * (rv - lap);
+        if constexpr (MODE == 0) {
+          if (zsa) out += ldv<T, VX>(zsa, ic);
+        }
         *reinterpret_cast<V*>(zo + ic) = out;
         if constexpr (DOT) {
 #pragma unroll
@@ -891,13 +906,14 @@ template <typename T, bool L0>
 __global__ __launch_bounds__(kBlock) void k_mg_restrict(const SysDev* __restrict__ sds,
                                                         const Rec* __restrict__ recs,
                                                         const int* __restrict__ b2r, int l_arg,
-                                                        int zin, int wl) {
+                                                        int zin, int wl, int so) {
   const int l = L0 ? 0 : l_arg;
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   const Rec rc = recs[b2r[b]];
   const SysDev& s = sds[rc.sys];
   if (s.sc->done) return;
   if (wl >= 0 && wl >= s.mg_w) return;
+  if (so && !s.sc->first) return;
   const MgLevel& Lf = s.lev[l];
   const MgLevel& Lc = s.lev[l + 1];
   const LevelView<T> F(Lf);
@@ -920,46 +936,207 @@ __global__ __launch_bounds__(kBlock) void k_mg_restrict(const SysDev* __restrict
   }
 }
 
-// Coarsest level: `sweeps` damped Jacobi sweeps from z = 0, in LDS, one
-// workgroup per system (levels of at most kCoarseMax cells).
-constexpr int kCoarseMax = 8192;
+// A level of at most 64 cells held by the lanes of one wavefront (cell i on
+// lane i): neighbour values by lane shuffles, no LDS and no barriers.  Lanes
+// past the level's cells carry zero coefficients and read themselves.
+template <typename T>
+struct WaveLevel {
+  int n;
+  int nb[6];  // lanes of the xm, xp, ym, yp, zm, zp neighbours
+  T kx, kxm, ky, kym, kz, kzm, d, m;
+  __device__ void load(const MgLevel& L, int lane) {
+    n = (int)L.n;
+    const bool on = lane < n;
+    const int c = on ? lane : 0;
+    const int x = c % L.nx, y = (c / L.nx) % L.ny, z = c / (L.nx * L.ny);
+    const int pl = L.nx * L.ny;
+    nb[0] = on ? c - x + wrapm(x, L.nx) : lane;
+    nb[1] = on ? c - x + wrapp(x, L.nx) : lane;
+    nb[2] = on ? c + (wrapm(y, L.ny) - y) * L.nx : lane;
+    nb[3] = on ? c + (wrapp(y, L.ny) - y) * L.nx : lane;
+    nb[4] = on ? c + (wrapm(z, L.nz) - z) * pl : lane;
+    nb[5] = on ? c + (wrapp(z, L.nz) - z) * pl : lane;
+    const MgCoefs<T> co(L);
+    kx = on ? co.comp1(0, c) : (T)0;
+    kxm = on ? co.comp1(0, nb[0]) : (T)0;
+    ky = on ? co.comp1(1, c) : (T)0;
+    kym = on ? co.comp1(1, nb[2]) : (T)0;
+    kz = on ? co.comp1(2, c) : (T)0;
+    kzm = on ? co.comp1(2, nb[4]) : (T)0;
+    d = on ? co.comp1(3, c) : (T)0;
+    m = (on && L.m) ? ((const T*)L.m)[c] : (T)0;
+  }
+  // (L u) of this lane's cell; every lane of the wavefront must call it
+  __device__ T lap(T u) const {
+    const T uxm = __shfl(u, nb[0], 64), uxp = __shfl(u, nb[1], 64);
+    const T uym = __shfl(u, nb[2], 64), uyp = __shfl(u, nb[3], 64);
+    const T uzm = __shfl(u, nb[4], 64), uzp = __shfl(u, nb[5], 64);
+    T s = kx * (u - uxp) + kxm * (u - uxm) + ky * (u - uyp) + kym * (u - uym) + kz * (u - uzp) +
+          kzm * (u - uzm);
+    return s + m * u;
+  }
+};
+
+// The bottom of the cycle: levels lb..top of a system (its cycle from level
+// lb down; lb > every W level) in one workgroup.  Levels of more than 64
+// cells: z and a second buffer of every level in LDS, r in global memory
+// (cached), barriers between the steps; the last one or two levels of at most
+// 64 cells: one wavefront, a cell per lane.  The same operations as the
+// per-level launches: zero-guess sweep + sweep, restriction, `sweeps` damped
+// Jacobi sweeps from zero on the coarsest level, prolongation + sweep + sweep;
+// the correction of level lb goes to its z[1] (z[0] when lb is the coarsest).
+// Each system takes part only in the launch for its own lb (s.mg_lb == l0).
+// The engine chooses lb so that the levels fit in LDS.
+constexpr int kBottomThreads = 1024;
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_mg_coarse(const SysDev* __restrict__ sds,
-                                                      int sweeps, int wl) {
+__global__ __launch_bounds__(kBottomThreads) void k_mg_bottom(const SysDev* __restrict__ sds,
+                                                              int sweeps, int l0, int wl,
+                                                              int so) {
   extern __shared__ unsigned char smem[];
   const SysDev& s = sds[blockIdx.x];
-  if (s.nlev <= 0 || s.sc->done) return;
+  if (s.nlev <= 0 || s.sc->done || s.mg_lb != l0) return;
   if (wl >= 0 && wl >= s.mg_w) return;
-  const MgLevel& Lv = s.lev[s.nlev - 1];
-  const LevelView<T> V(Lv);
-  const int n = (int)Lv.n;
-  T* za = (T*)smem;
-  T* zb = za + n;
-  const T* __restrict__ r = (const T*)Lv.r;
+  if (so && !s.sc->first) return;
+  const int top = s.nlev - 1;
+  const int tid = threadIdx.x, nt = blockDim.x;
   const T w = (T)kOmega;
-  for (int i = threadIdx.x; i < n; i += kBlock) za[i] = w * V.dinv(i) * r[i];
-  __syncthreads();
-  for (int k = 1; k < sweeps; ++k) {
-    for (int i = threadIdx.x; i < n; i += kBlock) {
+  T* za[kMaxLevels];
+  T* zb[kMaxLevels];
+  {
+    T* p = (T*)smem;
+    for (int l = l0; l <= top; ++l) {
+      za[l] = p;
+      p += s.lev[l].n;
+      zb[l] = p;
+      p += s.lev[l].n;
+    }
+  }
+  // levels lw..top go to one wavefront (a level below one of <= 64 cells has
+  // <= 8, and a coarser one would have < 8: at most two such levels)
+  int lw = l0;
+  while (lw <= top && s.lev[lw].n > 64) ++lw;
+  // out = u + w D^-1 (r - L u) on level l, u and out in LDS
+  auto sweep = [&](int l, const T* u, T* out) {
+    const LevelView<T> V(s.lev[l]);
+    const T* __restrict__ r = (const T*)s.lev[l].r;
+    const int n = (int)s.lev[l].n;
+    for (int i = tid; i < n; i += nt) {
       const int x = i % V.nx, y = (i / V.nx) % V.ny, z = i / (V.nx * V.ny);
-      auto u = [&](long long j) -> T { return za[j]; };
-      zb[i] = za[i] + w * V.dinv(i) * (r[i] - V.lap(x, y, z, u));
+      auto uf = [&](long long j) -> T { return u[j]; };
+      out[i] = u[i] + w * V.dinv(i) * (r[i] - V.lap(x, y, z, uf));
     }
     __syncthreads();
-    T* t = za;
-    za = zb;
-    zb = t;
+  };
+  for (int l = l0; l < lw && l < top; ++l) {
+    const LevelView<T> V(s.lev[l]);
+    const T* __restrict__ r = (const T*)s.lev[l].r;
+    const int n = (int)s.lev[l].n;
+    for (int i = tid; i < n; i += nt) zb[l][i] = w * V.dinv(i) * r[i];
+    __syncthreads();
+    sweep(l, zb[l], za[l]);
+    // restriction of r - L z into r of level l + 1
+    const MgLevel& Lc = s.lev[l + 1];
+    T* __restrict__ rc = (T*)Lc.r;
+    const T* zl = za[l];
+    auto uf = [&](long long j) -> T { return zl[j]; };
+    for (int c = tid; c < (int)Lc.n; c += nt) {
+      const int X = c % Lc.nx, Y = (c / Lc.nx) % Lc.ny, Z = c / (Lc.nx * Lc.ny);
+      T sum = 0;
+      for (int dz = 0; dz < 2; ++dz)
+        for (int dy = 0; dy < 2; ++dy)
+          for (int dx = 0; dx < 2; ++dx) {
+            const int fx = 2 * X + dx, fy = 2 * Y + dy, fz = 2 * Z + dz;
+            sum += r[V.at(fx, fy, fz)] - V.lap(fx, fy, fz, uf);
+          }
+      rc[c] = sum;
+    }
+    __syncthreads();
   }
-  T* out = (T*)Lv.z[0];
-  for (int i = threadIdx.x; i < n; i += kBlock) out[i] = za[i];
+  if (lw <= top) {
+    if (tid < 64) {
+      const int lane = tid;
+      WaveLevel<T> A;
+      A.load(s.lev[lw], lane);
+      const T ra = lane < A.n ? ((const T*)s.lev[lw].r)[lane] : (T)0;
+      T z = w * A.d * ra;
+      if (lw == top) {
+        for (int k = 1; k < sweeps; ++k) z = z + w * A.d * (ra - A.lap(z));
+      } else {
+        const MgLevel& Lf = s.lev[lw];
+        const MgLevel& Lc = s.lev[lw + 1];
+        z = z + w * A.d * (ra - A.lap(z));
+        const T res = ra - A.lap(z);
+        WaveLevel<T> B;
+        B.load(Lc, lane);
+        const bool con = lane < B.n;
+        const int C = con ? lane : 0;
+        const int X = C % Lc.nx, Y = (C / Lc.nx) % Lc.ny, Z = C / (Lc.nx * Lc.ny);
+        T rb = 0;
+        for (int dz = 0; dz < 2; ++dz)
+          for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx) {
+              const int f = ((2 * Z + dz) * Lf.ny + 2 * Y + dy) * Lf.nx + 2 * X + dx;
+              rb += __shfl(res, f, 64);
+            }
+        if (!con) rb = (T)0;
+        T zc = w * B.d * rb;
+        for (int k = 1; k < sweeps; ++k) zc = zc + w * B.d * (rb - B.lap(zc));
+        // prolongation of the coarsest correction, two sweeps
+        const int c = lane < A.n ? lane : 0;
+        const int x = c % Lf.nx, y = (c / Lf.nx) % Lf.ny, zz = c / (Lf.nx * Lf.ny);
+        const T u = z + __shfl(zc, ((zz >> 1) * Lc.ny + (y >> 1)) * Lc.nx + (x >> 1), 64);
+        z = u + w * A.d * (ra - A.lap(u));
+        z = z + w * A.d * (ra - A.lap(z));
+      }
+      if (lane < A.n) za[lw][lane] = z;
+    }
+    __syncthreads();
+  } else {
+    // a coarsest level of more than 64 cells: block-wide sweeps
+    const LevelView<T> V(s.lev[top]);
+    const T* __restrict__ r = (const T*)s.lev[top].r;
+    const int n = (int)s.lev[top].n;
+    T* a = za[top];
+    T* b = zb[top];
+    for (int i = tid; i < n; i += nt) a[i] = w * V.dinv(i) * r[i];
+    __syncthreads();
+    for (int k = 1; k < sweeps; ++k) {
+      sweep(top, a, b);
+      T* t = a;
+      a = b;
+      b = t;
+    }
+    if (a != za[top]) {
+      for (int i = tid; i < n; i += nt) za[top][i] = a[i];
+      __syncthreads();
+    }
+  }
+  for (int l = (lw <= top ? lw : top) - 1; l >= l0; --l) {
+    const MgLevel& Lf = s.lev[l];
+    const MgLevel& Lc = s.lev[l + 1];
+    const int n = (int)Lf.n;
+    for (int i = tid; i < n; i += nt) {
+      const int x = i % Lf.nx, y = (i / Lf.nx) % Lf.ny, z = i / (Lf.nx * Lf.ny);
+      zb[l][i] = za[l][i] + za[l + 1][((long long)(z >> 1) * Lc.ny + (y >> 1)) * Lc.nx + (x >> 1)];
+    }
+    __syncthreads();
+    sweep(l, zb[l], za[l]);
+    sweep(l, za[l], zb[l]);
+    for (int i = tid; i < n; i += nt) za[l][i] = zb[l][i];
+    __syncthreads();
+  }
+  T* out = (T*)s.lev[l0].z[l0 == top ? 0 : 1];
+  for (int i = tid; i < (int)s.lev[l0].n; i += nt) out[i] = za[l0][i];
 }
 
 // (r, z) of the preconditioned CG: beta = (r,z) / (r,z)_old (0 after a start)
-__global__ __launch_bounds__(kBlock) void k_fin_precond(const SysDev* __restrict__ sds) {
+__global__ __launch_bounds__(kBlock) void k_fin_precond(const SysDev* __restrict__ sds,
+                                                        int so) {
   const SysDev& s = sds[blockIdx.x];
   Scalars* sc = s.sc;
   if (s.nlev <= 0 || sc->done) return;
+  if (so && !sc->first) return;
   double acc[1];
   reduce_partials<1, 0u>(acc, s.partials, s.nblk_pc);
   if (threadIdx.x == 0) {
@@ -1551,32 +1728,33 @@ int iter_kernels(int method, int pc) {
 // levels drop out of the deeper launches), then (r, z) and beta.
 template <typename T, int MODE, bool DOT, bool L0>
 static void launch_sweep_l(const LaunchSet& L, hipStream_t st, int l, int zin, int zout,
-                           int wl) {
+                           int wl, int addzs) {
   switch (L.vx_l[l]) {
     case 4:
       APH_LAUNCH((k_mg_sweep<T, 4, MODE, DOT, L0>), L.nlf[l], st, L.sds, L.rec_lf[l],
-                 L.b2r_lf[l], l, zin, zout, wl);
+                 L.b2r_lf[l], l, zin, zout, wl, L.start_only, addzs);
       break;
     case 2:
       APH_LAUNCH((k_mg_sweep<T, 2, MODE, DOT, L0>), L.nlf[l], st, L.sds, L.rec_lf[l],
-                 L.b2r_lf[l], l, zin, zout, wl);
+                 L.b2r_lf[l], l, zin, zout, wl, L.start_only, addzs);
       break;
     default:
       if constexpr (MODE == 3) {
         APH_LAUNCH((k_mg_restrict<T, L0>), L.nlc[l], st, L.sds, L.rec_lc[l], L.b2r_lc[l], l,
-                   zin, wl);
+                   zin, wl, L.start_only);
       } else {
         APH_LAUNCH((k_mg_sweep<T, 1, MODE, DOT, L0>), L.nlf[l], st, L.sds, L.rec_lf[l],
-                   L.b2r_lf[l], l, zin, zout, wl);
+                   L.b2r_lf[l], l, zin, zout, wl, L.start_only, addzs);
       }
       break;
   }
 }
 
 template <typename T, int MODE, bool DOT>
-static void launch_sweep(const LaunchSet& L, hipStream_t st, int l, int zin, int zout, int wl) {
-  if (l == 0) launch_sweep_l<T, MODE, DOT, true>(L, st, l, zin, zout, wl);
-  else launch_sweep_l<T, MODE, DOT, false>(L, st, l, zin, zout, wl);
+static void launch_sweep(const LaunchSet& L, hipStream_t st, int l, int zin, int zout, int wl,
+                         int addzs = 0) {
+  if (l == 0) launch_sweep_l<T, MODE, DOT, true>(L, st, l, zin, zout, wl, addzs);
+  else launch_sweep_l<T, MODE, DOT, false>(L, st, l, zin, zout, wl, addzs);
 }
 
 // The cycle from level l down, recursively.  Per level above the coarsest:
@@ -1588,33 +1766,38 @@ static void launch_sweep(const LaunchSet& L, hipStream_t st, int l, int zin, int
 // level 0 it also forms (r, z)).  The correction of every level ends in z[1],
 // of the coarsest in z[0].  wl: the innermost W level whose second correction
 // this is (-1: none); each system takes part only while wl < its mg_w.
+// second: this cycle is the second correction of level l - 1; its last sweep
+// adds the first correction (zs of level l), so z[1] of level l holds the sum
+// the prolongation needs.  (A second correction starts at a level <= mg_w,
+// above the bottom kernel's first level, and below the coarsest.)
 // `mark(kind)` precedes every launch (profiling): 0 = level-0 pre-smoothing
-// sweep, 1 = level-0 restriction, 2 = coarser-level launch, 3 = coarsest
-// solve, 5 and 6 = level-0 first and second post-smoothing sweeps.
+// sweep, 1 = level-0 restriction, 2 = coarser-level launch, 3 = bottom
+// levels (k_mg_bottom), 5 and 6 = level-0 first and second post-smoothing
+// sweeps.
 template <typename T, typename Mark>
-static void launch_cycle(const LaunchSet& L, hipStream_t st, int l, int wl, Mark& mark) {
-  const int top = L.nlev_max - 1;
-  if (l == top) {
+static void launch_cycle(const LaunchSet& L, hipStream_t st, int l, int wl, bool second,
+                         Mark& mark) {
+  if (L.bottom_at[l]) {
     mark(3);
-    hipLaunchKernelGGL(k_mg_coarse<T>, dim3(L.ns), dim3(kBlock), L.coarse_lds, st, L.sds,
-                       L.coarse_sweeps, wl);
-    return;
+    hipLaunchKernelGGL(k_mg_bottom<T>, dim3(L.ns), dim3(kBottomThreads), L.coarse_lds, st,
+                       L.sds, L.coarse_sweeps, l, wl, L.start_only);
   }
+  if (l >= L.deep) return;  // every system's cycle ends at or above level l
   mark(l == 0 ? 0 : 2);
   launch_sweep<T, 1, false>(L, st, l, 0, 1, wl);
   mark(l == 0 ? 1 : 2);
   launch_sweep<T, 3, false>(L, st, l, 1, 1, wl);
-  launch_cycle<T>(L, st, l + 1, wl, mark);
-  if (l < L.mg_w && l + 1 < top) {
+  launch_cycle<T>(L, st, l + 1, wl, false, mark);
+  if (l < L.mg_w) {
     mark(2);
     launch_sweep<T, 4, false>(L, st, l + 1, 1, 1, l);
-    launch_cycle<T>(L, st, l + 1, l, mark);
+    launch_cycle<T>(L, st, l + 1, l, true, mark);
   }
   mark(l == 0 ? 5 : 2);
   launch_sweep<T, 2, false>(L, st, l, 1, 0, wl);
   mark(l == 0 ? 6 : 2);
   if (l == 0) launch_sweep<T, 0, true>(L, st, l, 0, 1, wl);
-  else launch_sweep<T, 0, false>(L, st, l, 0, 1, wl);
+  else launch_sweep<T, 0, false>(L, st, l, 0, 1, wl, second ? 1 : 0);
 }
 
 // The preconditioner z = M r on level 0 and the (r, z) finalize; `ev`
@@ -1631,9 +1814,9 @@ static int launch_vcycle(const LaunchSet& L, hipStream_t st, hipEvent_t* ev = nu
     }
     ++k;
   };
-  launch_cycle<T>(L, st, 0, -1, mark);
+  launch_cycle<T>(L, st, 0, -1, false, mark);
   mark(4);
-  APH_LAUNCH(k_fin_precond, L.ns, st, L.sds);
+  APH_LAUNCH(k_fin_precond, L.ns, st, L.sds, L.start_only);
   if (ev) (void)hipEventRecord(ev[k], st);
   return k;
 }
@@ -1646,10 +1829,10 @@ hipError_t launch_vcycle_profiled(int fp32, const LaunchSet& L, hipStream_t st, 
 }
 
 static int cycle_launches(const LaunchSet& L, int l) {
-  const int top = L.nlev_max - 1;
-  if (l >= top) return 1;
-  int n = 4 + cycle_launches(L, l + 1);
-  if (l < L.mg_w && l + 1 < top) n += 1 + cycle_launches(L, l + 1);
+  int n = L.bottom_at[l] ? 1 : 0;
+  if (l >= L.deep) return n;
+  n += 4 + cycle_launches(L, l + 1);
+  if (l < L.mg_w) n += 1 + cycle_launches(L, l + 1);
   return n;
 }
 
@@ -1710,9 +1893,14 @@ hipError_t launch_start(int method, int fp32, int vx, const LaunchSet& L, hipStr
     APH_LAUNCH(k_fin_ref_init, L.ns, st, L.sds);
   } else if (method == kPcg) {
     // the start is a reliable update (ru = 1): x64 = x0, r^ = S (b - A x0),
-    // followed by z = M^-1 r^ with the multigrid preconditioner
+    // followed by z = M^-1 r^ with the multigrid preconditioner.  Systems
+    // already iterating (continuous batching) are at an iteration boundary and
+    // take no part: the reliable-update kernels skip them (ru = 0), the cycle
+    // and its finalize are launched start-only (first == 1).
+    LaunchSet Ls = L;
+    Ls.start_only = 1;
     for (int j = 4; j < iter_kernels(method, L.pc); ++j) {
-      const hipError_t e = launch_kernel(method, fp32, vx, L, st, j);
+      const hipError_t e = launch_kernel(method, fp32, vx, Ls, st, j);
       if (e != hipSuccess) return e;
     }
   }

@@ -79,6 +79,11 @@ enum {
   APHPOIS_NULL_ON = 2
 };
 
+/* W levels of the multigrid cycle by default: at 512^3 (9 levels) the
+ * mixed-precision solve to 1e-8 takes 117 iterations with a V-cycle and 44
+ * with three W levels, at a third more work per iteration. */
+#define APHPOIS_MG_W_DEFAULT 3
+
 typedef struct aphpois_desc {
   int nx, ny, nz;   /* inner cells; the grid is periodic in x, y and z */
   double h3;        /* GetCellSize().prod(), used by the reference norms */
@@ -95,7 +100,8 @@ typedef struct aphpois_desc {
   long long mem_budget; /* bytes of device memory this handle may use, 0 = no limit */
   int precond;      /* PCG preconditioner: APHPOIS_PRECOND_* */
   int mg_w;         /* multigrid: the top mg_w levels apply their coarse
-                       correction twice (0 = V-cycle); see aphpois_desc_default */
+                       correction twice (0 = V-cycle); default
+                       APHPOIS_MG_W_DEFAULT */
 } aphpois_desc;
 
 typedef struct aphpois_info {
@@ -222,7 +228,7 @@ int aphpois_set_profiling(aphpois* h, int on);
  * reducing kernel is followed by its finalize kernel): PCG {spmv, fin,
  * update, fin, ru_accum, fin, ru_resid, fin} and, with the multigrid
  * preconditioner, {V-cycle (all its launches together), level-0 sweep (per
- * launch), level-0 restriction, coarser levels (per V-cycle), coarsest solve,
+ * launch), level-0 restriction, coarser levels (per cycle), bottom levels,
  * level-0 pre-sweep, level-0 first post-sweep, level-0 second post-sweep};
  * CONJUGATE {spmv, fin, update, fin}; JACOBI {sweep, fin}.  avg_ms and count
  * hold 16 entries; *nkernels receives the number used. */

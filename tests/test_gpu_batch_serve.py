@@ -26,7 +26,8 @@ def _sys(n, ns, rmin, rmax, ratio, seed):
     return e.reshape(n, n, n, 8)
 
 
-def _single(E, precision=L.MIXED, x0=None, tol=1e-8, precond=L.PRECOND_JACOBI, mg_w=0):
+def _single(E, precision=L.MIXED, x0=None, tol=1e-8, precond=L.PRECOND_JACOBI,
+            mg_w=L.MG_W_DEFAULT):
     s = Poisson(E.shape[:3], method=L.PCG, precision=precision, resmode=L.RES_REL_TRUE,
                 tol=tol, maxiter=50000, precond=precond, mg_w=mg_w)
     s.set_system(E)
@@ -37,7 +38,7 @@ def _single(E, precision=L.MIXED, x0=None, tol=1e-8, precond=L.PRECOND_JACOBI, m
 
 @pytest.mark.parametrize("precision", [L.MIXED, L.FP64])
 @pytest.mark.parametrize("precond,w", [(L.PRECOND_JACOBI, 0), (L.PRECOND_MG, 0),
-                                       (L.PRECOND_MG, 3)])
+                                       (L.PRECOND_MG, 3), (L.PRECOND_MG, 6)])
 def test_batch_equals_single(precision, precond, w):
     """Systems of different depths (32^3 and 48^3: 5 levels, 64^3: 6): with
     W levels each system keeps its own number (mg_w clamped to its depth)."""
@@ -74,9 +75,10 @@ def test_batch_warm_start_new_rhs():
     b.close()
 
 
-def test_server_alone_equals_mix():
+@pytest.mark.parametrize("precond", [L.PRECOND_JACOBI, L.PRECOND_MG])
+def test_server_alone_equals_mix(precond):
     A = _sys(*SYSTEMS[2])
-    srv = Server(precision=L.MIXED, tol=1e-8, mem_budget=0)
+    srv = Server(precision=L.MIXED, tol=1e-8, mem_budget=0, precond=precond)
     ra = srv.submit(A, nsteps=3, perturb=0.01, seed=5, want_x=True)
     alone = srv.wait(ra, timeout=120)
     assert alone["status"] == 0 and alone["steps_done"] == 3
