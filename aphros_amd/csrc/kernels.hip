@@ -589,13 +589,21 @@ __global__ __launch_bounds__(kBlock) void k_fin_ru_resid(const SysDev* __restric
 // ===========================================================================
 // Multigrid preconditioner z = M^-1 r: one symmetric V-cycle over the levels
 // of each system.  Level operators (L u)_c = sum_f k_f (u_c - u_nb) + m_c u_c
-// with the coarse conductances from coarsening (k_mg_coarsen); damped Jacobi
-// smoothing (weight kOmega, as many sweeps after the coarse correction as
-// before it); restriction = sum over the 2x2x2 children, prolongation =
-// injection (its transpose), so the V-cycle is a symmetric linear operator.
+// with the coarse conductances from coarsening (k_mg_coarsen); smoothing by
+// two damped Jacobi sweeps before and two after the coarse correction, with
+// the weights kOmegaA, kOmegaB of the degree-2 Chebyshev polynomial for the
+// spectrum interval [kChebLo, kChebHi] of D^-1 L (w = 1 / (c -+ d cos(pi/4)),
+// c = (lo + hi) / 2, d = (hi - lo) / 2): pre-smoothing A then B (A from a
+// zero guess), post-smoothing B then A; both apply the same polynomial of
+// D^-1 L, so the cycle stays symmetric.  The coarsest level: `sweeps` sweeps
+// of weight kOmega.  Restriction = sum over the 2x2x2 children, prolongation
+// = injection (its transpose), so the V-cycle is a symmetric linear operator.
 // Arithmetic in the engine's element type T.
 // ===========================================================================
 constexpr double kOmega = 0.9;
+constexpr double kChebLo = 0.5, kChebHi = 1.8;
+constexpr double kOmegaA = 0.6212648748965077;  // 1 / (c + d cos(pi/4))
+constexpr double kOmegaB = 1.4484764074432;     // 1 / (c - d cos(pi/4))
 
 template <typename T>
 struct LevelView {
@@ -737,7 +745,9 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
   if constexpr (MODE == 0) {
     if (addzs) zsa = (const T*)Lv.zs;
   }
-  const T w = (T)kOmega;
+  // MODE 1: zero guess (weight A) + sweep B; MODE 2: sweep B; MODE 0: sweep A
+  const T wa = (T)kOmegaA;
+  const T w = MODE == 0 ? (T)kOmegaA : (T)kOmegaB;
   const long long pl = (long long)nx * ny;
   const long long row = (long long)y * nx;
   const int ym = wrapm(y, ny), yp = wrapp(y, ny);
@@ -747,7 +757,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
   auto uv = [&](int xx, int yy, int zz) -> V {
     const long long i = (long long)zz * pl + (long long)yy * nx + xx;
     if constexpr (MODE == 1) {
-      return w * co.template comp<VX>(3, i) * ldv<T, VX>(r, i);
+      return wa * co.template comp<VX>(3, i) * ldv<T, VX>(r, i);
     } else if constexpr (MODE == 2) {
       V u = ldv<T, VX>(zi, i);
       const long long cb = ((long long)(zz >> 1) * cny + (yy >> 1)) * cnx + (xx >> 1);
@@ -767,7 +777,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
   auto u1 = [&](int xx, int yy, int zz) -> T {
     const long long i = (long long)zz * pl + (long long)yy * nx + xx;
     if constexpr (MODE == 1) {
-      return w * co.comp1(3, i) * r[i];
+      return wa * co.comp1(3, i) * r[i];
     } else if constexpr (MODE == 2) {
       const long long ci = ((long long)(zz >> 1) * cny + (yy >> 1)) * cnx + (xx >> 1);
       return zi[i] + zc[ci];
@@ -784,7 +794,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
     if constexpr (MODE == 1) {
       const long long i0 = (long long)za * pl + row + x0;
       rcur = ldv<T, VX>(r, i0);
-      uc = w * co.template comp<VX>(3, i0) * rcur;
+      uc = wa * co.template comp<VX>(3, i0) * rcur;
     } else {
       uc = uv(x0, y, za);
     }
@@ -805,7 +815,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
       if constexpr (MODE == 1) {
         const long long in = (long long)wrapp(z, nz) * pl + row + x0;
         rnext = ldv<T, VX>(r, in);
-        un = w * co.template comp<VX>(3, in) * rnext;
+        un = wa * co.template comp<VX>(3, in) * rnext;
       } else {
         un = uv(x0, y, wrapp(z, nz));
       }
@@ -1000,7 +1010,7 @@ __global__ __launch_bounds__(kBottomThreads) void k_mg_bottom(const SysDev* __re
   if (so && !s.sc->first) return;
   const int top = s.nlev - 1;
   const int tid = threadIdx.x, nt = blockDim.x;
-  const T w = (T)kOmega;
+  const T w = (T)kOmega, wa = (T)kOmegaA, wb = (T)kOmegaB;
   T* za[kMaxLevels];
   T* zb[kMaxLevels];
   {
@@ -1016,15 +1026,15 @@ __global__ __launch_bounds__(kBottomThreads) void k_mg_bottom(const SysDev* __re
   // <= 8, and a coarser one would have < 8: at most two such levels)
   int lw = l0;
   while (lw <= top && s.lev[lw].n > 64) ++lw;
-  // out = u + w D^-1 (r - L u) on level l, u and out in LDS
-  auto sweep = [&](int l, const T* u, T* out) {
+  // out = u + om D^-1 (r - L u) on level l, u and out in LDS
+  auto sweep = [&](int l, const T* u, T* out, T om) {
     const LevelView<T> V(s.lev[l]);
     const T* __restrict__ r = (const T*)s.lev[l].r;
     const int n = (int)s.lev[l].n;
     for (int i = tid; i < n; i += nt) {
       const int x = i % V.nx, y = (i / V.nx) % V.ny, z = i / (V.nx * V.ny);
       auto uf = [&](long long j) -> T { return u[j]; };
-      out[i] = u[i] + w * V.dinv(i) * (r[i] - V.lap(x, y, z, uf));
+      out[i] = u[i] + om * V.dinv(i) * (r[i] - V.lap(x, y, z, uf));
     }
     __syncthreads();
   };
@@ -1032,9 +1042,9 @@ __global__ __launch_bounds__(kBottomThreads) void k_mg_bottom(const SysDev* __re
     const LevelView<T> V(s.lev[l]);
     const T* __restrict__ r = (const T*)s.lev[l].r;
     const int n = (int)s.lev[l].n;
-    for (int i = tid; i < n; i += nt) zb[l][i] = w * V.dinv(i) * r[i];
+    for (int i = tid; i < n; i += nt) zb[l][i] = wa * V.dinv(i) * r[i];
     __syncthreads();
-    sweep(l, zb[l], za[l]);
+    sweep(l, zb[l], za[l], wb);
     // restriction of r - L z into r of level l + 1
     const MgLevel& Lc = s.lev[l + 1];
     T* __restrict__ rc = (T*)Lc.r;
@@ -1059,13 +1069,15 @@ __global__ __launch_bounds__(kBottomThreads) void k_mg_bottom(const SysDev* __re
       WaveLevel<T> A;
       A.load(s.lev[lw], lane);
       const T ra = lane < A.n ? ((const T*)s.lev[lw].r)[lane] : (T)0;
-      T z = w * A.d * ra;
+      T z;
       if (lw == top) {
+        z = w * A.d * ra;
         for (int k = 1; k < sweeps; ++k) z = z + w * A.d * (ra - A.lap(z));
       } else {
         const MgLevel& Lf = s.lev[lw];
         const MgLevel& Lc = s.lev[lw + 1];
-        z = z + w * A.d * (ra - A.lap(z));
+        z = wa * A.d * ra;
+        z = z + wb * A.d * (ra - A.lap(z));
         const T res = ra - A.lap(z);
         WaveLevel<T> B;
         B.load(Lc, lane);
@@ -1086,8 +1098,8 @@ __global__ __launch_bounds__(kBottomThreads) void k_mg_bottom(const SysDev* __re
         const int c = lane < A.n ? lane : 0;
         const int x = c % Lf.nx, y = (c / Lf.nx) % Lf.ny, zz = c / (Lf.nx * Lf.ny);
         const T u = z + __shfl(zc, ((zz >> 1) * Lc.ny + (y >> 1)) * Lc.nx + (x >> 1), 64);
-        z = u + w * A.d * (ra - A.lap(u));
-        z = z + w * A.d * (ra - A.lap(z));
+        z = u + wb * A.d * (ra - A.lap(u));
+        z = z + wa * A.d * (ra - A.lap(z));
       }
       if (lane < A.n) za[lw][lane] = z;
     }
@@ -1102,7 +1114,7 @@ __global__ __launch_bounds__(kBottomThreads) void k_mg_bottom(const SysDev* __re
     for (int i = tid; i < n; i += nt) a[i] = w * V.dinv(i) * r[i];
     __syncthreads();
     for (int k = 1; k < sweeps; ++k) {
-      sweep(top, a, b);
+      sweep(top, a, b, w);
       T* t = a;
       a = b;
       b = t;
@@ -1121,8 +1133,8 @@ __global__ __launch_bounds__(kBottomThreads) void k_mg_bottom(const SysDev* __re
       zb[l][i] = za[l][i] + za[l + 1][((long long)(z >> 1) * Lc.ny + (y >> 1)) * Lc.nx + (x >> 1)];
     }
     __syncthreads();
-    sweep(l, zb[l], za[l]);
-    sweep(l, za[l], zb[l]);
+    sweep(l, zb[l], za[l], wb);
+    sweep(l, za[l], zb[l], wa);
     for (int i = tid; i < n; i += nt) za[l][i] = zb[l][i];
     __syncthreads();
   }

@@ -499,15 +499,16 @@ int orc_pcg_jacobi(const double* expr, int nx, int ny, int nz, const double* x0,
  * conductance is 0.5 * (sum of the 4 fine faces it covers), the coarse mass
  * 0.5 * (sum over the 8 children); D = diagonal of the level operator.
  *
- * V-cycle z = M r on level l (w = 0.9):
- *   l < top: u = w D^-1 r; z = u + w D^-1 (r - L u);
+ * V-cycle z = M r on level l (a, b = kMgOmegaA, kMgOmegaB; w = 0.9):
+ *   l < top: u = a D^-1 r; z = u + b D^-1 (r - L u);
  *            r_{l+1} = sum over the 2x2x2 children of (r - L z);
  *            z_{l+1} = M r_{l+1};
  *            u = z + P z_{l+1} (P = injection to the children);
- *            z = u + w D^-1 (r - L u); z = z + w D^-1 (r - L z)
+ *            z = u + b D^-1 (r - L u); z = z + a D^-1 (r - L z)
  *   top:     u = w D^-1 r, then `sweeps - 1` more damped Jacobi sweeps
- * Restriction is the transpose of the injection and the pre and post sweeps
- * mirror each other, so M is symmetric.
+ * Restriction is the transpose of the injection, and pre- and post-smoothing
+ * apply the same polynomial of D^-1 L (the error factor (1 - a D^-1 L)
+ * (1 - b D^-1 L), a Chebyshev polynomial of degree 2), so M is symmetric.
  * W levels: on the top w levels (l < w, and only where level l+1 is not the
  * coarsest: w <= nlev - 2) the coarse correction is applied twice: with
  * zs = z_{l+1}, r_{l+1} -= L_{l+1} zs, z_{l+1} = M r_{l+1}, and the
@@ -527,7 +528,12 @@ typedef struct {
   double *kx, *ky, *kz, *m, *dinv, *r, *z, *t;
 } mglev_t;
 
+/* coarsest level: damped Jacobi, weight kMgOmega; other levels: the two
+ * weights of the degree-2 Chebyshev polynomial on [0.5, 1.8] of the spectrum
+ * of D^-1 L (1 / (1.15 +- 0.65 cos(pi/4))) */
 static const double kMgOmega = 0.9;
+static const double kMgOmegaA = 0.6212648748965077;
+static const double kMgOmegaB = 1.4484764074432;
 
 static double wall_seconds(void) {
 #ifdef _OPENMP
@@ -552,13 +558,13 @@ static inline double mg_lap(const mglev_t* L, const double* u, int x, int y, int
 }
 
 /* out = u + w D^-1 (r - L u) */
-static void mg_sweep(const mglev_t* L, const double* u, double* out) {
+static void mg_sweep(const mglev_t* L, const double* u, double* out, double w) {
 #pragma omp parallel for schedule(static)
   for (int z = 0; z < L->nz; ++z)
     for (int y = 0; y < L->ny; ++y)
       for (int x = 0; x < L->nx; ++x) {
         const size_t c = (size_t)x + (size_t)L->nx * ((size_t)y + (size_t)L->ny * z);
-        out[c] = u[c] + kMgOmega * L->dinv[c] * (L->r[c] - mg_lap(L, u, x, y, z));
+        out[c] = u[c] + w * L->dinv[c] * (L->r[c] - mg_lap(L, u, x, y, z));
       }
 }
 
@@ -566,10 +572,11 @@ static void mg_vcycle(mglev_t* lev, int l, int top, int sweeps, int w) {
   mglev_t* L = &lev[l];
   const long long n = (long long)L->n;
 #pragma omp parallel for schedule(static)
-  for (long long c = 0; c < n; ++c) L->t[c] = kMgOmega * L->dinv[c] * L->r[c];
+  for (long long c = 0; c < n; ++c)
+    L->t[c] = (l == top ? kMgOmega : kMgOmegaA) * L->dinv[c] * L->r[c];
   if (l == top) {
     for (int k = 1; k < sweeps; ++k) {
-      mg_sweep(L, L->t, L->z);
+      mg_sweep(L, L->t, L->z, kMgOmega);
       double* tmp = L->t;
       L->t = L->z;
       L->z = tmp;
@@ -579,7 +586,7 @@ static void mg_vcycle(mglev_t* lev, int l, int top, int sweeps, int w) {
     L->z = tmp;
     return;
   }
-  mg_sweep(L, L->t, L->z);
+  mg_sweep(L, L->t, L->z, kMgOmegaB);
   mglev_t* C = &lev[l + 1];
 #pragma omp parallel for schedule(static)
   for (int z = 0; z < C->nz; ++z)
@@ -622,8 +629,8 @@ static void mg_vcycle(mglev_t* lev, int l, int top, int sweeps, int w) {
         L->t[c] = L->z[c] + (zs ? C->z[cc] + zs[cc] : C->z[cc]);
       }
   free(zs);
-  mg_sweep(L, L->t, L->z);
-  mg_sweep(L, L->z, L->t);
+  mg_sweep(L, L->t, L->z, kMgOmegaB);
+  mg_sweep(L, L->z, L->t, kMgOmegaA);
   double* tmp = L->t;
   L->t = L->z;
   L->z = tmp;

@@ -18,7 +18,11 @@ import pytest
 
 from oracle import oracle as O
 
-OMEGA = 0.9
+OMEGA = 0.9  # coarsest level
+# the degree-2 Chebyshev polynomial on [0.5, 1.8] for D^-1 A: its two roots
+_C, _D = (0.5 + 1.8) / 2, (1.8 - 0.5) / 2
+OMEGA_A = 1 / (_C + _D * np.cos(np.pi / 4))
+OMEGA_B = 1 / (_C - _D * np.cos(np.pi / 4))
 
 
 def _system(n, seed, ratio=1000.0, mass=0.0, sign=1, shape=None):
@@ -73,28 +77,28 @@ def _dense_vcycle(e, sweeps, scale=0.5, shift=0, w=0):
         A.append(scale * P.T @ A[-1] @ P)
         shapes.append(cs)
 
-    def smooth(l, u, r):
+    def smooth(l, u, r, om):
         d = np.diag(A[l])
         dinv = np.where(d != 0, 1 / np.where(d != 0, d, 1), 0)
-        return u + OMEGA * dinv * (r - A[l] @ u)
+        return u + om * dinv * (r - A[l] @ u)
 
     def vc(l, r):
         d = np.diag(A[l])
         dinv = np.where(d != 0, 1 / np.where(d != 0, d, 1), 0)
-        u = OMEGA * dinv * r
         if l == len(A) - 1:
+            u = OMEGA * dinv * r
             for _ in range(sweeps - 1):
-                u = smooth(l, u, r)
+                u = smooth(l, u, r, OMEGA)
             return u
-        z = smooth(l, u, r)
+        z = smooth(l, OMEGA_A * dinv * r, r, OMEGA_B)
         rc = Ps[l].T @ (r - A[l] @ z)
         zc = vc(l + 1, rc)
         if l < w and l + 1 < len(A) - 1:
             zs = zc
             zc = vc(l + 1, rc - A[l + 1] @ zs)
             zc = zc + zs
-        z = smooth(l, z + Ps[l] @ zc, r)
-        return smooth(l, z, r)
+        z = smooth(l, z + Ps[l] @ zc, r, OMEGA_B)
+        return smooth(l, z, r, OMEGA_A)
 
     n = A[0].shape[0]
     return np.stack([vc(0, np.eye(n)[:, j]) for j in range(n)], axis=1), len(A)
@@ -137,9 +141,27 @@ def test_w_levels_symmetric_and_fewer_iterations():
     z2, _ = O.mg_apply(e, r2, w=2)
     a, b = np.vdot(z1, r2), np.vdot(r1, z2)
     assert abs(a - b) <= 1e-12 * np.sqrt(np.vdot(z1, r1) * np.vdot(z2, r2))
+    # heavy spheres on a 64^3 grid (6 levels): the second corrections pay off
+    e = _spheres(64, 12, 1)
     _, it0, rel0, _ = O.mgpcg(e, tol=1e-10, w=0)
     _, it2, rel2, _ = O.mgpcg(e, tol=1e-10, w=2)
     assert rel0 <= 1e-10 and rel2 <= 1e-10 and it2 < it0
+
+
+def _spheres(n, k, seed, ratio=1000.0):
+    """Density `ratio` inside k random periodic spheres (radius 0.05-0.15)."""
+    rng = np.random.default_rng(seed)
+    g = (np.arange(n) + 0.5) / n
+    Z, Y, X = np.meshgrid(g, g, g, indexing="ij")
+    rho = np.ones((n, n, n))
+    for _ in range(k):
+        c = rng.random(3)
+        r = 0.05 + 0.1 * rng.random()
+        d2 = sum(np.minimum(np.abs(a - ci), 1 - np.abs(a - ci)) ** 2 for a, ci in zip((X, Y, Z), c))
+        rho[d2 < r * r] = ratio
+    f = rng.standard_normal((n, n, n))
+    f -= f.mean()
+    return O.assemble(rho, f)
 
 
 def test_vcycle_symmetric_positive():
