@@ -43,7 +43,9 @@ struct Scalars {
 // One level of the multigrid preconditioner (level 0 = the fine grid).
 // Operator of every level: (L u)_c = sum_faces k_f (u_c - u_nb) + m_c u_c.
 struct MgLevel {
-  int nx, ny, nz, pad;
+  int nx, ny, nz;
+  int post;  // 1: both post-smoothing sweeps fused (k_mg_post); the
+             // pre-smoothed iterate is then z[0], else z[1]
   long long n;
   // coefficients: FP64 engines: plus-face conductances and D^-1 as separate
   // arrays (kx, ky, kz, dinv); FP32 engines: all four in bfloat16, packed by
@@ -110,6 +112,7 @@ struct Rec {
   int lx;  // threads per row (stencil), rows per block = kBlock / lx
   int bz;  // planes per block (stencil)
   int nblk;
+  int ty;  // rows of threads of a whole-row tile (k_mg_post: output rows + 2 halo rows)
 };
 
 // Physical workgroup id -> logical id such that the workgroups that run on

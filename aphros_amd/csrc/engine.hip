@@ -43,6 +43,33 @@ Rec make_stencil_rec(int nx, int ny, int nz, int vx, int blocks_target, const Tu
   return r;
 }
 
+// Fused post-smoothing (k_mg_post) of a level: whole-row tiles of lx = nx / vx
+// threads per row and ty = min(kPostBlock / lx, ny + 2) rows of threads
+// (ty - 2 output rows), bz planes.  Only for the large levels (n >= 2^24
+// cells: the march is long enough to hide the halo rows and planes it
+// recomputes) whose rows fit 1/8 of the workgroup (ty >= 8 rows).
+constexpr long long kFuseMinCells = 1LL << 24;
+
+bool make_post_rec(int nx, int ny, int nz, int vx, Rec* out) {
+  const long long n = (long long)nx * ny * nz;
+  if (vx <= 0 || nx % vx != 0 || n < kFuseMinCells) return false;
+  const int lx = nx / vx;
+  if (lx > kPostBlock / 8) return false;
+  Rec r{};
+  r.lx = lx;
+  r.ty = std::min(kPostBlock / lx, ny + 2);
+  r.tiles_x = 1;
+  r.tiles_y = (int)cdiv(ny, r.ty - 2);
+  // about 1024 workgroups (two resident per CU), marches of >= 8 planes
+  const long long target = std::min(1024LL, std::max(1LL, n / 16384));
+  const long long tz = std::max(1LL, target / r.tiles_y);
+  r.bz = (int)std::min((long long)nz, std::max((long long)std::min(8, nz), cdiv(nz, tz)));
+  r.tiles_z = (int)cdiv(nz, r.bz);
+  r.nblk = r.tiles_y * r.tiles_z;
+  *out = r;
+  return true;
+}
+
 }  // namespace
 
 int Engine::fail(int code, const std::string& msg) {
@@ -326,6 +353,14 @@ int Engine::add_system(int nx, int ny, int nz, double h3, int* slot) {
         }
       }
       S.dev.mg_lb = lb;
+      // levels above the bottom kernel that fuse their post-smoothing (a
+      // property of the system alone, like all of its tiling)
+      for (int l = 0; l < lb; ++l) {
+        Rec t;
+        S.lev[l].post = make_post_rec(S.lev[l].nx, S.lev[l].ny, S.lev[l].nz, fp32_ ? 4 : 2, &t)
+                            ? 1
+                            : 0;
+      }
       for (int l = 1; l <= w && rc == APHPOIS_OK; ++l) {
         void* zs = nullptr;
         A(&zs, S.lev[l].n * es);
@@ -711,6 +746,10 @@ int Engine::build_tables() {
       }
     }
   }
+  // post-smoothing per level: fused for the systems whose level says so
+  int post_fused[kMaxLevels] = {}, post_plain[kMaxLevels] = {};
+  std::vector<std::vector<Rec>> rlp(kMaxLevels);
+  std::vector<std::vector<int>> blp(kMaxLevels);
   for (int k = 0; k < ns; ++k) {
     System& s = *slots_[launch_slots_[k]];
     // The tiling of a system depends on that system only, so its reduction
@@ -780,6 +819,18 @@ int Engine::build_tables() {
         for (int i = 0; i < g.nblk; ++i) blc[l].push_back((int)rlc[l].size());
         rlc[l].push_back(g);
         if (l == 0) nblk_pc = f.nblk;
+        if (F.post) {
+          post_fused[l] = 1;
+          Rec p{};
+          make_post_rec(F.nx, F.ny, F.nz, fp32_ ? 4 : 2, &p);
+          p.sys = k;
+          p.blk0 = (int)blp[l].size();
+          for (int i = 0; i < p.nblk; ++i) blp[l].push_back((int)rlp[l].size());
+          rlp[l].push_back(p);
+          if (l == 0) nblk_pc = p.nblk;
+        } else {
+          post_plain[l] = 1;
+        }
       }
     }
     s.dev.nblk_pc = nblk_pc;
@@ -803,8 +854,8 @@ int Engine::build_tables() {
   long long nrec = (long long)rst.size() + rsm.size() + rru.size();
   long long nb2r = (long long)bst.size() + bsm.size() + bru.size();
   for (int l = 0; l < kMaxLevels; ++l) {
-    nrec += (long long)rlf[l].size() + rlc[l].size();
-    nb2r += (long long)blf[l].size() + blc[l].size();
+    nrec += (long long)rlf[l].size() + rlc[l].size() + rlp[l].size();
+    nb2r += (long long)blf[l].size() + blc[l].size() + blp[l].size();
   }
   if (nrec > rec_cap_) {
     if (d_rec_) APH_HIP(hipFree(d_rec_));
@@ -828,16 +879,20 @@ int Engine::build_tables() {
   for (int v : bsm) b2r.push_back(v);
   for (int v : bru) b2r.push_back(v);
   size_t rec_lf_ofs[kMaxLevels], rec_lc_ofs[kMaxLevels], b2r_lf_ofs[kMaxLevels],
-      b2r_lc_ofs[kMaxLevels];
+      b2r_lc_ofs[kMaxLevels], rec_lp_ofs[kMaxLevels], b2r_lp_ofs[kMaxLevels];
   for (int l = 0; l < kMaxLevels; ++l) {
     rec_lf_ofs[l] = recs.size();
     recs.insert(recs.end(), rlf[l].begin(), rlf[l].end());
     rec_lc_ofs[l] = recs.size();
     recs.insert(recs.end(), rlc[l].begin(), rlc[l].end());
+    rec_lp_ofs[l] = recs.size();
+    recs.insert(recs.end(), rlp[l].begin(), rlp[l].end());
     b2r_lf_ofs[l] = b2r.size();
     b2r.insert(b2r.end(), blf[l].begin(), blf[l].end());
     b2r_lc_ofs[l] = b2r.size();
     b2r.insert(b2r.end(), blc[l].begin(), blc[l].end());
+    b2r_lp_ofs[l] = b2r.size();
+    b2r.insert(b2r.end(), blp[l].begin(), blp[l].end());
   }
   if (ns > 0) {
     APH_HIP(hipMemcpy(d_sds_, sds.data(), sizeof(SysDev) * ns, hipMemcpyHostToDevice));
@@ -871,6 +926,12 @@ int Engine::build_tables() {
     L_.b2r_lc[l] = d_b2r_ + b2r_lc_ofs[l];
     L_.nlf[l] = (int)blf[l].size();
     L_.nlc[l] = (int)blc[l].size();
+    L_.post_fused[l] = post_fused[l];
+    L_.post_plain[l] = post_plain[l];
+    L_.vx_p[l] = fp32_ ? 4 : 2;
+    L_.rec_lp[l] = d_rec_ + rec_lp_ofs[l];
+    L_.b2r_lp[l] = d_b2r_ + b2r_lp_ofs[l];
+    L_.nlp[l] = (int)blp[l].size();
   }
   dirty_ = false;
   return APHPOIS_OK;

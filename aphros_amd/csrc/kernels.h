@@ -34,7 +34,19 @@ struct LaunchSet {
   const Rec* rec_lc[kMaxLevels] = {};
   const int* b2r_lc[kMaxLevels] = {};
   int nlc[kMaxLevels] = {};
+  // post-smoothing of level l: some systems fused (k_mg_post over the
+  // whole-row tiles rec_lp, cells per thread vx_p), some with two sweeps
+  // (the sweep launches skip the systems whose level has post = 1)
+  int post_fused[kMaxLevels] = {};
+  int post_plain[kMaxLevels] = {};
+  int vx_p[kMaxLevels] = {};
+  const Rec* rec_lp[kMaxLevels] = {};
+  const int* b2r_lp[kMaxLevels] = {};
+  int nlp[kMaxLevels] = {};
 };
+
+// threads of a k_mg_post workgroup
+constexpr int kPostBlock = 1024;
 
 // kernels of one iteration, each reducing kernel followed by its finalize
 // kernel: PCG: spmv, fin, update, fin, ru_accum, fin, ru_resid, fin and, with

@@ -697,6 +697,11 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
   // so: the start of newly prepared systems (first == 1); running ones skip
   if (so && !s.sc->first) return;
   const MgLevel& Lv = s.lev[l];
+  // the post-smoothing of this system's level runs fused (k_mg_post)
+  if ((MODE == 0 || MODE == 2) && Lv.post) return;
+  // zin / zout < 0: the level's pre-smoothed iterate (z[0] if fused, else z[1])
+  if (zin < 0) zin = Lv.post ? 0 : 1;
+  if (zout < 0) zout = Lv.post ? 0 : 1;
   const int nx = Lv.nx, ny = Lv.ny, nz = Lv.nz;
   const int lb = b - rc.blk0;
   // tile of the level grid
@@ -931,7 +936,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_restrict(const SysDev* __restrict
   const Cell c = cell_of(rc, lb, Lc.nx, Lc.ny, Lc.nz);
   if (!c.active) return;
   const T* __restrict__ r = (const T*)Lf.r;
-  const T* __restrict__ zi = (const T*)Lf.z[zin];
+  const T* __restrict__ zi = (const T*)Lf.z[zin >= 0 ? zin : (Lf.post ? 0 : 1)];
   T* __restrict__ rcoarse = (T*)Lc.r;
   auto u = [&](long long i) -> T { return zi[i]; };
   for (int z = c.za; z < c.zb; ++z) {
@@ -943,6 +948,182 @@ __global__ __launch_bounds__(kBlock) void k_mg_restrict(const SysDev* __restrict
           sum += r[F.at(fx, fy, fz)] - F.lap(fx, fy, fz, u);
         }
     rcoarse[((long long)z * Lc.ny + c.y) * Lc.nx + c.x] = sum;
+  }
+}
+
+// Both post-smoothing sweeps of a level in one z march:
+//   u = z[0] + P z_c;  v = u + wB D^-1 (r - L u);  z[1] = v + wA D^-1 (r - L v)
+// (+ zs with addzs, the first correction of a W level's second correction;
+// DOT: partials of (r, z[1])).  The pre-smoothing of such a level (post = 1)
+// writes z[0], so z[1] ends with the correction, as after the two separate
+// sweeps.
+// A workgroup covers whole rows (rc.lx = nx / VX threads per row) and rc.bz
+// planes.  Its rc.ty rows of threads are the output rows 1 .. ty-2 and one
+// halo row on each side: v is computed on all ty rows (the halo rows' v is
+// recomputed rather than exchanged between workgroups), z[1] on the output
+// rows two planes behind v.  Step k of the march forms z[1] on plane k-2 and
+// then v on plane k: both sets of loads are independent of the step's own
+// results, so one barrier per step suffices (u planes double-buffered in LDS
+// for the v sweep, v planes triple-buffered for the z[1] sweep).  The u rows
+// just outside the tile (for the halo rows' v) and the u planes before and
+// after the tile come from memory.
+template <typename T, int VX, bool DOT, bool L0>
+__global__ __launch_bounds__(kPostBlock) void k_mg_post(const SysDev* __restrict__ sds,
+                                                        const Rec* __restrict__ recs,
+                                                        const int* __restrict__ b2r, int l_arg,
+                                                        int wl, int so, int addzs) {
+  const int l = L0 ? 0 : l_arg;
+  using V = vec_t<T, VX>;
+  __shared__ T su[2][kPostBlock * VX];
+  __shared__ T sv[3][kPostBlock * VX];
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const Rec rc = recs[b2r[b]];
+  const SysDev& s = sds[rc.sys];
+  if (s.sc->done) return;
+  if (wl >= 0 && wl >= s.mg_w) return;
+  if (so && !s.sc->first) return;
+  const MgLevel& Lv = s.lev[l];
+  const MgLevel& Lc = s.lev[l + 1];
+  const int nx = Lv.nx, ny = Lv.ny, nz = Lv.nz;
+  const int lb = b - rc.blk0;
+  const int lx = rc.lx, ty = rc.ty;  // lx * VX == nx, ty <= kPostBlock / lx
+  const int tiy = lb % rc.tiles_y, tiz = lb / rc.tiles_y;
+  const int ix = threadIdx.x % lx, iy = threadIdx.x / lx;
+  const bool active = iy < ty;
+  // grid row of this thread row: output rows y0 .. y0+ty-3, halo rows around
+  // (yr in [-1, 2 ny - 1] because ty <= ny + 2)
+  const int y0 = tiy * (ty - 2);
+  const int yr = y0 - 1 + iy;
+  const int y = yr < 0 ? yr + ny : (yr >= ny ? yr - ny : yr);
+  const bool outrow = active && iy >= 1 && iy <= ty - 2 && yr < ny;
+  const int za = tiz * rc.bz, zb = min(za + rc.bz, nz);
+  const int x0 = ix * VX;
+  const int lofs = iy * nx + x0;
+  const int lxm = iy * nx + (ix == 0 ? nx - 1 : x0 - 1);
+  const int lxp = iy * nx + (ix == lx - 1 ? 0 : x0 + VX);
+  const int xm = x0 == 0 ? nx - 1 : x0 - 1;
+  const int ym = wrapm(y, ny), yp = wrapp(y, ny);
+  const MgCoefs<T> co(Lv);
+  const T* __restrict__ mm = (const T*)Lv.m;
+  const T* __restrict__ r = (const T*)Lv.r;
+  const T* __restrict__ zi = (const T*)Lv.z[0];
+  T* __restrict__ zo = (T*)Lv.z[1];
+  const T* __restrict__ zc = (const T*)Lc.z[l + 1 == s.nlev - 1 ? 0 : 1];
+  const T* __restrict__ zsa = addzs ? (const T*)Lv.zs : nullptr;
+  const int cnx = Lc.nx, cny = Lc.ny;
+  const T wa = (T)kOmegaA, wb = (T)kOmegaB;
+  const long long pl = (long long)nx * ny;
+  const long long row = (long long)y * nx;
+  auto zw = [&](int k) { return k < 0 ? k + nz : (k >= nz ? k - nz : k); };
+  // u = z[0] + P z_c at cells x0 .. x0+VX-1 of row yy, plane zz
+  auto uv = [&](int yy, int zz) -> V {
+    V u = ldv<T, VX>(zi, (long long)zz * pl + (long long)yy * nx + x0);
+    const long long cb = ((long long)(zz >> 1) * cny + (yy >> 1)) * cnx + (x0 >> 1);
+    if constexpr (VX >= 2) {
+      const vec_t<T, VX / 2> c = ldv<T, VX / 2>(zc, cb);
+#pragma unroll
+      for (int v = 0; v < VX; ++v) u[v] += c[v / 2];
+    } else {
+      u[0] += zc[cb];
+    }
+    return u;
+  };
+  // sum_f k_f (w_c - w_nb) + m w_c over the x, y faces (neighbours in the LDS
+  // plane sp, or from `far` for the rows outside the tile) and the z faces
+  auto lap = [&](const T* sp, long long base, V w, V wzm, V wzp, V kxc, V kyc, V kzc, V kzm,
+                 auto far) -> V {
+    V l7 = kzc * (w - wzp) + kzm * (w - wzm);
+    V wy;
+    if (iy > 0) {
+#pragma unroll
+      for (int v = 0; v < VX; ++v) wy[v] = sp[lofs - nx + v];
+    } else {
+      wy = far(ym);
+    }
+    l7 += co.template comp<VX>(1, base + (long long)ym * nx + x0) * (w - wy);
+    if (iy + 1 < ty) {
+#pragma unroll
+      for (int v = 0; v < VX; ++v) wy[v] = sp[lofs + nx + v];
+    } else {
+      wy = far(yp);
+    }
+    l7 += kyc * (w - wy);
+    const T wxm = sp[lxm], wxp = sp[lxp];
+    const T kxm = co.comp1(0, base + row + xm);
+#pragma unroll
+    for (int v = 0; v < VX; ++v) {
+      const T up = v + 1 < VX ? w[(v + 1) % VX] : wxp;
+      const T uw = v > 0 ? w[(v + VX - 1) % VX] : wxm;
+      const T kw = v > 0 ? kxc[(v + VX - 1) % VX] : kxm;
+      l7[v] += kxc[v] * (w[v] - up) + kw * (w[v] - uw);
+    }
+    if (mm) l7 += ldv<T, VX>(mm, base + row + x0) * w;
+    return l7;
+  };
+  // v sweep: u at planes k-1, k and kz at plane k-1; z[1] sweep: v at planes
+  // k-3, k-2, k-1 (own cells; the plane neighbours of plane k-2 from LDS)
+  V um = (T)0, uc = (T)0, kzB = (T)0;
+  V v3 = (T)0, v2 = (T)0, v1 = (T)0;
+  if (active) {
+    um = uv(y, zw(za - 2));
+    uc = uv(y, zw(za - 1));
+    kzB = co.template comp<VX>(2, (long long)zw(za - 2) * pl + row + x0);
+#pragma unroll
+    for (int v = 0; v < VX; ++v) su[0][lofs + v] = uc[v];
+  }
+  __syncthreads();
+  double dot = 0.0;
+  for (int k = za - 1; k <= zb + 1; ++k) {
+    const int st = k - (za - 1);
+    // z[1] on plane k - 2 (output planes za .. zb-1) from v of planes k-3,
+    // k-2, k-1, all formed in earlier steps
+    if (k - 2 >= za && outrow) {
+      const int kp = zw(k - 2);
+      const long long base = (long long)kp * pl;
+      const long long ic = base + row + x0;
+      V kx1, ky1, kz1, d1;
+      co.template own<VX>(ic, kx1, ky1, kz1, d1);
+      const V kzm = co.template comp<VX>(2, (long long)zw(k - 3) * pl + row + x0);
+      const V r1 = ldv<T, VX>(r, ic);
+      auto none = [&](int) -> V { return (T)0; };  // output rows have both y neighbours
+      const V lv = lap(sv[(st + 1) % 3], base, v2, v3, v1, kx1, ky1, kz1, kzm, none);
+      V out = v2 + wa * d1 * (r1 - lv);
+      if (zsa) out += ldv<T, VX>(zsa, ic);
+      *reinterpret_cast<V*>(zo + ic) = out;
+      if constexpr (DOT) {
+#pragma unroll
+        for (int v = 0; v < VX; ++v) dot += (double)r1[v] * (double)out[v];
+      }
+    }
+    // v on plane k (planes za-1 .. zb)
+    if (k <= zb && active) {
+      const int kk = zw(k);
+      const V un = uv(y, zw(k + 1));
+      if (k + 1 <= zb) {
+#pragma unroll
+        for (int v = 0; v < VX; ++v) su[(st + 1) & 1][lofs + v] = un[v];
+      }
+      const long long base = (long long)kk * pl;
+      const long long ic = base + row + x0;
+      V kxc, kyc, kzc, dc;
+      co.template own<VX>(ic, kxc, kyc, kzc, dc);
+      auto far = [&](int yy) -> V { return uv(yy, kk); };
+      const V lu = lap(su[st & 1], base, uc, um, un, kxc, kyc, kzc, kzB, far);
+      const V vn = uc + wb * dc * (ldv<T, VX>(r, ic) - lu);
+#pragma unroll
+      for (int v = 0; v < VX; ++v) sv[st % 3][lofs + v] = vn[v];
+      kzB = kzc;
+      um = uc;
+      uc = un;
+      v3 = v2;
+      v2 = v1;
+      v1 = vn;
+    }
+    __syncthreads();
+  }
+  if constexpr (DOT) {
+    double acc[1] = {dot};
+    block_partial<1, 0u, kPostBlock>(acc, s.partials, rc.nblk, lb);
   }
 }
 
@@ -1762,6 +1943,30 @@ static void launch_sweep_l(const LaunchSet& L, hipStream_t st, int l, int zin, i
   }
 }
 
+// both post-smoothing sweeps of level l, fused, for the systems in rec_lp[l]
+template <typename T, bool DOT, bool L0>
+static void launch_post_l(const LaunchSet& L, hipStream_t st, int l, int wl, int addzs) {
+  const dim3 g(L.nlp[l]), t(kPostBlock);
+  // FP32 engines use 4 cells per thread, FP64 engines 2 (the LDS planes of a
+  // 1024-thread workgroup: 5 x 1024 x VX x sizeof(T) bytes)
+  switch (L.vx_p[l]) {
+    case 4:
+      if constexpr (sizeof(T) == 4) {
+        hipLaunchKernelGGL((k_mg_post<T, 4, DOT, L0>), g, t, 0, st, L.sds, L.rec_lp[l],
+                           L.b2r_lp[l], l, wl, L.start_only, addzs);
+      }
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_mg_post<T, 2, DOT, L0>), g, t, 0, st, L.sds, L.rec_lp[l], L.b2r_lp[l],
+                         l, wl, L.start_only, addzs);
+      break;
+    default:
+      hipLaunchKernelGGL((k_mg_post<T, 1, DOT, L0>), g, t, 0, st, L.sds, L.rec_lp[l], L.b2r_lp[l],
+                         l, wl, L.start_only, addzs);
+      break;
+  }
+}
+
 template <typename T, int MODE, bool DOT>
 static void launch_sweep(const LaunchSet& L, hipStream_t st, int l, int zin, int zout, int wl,
                          int addzs = 0) {
@@ -1785,7 +1990,7 @@ static void launch_sweep(const LaunchSet& L, hipStream_t st, int l, int zin, int
 // `mark(kind)` precedes every launch (profiling): 0 = level-0 pre-smoothing
 // sweep, 1 = level-0 restriction, 2 = coarser-level launch, 3 = bottom
 // levels (k_mg_bottom), 5 and 6 = level-0 first and second post-smoothing
-// sweeps.
+// sweeps (5 alone: both, fused).
 template <typename T, typename Mark>
 static void launch_cycle(const LaunchSet& L, hipStream_t st, int l, int wl, bool second,
                          Mark& mark) {
@@ -1795,21 +2000,31 @@ static void launch_cycle(const LaunchSet& L, hipStream_t st, int l, int wl, bool
                        L.sds, L.coarse_sweeps, l, wl, L.start_only);
   }
   if (l >= L.deep) return;  // every system's cycle ends at or above level l
+  // the pre-smoothed iterate (zout / zin -1): z[0] of a system whose level
+  // fuses the post-smoothing (which then writes z[1]), else z[1] (the first
+  // post-sweep writes z[0], the second z[1])
   mark(l == 0 ? 0 : 2);
-  launch_sweep<T, 1, false>(L, st, l, 0, 1, wl);
+  launch_sweep<T, 1, false>(L, st, l, 0, -1, wl);
   mark(l == 0 ? 1 : 2);
-  launch_sweep<T, 3, false>(L, st, l, 1, 1, wl);
+  launch_sweep<T, 3, false>(L, st, l, -1, -1, wl);
   launch_cycle<T>(L, st, l + 1, wl, false, mark);
   if (l < L.mg_w) {
     mark(2);
     launch_sweep<T, 4, false>(L, st, l + 1, 1, 1, l);
     launch_cycle<T>(L, st, l + 1, l, true, mark);
   }
-  mark(l == 0 ? 5 : 2);
-  launch_sweep<T, 2, false>(L, st, l, 1, 0, wl);
-  mark(l == 0 ? 6 : 2);
-  if (l == 0) launch_sweep<T, 0, true>(L, st, l, 0, 1, wl);
-  else launch_sweep<T, 0, false>(L, st, l, 0, 1, wl, second ? 1 : 0);
+  if (L.post_fused[l]) {
+    mark(l == 0 ? 5 : 2);
+    if (l == 0) launch_post_l<T, true, true>(L, st, l, wl, 0);
+    else launch_post_l<T, false, false>(L, st, l, wl, second ? 1 : 0);
+  }
+  if (L.post_plain[l]) {
+    mark(l == 0 ? 5 : 2);
+    launch_sweep<T, 2, false>(L, st, l, 1, 0, wl);
+    mark(l == 0 ? 6 : 2);
+    if (l == 0) launch_sweep<T, 0, true>(L, st, l, 0, 1, wl);
+    else launch_sweep<T, 0, false>(L, st, l, 0, 1, wl, second ? 1 : 0);
+  }
 }
 
 // The preconditioner z = M r on level 0 and the (r, z) finalize; `ev`
@@ -1843,7 +2058,7 @@ hipError_t launch_vcycle_profiled(int fp32, const LaunchSet& L, hipStream_t st, 
 static int cycle_launches(const LaunchSet& L, int l) {
   int n = L.bottom_at[l] ? 1 : 0;
   if (l >= L.deep) return n;
-  n += 4 + cycle_launches(L, l + 1);
+  n += 2 + (L.post_fused[l] ? 1 : 0) + (L.post_plain[l] ? 2 : 0) + cycle_launches(L, l + 1);
   if (l < L.mg_w) n += 1 + cycle_launches(L, l + 1);
   return n;
 }

@@ -27,10 +27,10 @@ __device__ __forceinline__ void wave_reduce(double (&v)[NV]) {
   }
 }
 
-// Block-wide reduction; the result is valid in every thread.
-template <int NV, unsigned MaxMask>
+// Block-wide reduction over NT threads; the result is valid in every thread.
+template <int NV, unsigned MaxMask, int NT = kBlock>
 __device__ __forceinline__ void block_reduce(double (&v)[NV]) {
-  __shared__ double red[NV][kBlock / 64];
+  __shared__ double red[NV][NT / 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   wave_reduce<NV, MaxMask>(v);
   if (lane == 0) {
@@ -41,7 +41,7 @@ __device__ __forceinline__ void block_reduce(double (&v)[NV]) {
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
     double t = red[k][0];
-    for (int w = 1; w < kBlock / 64; ++w) {
+    for (int w = 1; w < NT / 64; ++w) {
       t = ((MaxMask >> k) & 1u) ? fmax(t, red[k][w]) : t + red[k][w];
     }
     v[k] = t;
@@ -50,10 +50,10 @@ __device__ __forceinline__ void block_reduce(double (&v)[NV]) {
 }
 
 // Writes this workgroup's partial values.  Must be reached by every thread.
-template <int NV, unsigned MaxMask>
+template <int NV, unsigned MaxMask, int NT = kBlock>
 __device__ __forceinline__ void block_partial(double (&v)[NV], double* partials, int nblk,
                                               int lb) {
-  block_reduce<NV, MaxMask>(v);
+  block_reduce<NV, MaxMask, NT>(v);
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) partials[(long long)k * nblk + lb] = v[k];

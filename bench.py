@@ -48,6 +48,10 @@ KERNELS = {
     # read r, z, kx, ky, kz; write r_coarse (1/8)
     "mg_restrict_l0": {"slot": 10, "vec": 2.125, "coef": 3.0, "m": 1,
                        "name": "k_mg_sweep<T, VX, 3, false, true>"},
+    # both post-sweeps fused (slot 15 then stays empty): read z_in, z_coarse
+    # (1/8), r, kx, ky, kz, dinv; write z; (r, z) partials
+    "mg_post_l0": {"slot": 14, "vec": 3.125, "coef": 4.0, "m": 1,
+                   "name": "k_mg_post<T, VX, true, true>"},
 }
 
 
@@ -66,8 +70,11 @@ def kernel_table(kt, ncell, elem, coef, has_m, per_iter_launches, jacobi=False):
     table = dict(KERNELS)
     if jacobi:
         table = {"spmv": SPMV_JACOBI, "update": KERNELS["update"]}
+    fused = len(kt) > 15 and kt[15][1] == 0
     for key, k in table.items():
         if k["slot"] >= len(kt):
+            continue
+        if key == ("mg_postsweep1_l0" if fused else "mg_post_l0"):
             continue
         ms, cnt = kt[k["slot"]]
         if cnt == 0 or ms <= 0:
@@ -309,7 +316,8 @@ def main():
 
     roofline, kernels = None, None
     if kt:
-        per_iter = {"mg_presweep_l0": 1, "mg_postsweep1_l0": 1, "mg_postsweep2_l0": 1}
+        per_iter = {"mg_presweep_l0": 1, "mg_postsweep1_l0": 1, "mg_postsweep2_l0": 1,
+                    "mg_post_l0": 1}
         elem = 4 if main_prec == L.MIXED else 8
         coef = 2 if precond == L.PRECOND_MG and main_prec == L.MIXED else elem
         kernels = kernel_table(kt, ncell, elem, coef, has_m, per_iter,

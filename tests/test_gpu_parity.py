@@ -148,6 +148,48 @@ def test_mg_vcycle_matches_oracle(c1, tlin32, precision, tol, case, w):
     assert info["iter"] == info2["iter"] and np.array_equal(x, x2)
 
 
+@pytest.fixture(scope="module")
+def c2():
+    e, rho, f = synth.config("c2_256")
+    return e.reshape(256, 256, 256, 8)
+
+
+@pytest.mark.parametrize("precision,tol", [(L.FP64, 1e-12), (L.MIXED, 1e-4)])
+@pytest.mark.parametrize("w", [0, 3])
+def test_mg_vcycle_large_fused_post(c2, precision, tol, w):
+    """256^3 (2^24 cells): the level-0 post-smoothing runs as one fused kernel
+    (whole-row tiles, halo rows and planes recomputed, z[0] -> z[1]); one
+    cycle against the oracle's, as for the small systems."""
+    _say(f"vcycle c2_256 precision={precision} w={w}")
+    s = Poisson(c2.shape[:3], method=L.PCG, precision=precision, resmode=L.RES_REL_TRUE,
+                tol=1e-8, maxiter=20000, precond=L.PRECOND_MG, mg_w=w)
+    s.set_system(c2)
+    rng = np.random.default_rng(13)
+    r = rng.standard_normal(c2.shape[:3])
+    r -= r.mean()
+    z = s.precond_apply(r)
+    zo, nlev = O.mg_apply(c2, r, bf16=precision == L.MIXED, w=w)
+    err = np.abs(z - zo).max() / np.abs(zo).max()
+    _say(f"  nlev={nlev} rel_err={err:.3e}")
+    assert err <= tol
+
+
+def test_pcg_large_fused_post_true_residual(c2):
+    """The fused level-0 post-smoothing also forms (r, z) for beta: a mixed
+    multigrid PCG solve at 256^3 reaches the FP64 true residual (recomputed on
+    the CPU) within the oracle's FP64 multigrid PCG iteration count + 2."""
+    s = Poisson(c2.shape[:3], method=L.PCG, precision=L.MIXED, resmode=L.RES_REL_TRUE,
+                tol=1e-8, maxiter=20000, precond=L.PRECOND_MG)
+    s.set_system(c2)
+    x, info = s.solve(None)
+    assert info["converged"] == 1 and info["status"] == 0
+    rel = O.true_relres(c2, x)
+    _, ito, _, _ = O.mgpcg(c2, tol=1e-8, w=L.MG_W_DEFAULT)
+    _say(f"  iter={info['iter']} oracle_iter={ito} relres={rel:.3e}")
+    assert rel <= 1e-8 * (1 + 1e-6)
+    assert info["iter"] <= ito + 2
+
+
 @pytest.mark.parametrize("w", [0, 2])
 def test_mg_fp64_iterations_match_oracle(c1, w):
     """FP64 multigrid PCG on the GPU and the oracle's FP64 multigrid PCG (same
