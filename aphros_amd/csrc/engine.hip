@@ -43,11 +43,13 @@ Rec make_stencil_rec(int nx, int ny, int nz, int vx, int blocks_target, const Tu
   return r;
 }
 
-// Fused post-smoothing (k_mg_post) of a level: whole-row tiles of lx = nx / vx
-// threads per row and ty = min(kPostBlock / lx, ny + 2) rows of threads
-// (ty - 2 output rows), bz planes.  Only for the large levels (n >= 2^24
-// cells: the march is long enough to hide the halo rows and planes it
-// recomputes) whose rows fit 1/8 of the workgroup (ty >= 8 rows).
+// Fused pre-smoothing + restriction (k_mg_pre) and post-smoothing (k_mg_post)
+// of a level: whole-row tiles of lx = nx / vx threads per row and
+// ty = min(kPostBlock / lx, ny + 2) rows of threads (even; ty - 2 output
+// rows), bz planes (even).  Only for the large levels (n >= 2^24 cells: the
+// march is long enough to hide the halo rows and planes it recomputes; such a
+// level coarsens, so its edges are even) whose rows fit 1/8 of the workgroup
+// (ty >= 8 rows).
 constexpr long long kFuseMinCells = 1LL << 24;
 
 bool make_post_rec(int nx, int ny, int nz, int vx, Rec* out) {
@@ -57,13 +59,16 @@ bool make_post_rec(int nx, int ny, int nz, int vx, Rec* out) {
   if (lx > kPostBlock / 8) return false;
   Rec r{};
   r.lx = lx;
-  r.ty = std::min(kPostBlock / lx, ny + 2);
+  // even: the fused restriction pairs output rows from an even row
+  r.ty = std::min(kPostBlock / lx, ny + 2) & ~1;
   r.tiles_x = 1;
   r.tiles_y = (int)cdiv(ny, r.ty - 2);
   // about 1024 workgroups (two resident per CU), marches of >= 8 planes
   const long long target = std::min(1024LL, std::max(1LL, n / 16384));
   const long long tz = std::max(1LL, target / r.tiles_y);
   r.bz = (int)std::min((long long)nz, std::max((long long)std::min(8, nz), cdiv(nz, tz)));
+  r.bz += r.bz & 1;  // plane pairs of the restriction (nz is even on these levels)
+  r.bz = std::min(r.bz, nz);
   r.tiles_z = (int)cdiv(nz, r.bz);
   r.nblk = r.tiles_y * r.tiles_z;
   *out = r;

@@ -697,8 +697,8 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
   // so: the start of newly prepared systems (first == 1); running ones skip
   if (so && !s.sc->first) return;
   const MgLevel& Lv = s.lev[l];
-  // the post-smoothing of this system's level runs fused (k_mg_post)
-  if ((MODE == 0 || MODE == 2) && Lv.post) return;
+  // this system's level runs fused kernels (k_mg_pre, k_mg_post) instead
+  if ((MODE == 0 || MODE == 1 || MODE == 2 || MODE == 3) && Lv.post) return;
   // zin / zout < 0: the level's pre-smoothed iterate (z[0] if fused, else z[1])
   if (zin < 0) zin = Lv.post ? 0 : 1;
   if (zout < 0) zout = Lv.post ? 0 : 1;
@@ -930,6 +930,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_restrict(const SysDev* __restrict
   if (wl >= 0 && wl >= s.mg_w) return;
   if (so && !s.sc->first) return;
   const MgLevel& Lf = s.lev[l];
+  if (Lf.post) return;  // restricted by k_mg_pre
   const MgLevel& Lc = s.lev[l + 1];
   const LevelView<T> F(Lf);
   const int lb = b - rc.blk0;
@@ -1124,6 +1125,194 @@ __global__ __launch_bounds__(kPostBlock) void k_mg_post(const SysDev* __restrict
   if constexpr (DOT) {
     double acc[1] = {dot};
     block_partial<1, 0u, kPostBlock>(acc, s.partials, rc.nblk, lb);
+  }
+}
+
+// Pre-smoothing and restriction of a level in one z march (the levels with
+// post = 1, on the tiles of k_mg_post: an even number of output rows from an
+// even row, an even number of planes from an even plane):
+//   u = wA D^-1 r;  z = u + wB D^-1 (r - L u) -> z[0] (output rows and planes);
+//   r_{l+1}[C] = sum over the 2x2x2 children c of (r - L z)_c.
+// Step k forms the residual on plane k-2 (z of planes k-3 .. k-1, from earlier
+// steps) and then z on plane k (halo rows and the planes za-1, zb included).
+// x pairs are summed in the thread, plane pairs across two steps, row pairs
+// through LDS after the next barrier (odd rows hand their sums to the even row
+// above).  One barrier per step: u planes double-buffered, z planes
+// triple-buffered in LDS.
+template <typename T, int VX, bool L0>
+__global__ __launch_bounds__(kPostBlock) void k_mg_pre(const SysDev* __restrict__ sds,
+                                                       const Rec* __restrict__ recs,
+                                                       const int* __restrict__ b2r, int l_arg,
+                                                       int wl, int so) {
+  static_assert(VX >= 2, "x pairs of the restriction are summed in the thread");
+  constexpr int NP = VX / 2;
+  const int l = L0 ? 0 : l_arg;
+  using V = vec_t<T, VX>;
+  __shared__ T su[2][kPostBlock * VX];
+  __shared__ T sz[3][kPostBlock * VX];
+  __shared__ T sacc[kPostBlock * NP];
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const Rec rc = recs[b2r[b]];
+  const SysDev& s = sds[rc.sys];
+  if (s.sc->done) return;
+  if (wl >= 0 && wl >= s.mg_w) return;
+  if (so && !s.sc->first) return;
+  const MgLevel& Lv = s.lev[l];
+  const MgLevel& Lc = s.lev[l + 1];
+  const int nx = Lv.nx, ny = Lv.ny, nz = Lv.nz;
+  const int lb = b - rc.blk0;
+  const int lx = rc.lx, ty = rc.ty;  // lx * VX == nx, ty even, <= kPostBlock / lx
+  const int tiy = lb % rc.tiles_y, tiz = lb / rc.tiles_y;
+  const int ix = threadIdx.x % lx, iy = threadIdx.x / lx;
+  const bool active = iy < ty;
+  const int y0 = tiy * (ty - 2);  // even
+  const int yr = y0 - 1 + iy;
+  const int y = yr < 0 ? yr + ny : (yr >= ny ? yr - ny : yr);
+  const bool outrow = active && iy >= 1 && iy <= ty - 2 && yr < ny;
+  const int za = tiz * rc.bz, zb = min(za + rc.bz, nz);  // za, zb even
+  const int x0 = ix * VX;
+  const int lofs = iy * nx + x0;
+  const int lxm = iy * nx + (ix == 0 ? nx - 1 : x0 - 1);
+  const int lxp = iy * nx + (ix == lx - 1 ? 0 : x0 + VX);
+  const int xm = x0 == 0 ? nx - 1 : x0 - 1;
+  const int ym = wrapm(y, ny), yp = wrapp(y, ny);
+  const MgCoefs<T> co(Lv);
+  const T* __restrict__ mm = (const T*)Lv.m;
+  const T* __restrict__ r = (const T*)Lv.r;
+  T* __restrict__ zo = (T*)Lv.z[0];
+  T* __restrict__ rco = (T*)Lc.r;
+  const int cnx = Lc.nx, cny = Lc.ny;
+  const T wa = (T)kOmegaA, wb = (T)kOmegaB;
+  const long long pl = (long long)nx * ny;
+  const long long row = (long long)y * nx;
+  auto zw = [&](int k) { return k < 0 ? k + nz : (k >= nz ? k - nz : k); };
+  // u = wA D^-1 r (the zero-guess sweep) at row yy, plane zz
+  auto uv = [&](int yy, int zz) -> V {
+    const long long i = (long long)zz * pl + (long long)yy * nx + x0;
+    return wa * co.template comp<VX>(3, i) * ldv<T, VX>(r, i);
+  };
+  auto lap = [&](const T* sp, long long base, V w, V wzm, V wzp, V kxc, V kyc, V kzc, V kzm,
+                 auto far) -> V {
+    V l7 = kzc * (w - wzp) + kzm * (w - wzm);
+    V wy;
+    if (iy > 0) {
+#pragma unroll
+      for (int v = 0; v < VX; ++v) wy[v] = sp[lofs - nx + v];
+    } else {
+      wy = far(ym);
+    }
+    l7 += co.template comp<VX>(1, base + (long long)ym * nx + x0) * (w - wy);
+    if (iy + 1 < ty) {
+#pragma unroll
+      for (int v = 0; v < VX; ++v) wy[v] = sp[lofs + nx + v];
+    } else {
+      wy = far(yp);
+    }
+    l7 += kyc * (w - wy);
+    const T wxm = sp[lxm], wxp = sp[lxp];
+    const T kxm = co.comp1(0, base + row + xm);
+#pragma unroll
+    for (int v = 0; v < VX; ++v) {
+      const T up = v + 1 < VX ? w[(v + 1) % VX] : wxp;
+      const T uw = v > 0 ? w[(v + VX - 1) % VX] : wxm;
+      const T kw = v > 0 ? kxc[(v + VX - 1) % VX] : kxm;
+      l7[v] += kxc[v] * (w[v] - up) + kw * (w[v] - uw);
+    }
+    if (mm) l7 += ldv<T, VX>(mm, base + row + x0) * w;
+    return l7;
+  };
+  // z sweep: u at planes k-1, k, r at plane k, kz at plane k-1; residual: z at
+  // planes k-3, k-2, k-1 (own cells); the x-pair sums of the current plane
+  // pair; even rows keep their sums of the last pair for the row pair
+  V um = (T)0, uc = (T)0, rcur = (T)0, kzB = (T)0;
+  V z3 = (T)0, z2 = (T)0, z1 = (T)0;
+  T racc[NP], rpend[NP];
+#pragma unroll
+  for (int j = 0; j < NP; ++j) racc[j] = rpend[j] = (T)0;
+  int kpend = -1;  // coarse plane of rpend (even rows), -1: none
+  if (active) {
+    um = uv(y, zw(za - 2));
+    const long long i0 = (long long)zw(za - 1) * pl + row + x0;
+    rcur = ldv<T, VX>(r, i0);
+    uc = wa * co.template comp<VX>(3, i0) * rcur;
+    kzB = co.template comp<VX>(2, (long long)zw(za - 2) * pl + row + x0);
+#pragma unroll
+    for (int v = 0; v < VX; ++v) su[0][lofs + v] = uc[v];
+  }
+  __syncthreads();
+  for (int k = za - 1; k <= zb + 1; ++k) {
+    const int st = k - (za - 1);
+    // row pairs of the plane pair completed in the previous step
+    if (kpend >= 0) {
+#pragma unroll
+      for (int j = 0; j < NP; ++j)
+        rco[((long long)kpend * cny + (y >> 1)) * cnx + (x0 >> 1) + j] =
+            rpend[j] + sacc[(threadIdx.x + lx) * NP + j];
+      kpend = -1;
+    }
+    // residual on plane k - 2 (planes za .. zb-1)
+    if (k - 2 >= za && outrow) {
+      const int kp = k - 2;  // in [za, zb), no wrap
+      const long long base = (long long)kp * pl;
+      const long long ic = base + row + x0;
+      V kx1, ky1, kz1, d1;
+      co.template own<VX>(ic, kx1, ky1, kz1, d1);
+      const V kzm = co.template comp<VX>(2, (long long)zw(kp - 1) * pl + row + x0);
+      auto none = [&](int) -> V { return (T)0; };
+      const V lz = lap(sz[(st + 1) % 3], base, z2, z3, z1, kx1, ky1, kz1, kzm, none);
+      const V res = ldv<T, VX>(r, ic) - lz;
+      if ((kp & 1) == 0) {
+#pragma unroll
+        for (int j = 0; j < NP; ++j) racc[j] = res[2 * j] + res[2 * j + 1];
+      } else {
+#pragma unroll
+        for (int j = 0; j < NP; ++j) racc[j] += res[2 * j] + res[2 * j + 1];
+        if (y & 1) {
+#pragma unroll
+          for (int j = 0; j < NP; ++j) sacc[threadIdx.x * NP + j] = racc[j];
+        } else {
+#pragma unroll
+          for (int j = 0; j < NP; ++j) rpend[j] = racc[j];
+          kpend = kp >> 1;
+        }
+      }
+    }
+    // z on plane k (planes za-1 .. zb)
+    if (k <= zb && active) {
+      const int kk = zw(k);
+      const long long in = (long long)zw(k + 1) * pl + row + x0;
+      const V rn = ldv<T, VX>(r, in);
+      const V un = wa * co.template comp<VX>(3, in) * rn;
+      if (k + 1 <= zb) {
+#pragma unroll
+        for (int v = 0; v < VX; ++v) su[(st + 1) & 1][lofs + v] = un[v];
+      }
+      const long long base = (long long)kk * pl;
+      const long long ic = base + row + x0;
+      V kxc, kyc, kzc, dc;
+      co.template own<VX>(ic, kxc, kyc, kzc, dc);
+      auto far = [&](int yy) -> V { return uv(yy, kk); };
+      const V lu = lap(su[st & 1], base, uc, um, un, kxc, kyc, kzc, kzB, far);
+      const V zn = uc + wb * dc * (rcur - lu);
+#pragma unroll
+      for (int v = 0; v < VX; ++v) sz[st % 3][lofs + v] = zn[v];
+      if (outrow && k >= za && k < zb) *reinterpret_cast<V*>(zo + ic) = zn;
+      kzB = kzc;
+      um = uc;
+      uc = un;
+      rcur = rn;
+      z3 = z2;
+      z2 = z1;
+      z1 = zn;
+    }
+    __syncthreads();
+  }
+  // the last plane pair (planes zb-2, zb-1)
+  if (kpend >= 0) {
+#pragma unroll
+    for (int j = 0; j < NP; ++j)
+      rco[((long long)kpend * cny + (y >> 1)) * cnx + (x0 >> 1) + j] =
+          rpend[j] + sacc[(threadIdx.x + lx) * NP + j];
   }
 }
 
@@ -1943,6 +2132,21 @@ static void launch_sweep_l(const LaunchSet& L, hipStream_t st, int l, int zin, i
   }
 }
 
+// pre-smoothing + restriction of level l, fused, for the systems in rec_lp[l]
+template <typename T, bool L0>
+static void launch_pre_l(const LaunchSet& L, hipStream_t st, int l, int wl) {
+  const dim3 g(L.nlp[l]), t(kPostBlock);
+  if (L.vx_p[l] == 4) {
+    if constexpr (sizeof(T) == 4) {
+      hipLaunchKernelGGL((k_mg_pre<T, 4, L0>), g, t, 0, st, L.sds, L.rec_lp[l], L.b2r_lp[l], l,
+                         wl, L.start_only);
+    }
+  } else {
+    hipLaunchKernelGGL((k_mg_pre<T, 2, L0>), g, t, 0, st, L.sds, L.rec_lp[l], L.b2r_lp[l], l, wl,
+                       L.start_only);
+  }
+}
+
 // both post-smoothing sweeps of level l, fused, for the systems in rec_lp[l]
 template <typename T, bool DOT, bool L0>
 static void launch_post_l(const LaunchSet& L, hipStream_t st, int l, int wl, int addzs) {
@@ -2003,10 +2207,17 @@ static void launch_cycle(const LaunchSet& L, hipStream_t st, int l, int wl, bool
   // the pre-smoothed iterate (zout / zin -1): z[0] of a system whose level
   // fuses the post-smoothing (which then writes z[1]), else z[1] (the first
   // post-sweep writes z[0], the second z[1])
-  mark(l == 0 ? 0 : 2);
-  launch_sweep<T, 1, false>(L, st, l, 0, -1, wl);
-  mark(l == 0 ? 1 : 2);
-  launch_sweep<T, 3, false>(L, st, l, -1, -1, wl);
+  if (L.post_fused[l]) {
+    mark(l == 0 ? 0 : 2);
+    if (l == 0) launch_pre_l<T, true>(L, st, l, wl);
+    else launch_pre_l<T, false>(L, st, l, wl);
+  }
+  if (L.post_plain[l]) {
+    mark(l == 0 ? 0 : 2);
+    launch_sweep<T, 1, false>(L, st, l, 0, -1, wl);
+    mark(l == 0 ? 1 : 2);
+    launch_sweep<T, 3, false>(L, st, l, -1, -1, wl);
+  }
   launch_cycle<T>(L, st, l + 1, wl, false, mark);
   if (l < L.mg_w) {
     mark(2);
@@ -2058,7 +2269,7 @@ hipError_t launch_vcycle_profiled(int fp32, const LaunchSet& L, hipStream_t st, 
 static int cycle_launches(const LaunchSet& L, int l) {
   int n = L.bottom_at[l] ? 1 : 0;
   if (l >= L.deep) return n;
-  n += 2 + (L.post_fused[l] ? 1 : 0) + (L.post_plain[l] ? 2 : 0) + cycle_launches(L, l + 1);
+  n += (L.post_fused[l] ? 2 : 0) + (L.post_plain[l] ? 4 : 0) + cycle_launches(L, l + 1);
   if (l < L.mg_w) n += 1 + cycle_launches(L, l + 1);
   return n;
 }

@@ -52,6 +52,10 @@ KERNELS = {
     # (1/8), r, kx, ky, kz, dinv; write z; (r, z) partials
     "mg_post_l0": {"slot": 14, "vec": 3.125, "coef": 4.0, "m": 1,
                    "name": "k_mg_post<T, VX, true, true>"},
+    # pre-sweep and restriction fused (slot 10 then stays empty): read r, kx,
+    # ky, kz, dinv; write z, r_coarse (1/8)
+    "mg_pre_l0": {"slot": 13, "vec": 2.125, "coef": 4.0, "m": 1,
+                  "name": "k_mg_pre<T, VX, true>"},
 }
 
 
@@ -71,10 +75,13 @@ def kernel_table(kt, ncell, elem, coef, has_m, per_iter_launches, jacobi=False):
     if jacobi:
         table = {"spmv": SPMV_JACOBI, "update": KERNELS["update"]}
     fused = len(kt) > 15 and kt[15][1] == 0
+    fused_pre = len(kt) > 10 and kt[10][1] == 0
+    skip = {"mg_postsweep1_l0" if fused else "mg_post_l0"}
+    skip |= {"mg_presweep_l0", "mg_restrict_l0"} if fused_pre else {"mg_pre_l0"}
     for key, k in table.items():
         if k["slot"] >= len(kt):
             continue
-        if key == ("mg_postsweep1_l0" if fused else "mg_post_l0"):
+        if key in skip:
             continue
         ms, cnt = kt[k["slot"]]
         if cnt == 0 or ms <= 0:
@@ -317,7 +324,7 @@ def main():
     roofline, kernels = None, None
     if kt:
         per_iter = {"mg_presweep_l0": 1, "mg_postsweep1_l0": 1, "mg_postsweep2_l0": 1,
-                    "mg_post_l0": 1}
+                    "mg_post_l0": 1, "mg_pre_l0": 1}
         elem = 4 if main_prec == L.MIXED else 8
         coef = 2 if precond == L.PRECOND_MG and main_prec == L.MIXED else elem
         kernels = kernel_table(kt, ncell, elem, coef, has_m, per_iter,
