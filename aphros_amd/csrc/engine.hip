@@ -736,7 +736,7 @@ int Engine::build_tables() {
   std::vector<std::vector<Rec>> rlf(kMaxLevels), rlc(kMaxLevels);
   std::vector<std::vector<int>> blf(kMaxLevels), blc(kMaxLevels);
   int nlev_max = 0;
-  long long coarse_max = 0;  // LDS values of the bottom kernel
+  long long coarse_max = 0;  // LDS bytes of the bottom kernel
   int bottom_at[kMaxLevels] = {};
   int deep = 0;
   // x vector width of each level's sweeps: the widest that divides the level's
@@ -792,9 +792,15 @@ int Engine::build_tables() {
     if (pc_) {
       const int nl = (int)s.lev.size();
       nlev_max = std::max(nlev_max, nl);
-      long long bottom = 0;
-      for (int l = s.dev.mg_lb; l < nl; ++l) bottom += 2 * s.lev[l].n;
-      coarse_max = std::max(coarse_max, bottom);
+      // LDS of the bottom kernel (same rule as the kernel: everything
+      // resident if it fits, else z and a second buffer per level)
+      const int es = fp32_ ? 4 : 8;
+      long long zbytes = 0, fbytes = 0;
+      for (int l = s.dev.mg_lb; l < nl; ++l) {
+        zbytes += bottom_z_bytes(s.lev[l].n, es);
+        fbytes += bottom_full_bytes(s.lev[l].n, es, s.dev.has_m);
+      }
+      coarse_max = std::max(coarse_max, fbytes <= kBottomLdsMax ? fbytes : zbytes);
       bottom_at[s.dev.mg_lb] = 1;
       deep = std::max(deep, s.dev.mg_lb);
       for (int l = 0; l < s.dev.mg_lb; ++l) {
@@ -921,7 +927,7 @@ int Engine::build_tables() {
   for (int i : launch_slots_) L_.mg_w = std::max(L_.mg_w, slots_[i]->dev.mg_w);
   for (int l = 0; l < kMaxLevels; ++l) L_.vx_l[l] = vxl[l];
   L_.coarse_sweeps = 40;
-  L_.coarse_lds = (int)(coarse_max * (fp32_ ? 4 : 8));
+  L_.coarse_lds = (int)coarse_max;
   for (int l = 0; l < kMaxLevels; ++l) L_.bottom_at[l] = bottom_at[l];
   L_.deep = deep;
   for (int l = 0; l < kMaxLevels; ++l) {

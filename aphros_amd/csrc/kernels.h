@@ -48,6 +48,24 @@ struct LaunchSet {
 // threads of a k_mg_post workgroup
 constexpr int kPostBlock = 1024;
 
+// LDS of the bottom-levels kernel, per level of n cells (element size es):
+// z and a second buffer; with everything resident also the coefficients
+// (FP32: bfloat16 packed by 4 cells; FP64: kx, ky, kz, D^-1), the mass term
+// and the right-hand side.  Everything is resident when the sum over the
+// bottom levels stays within kBottomLdsMax.
+constexpr long long kBottomLdsMax = 156 * 1024;
+__host__ __device__ inline long long lds_al16(long long b) { return (b + 15) & ~15LL; }
+__host__ __device__ inline long long bottom_coef_bytes(long long n, int es) {
+  return es == 4 ? (n + 3) / 4 * 32 : 4 * lds_al16(8 * n);
+}
+__host__ __device__ inline long long bottom_z_bytes(long long n, int es) {
+  return 2 * lds_al16(n * es);
+}
+__host__ __device__ inline long long bottom_full_bytes(long long n, int es, int has_m) {
+  return bottom_z_bytes(n, es) + lds_al16(bottom_coef_bytes(n, es)) +
+         (has_m ? lds_al16(n * es) : 0) + lds_al16(n * es);
+}
+
 // kernels of one iteration, each reducing kernel followed by its finalize
 // kernel: PCG: spmv, fin, update, fin, ru_accum, fin, ru_resid, fin and, with
 // the multigrid preconditioner, [8] the V-cycle with its finalize;

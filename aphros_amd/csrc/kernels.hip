@@ -1359,9 +1359,11 @@ struct WaveLevel {
 
 // The bottom of the cycle: levels lb..top of a system (its cycle from level
 // lb down; lb > every W level) in one workgroup.  Levels of more than 64
-// cells: z and a second buffer of every level in LDS, r in global memory
-// (cached), barriers between the steps; the last one or two levels of at most
-// 64 cells: one wavefront, a cell per lane.  The same operations as the
+// cells: z and a second buffer of every level in LDS and, when they fit
+// (bottom_full_bytes), the levels' coefficients, mass and right-hand sides
+// too (copied in once; the restriction then writes the coarse right-hand
+// sides to LDS), barriers between the steps; the last one or two levels of at
+// most 64 cells: one wavefront, a cell per lane.  The same operations as the
 // per-level launches: zero-guess sweep + sweep, restriction, `sweeps` damped
 // Jacobi sweeps from zero on the coarsest level, prolongation + sweep + sweep;
 // the correction of level lb goes to its z[1] (z[0] when lb is the coarsest).
@@ -1374,6 +1376,7 @@ __global__ __launch_bounds__(kBottomThreads) void k_mg_bottom(const SysDev* __re
                                                               int sweeps, int l0, int wl,
                                                               int so) {
   extern __shared__ unsigned char smem[];
+  __shared__ MgLevel lv[kMaxLevels];  // the levels, with LDS pointers if resident
   const SysDev& s = sds[blockIdx.x];
   if (s.nlev <= 0 || s.sc->done || s.mg_lb != l0) return;
   if (wl >= 0 && wl >= s.mg_w) return;
@@ -1381,26 +1384,73 @@ __global__ __launch_bounds__(kBottomThreads) void k_mg_bottom(const SysDev* __re
   const int top = s.nlev - 1;
   const int tid = threadIdx.x, nt = blockDim.x;
   const T w = (T)kOmega, wa = (T)kOmegaA, wb = (T)kOmegaB;
-  T* za[kMaxLevels];
-  T* zb[kMaxLevels];
-  {
-    T* p = (T*)smem;
+  constexpr int es = sizeof(T);
+  long long full_bytes = 0;
+  for (int l = l0; l <= top; ++l) full_bytes += bottom_full_bytes(s.lev[l].n, es, s.has_m);
+  const bool full = full_bytes <= kBottomLdsMax;
+  // z and the second buffer of each level (pointer tables in LDS: indexed
+  // by level at run time, they would otherwise live in scratch)
+  __shared__ T* za[kMaxLevels];
+  __shared__ T* zb[kMaxLevels];
+  unsigned char* p = smem;
+  for (int l = l0; l <= top; ++l) {
+    if (tid == 0) za[l] = (T*)p;
+    p += lds_al16(s.lev[l].n * es);
+    if (tid == 0) zb[l] = (T*)p;
+    p += lds_al16(s.lev[l].n * es);
+  }
+  if (tid == 0) {
+    for (int l = l0; l <= top; ++l) lv[l] = s.lev[l];
+  }
+  if (full) {
+    // copy `bytes` (a multiple of 4) from global memory to LDS
+    auto copy = [&](void* dst, const void* src, long long bytes) {
+      const long long n16 = bytes >> 4;
+      for (long long i = tid; i < n16; i += nt) ((uint4*)dst)[i] = ((const uint4*)src)[i];
+      for (long long i = n16 * 4 + tid; i < (bytes >> 2); i += nt)
+        ((unsigned*)dst)[i] = ((const unsigned*)src)[i];
+    };
     for (int l = l0; l <= top; ++l) {
-      za[l] = p;
-      p += s.lev[l].n;
-      zb[l] = p;
-      p += s.lev[l].n;
+      const MgLevel& G = s.lev[l];
+      const long long n = G.n;
+      if constexpr (es == 4) {
+        copy(p, G.kc, bottom_coef_bytes(n, es));
+        if (tid == 0) lv[l].kc = p;
+        p += lds_al16(bottom_coef_bytes(n, es));
+      } else {
+        const void* src[4] = {G.kx, G.ky, G.kz, G.dinv};
+        for (int c = 0; c < 4; ++c) {
+          copy(p, src[c], 8 * n);
+          if (tid == 0) {
+            if (c == 0) lv[l].kx = p;
+            if (c == 1) lv[l].ky = p;
+            if (c == 2) lv[l].kz = p;
+            if (c == 3) lv[l].dinv = p;
+          }
+          p += lds_al16(8 * n);
+        }
+      }
+      if (s.has_m) {
+        copy(p, G.m, n * es);
+        if (tid == 0) lv[l].m = p;
+        p += lds_al16(n * es);
+      }
+      // right-hand side: level l0's from memory, the coarser ones restricted here
+      if (l == l0) copy(p, G.r, n * es);
+      if (tid == 0) lv[l].r = p;
+      p += lds_al16(n * es);
     }
   }
+  __syncthreads();
   // levels lw..top go to one wavefront (a level below one of <= 64 cells has
   // <= 8, and a coarser one would have < 8: at most two such levels)
   int lw = l0;
-  while (lw <= top && s.lev[lw].n > 64) ++lw;
+  while (lw <= top && lv[lw].n > 64) ++lw;
   // out = u + om D^-1 (r - L u) on level l, u and out in LDS
   auto sweep = [&](int l, const T* u, T* out, T om) {
-    const LevelView<T> V(s.lev[l]);
-    const T* __restrict__ r = (const T*)s.lev[l].r;
-    const int n = (int)s.lev[l].n;
+    const LevelView<T> V(lv[l]);
+    const T* __restrict__ r = (const T*)lv[l].r;
+    const int n = (int)lv[l].n;
     for (int i = tid; i < n; i += nt) {
       const int x = i % V.nx, y = (i / V.nx) % V.ny, z = i / (V.nx * V.ny);
       auto uf = [&](long long j) -> T { return u[j]; };
@@ -1409,14 +1459,14 @@ __global__ __launch_bounds__(kBottomThreads) void k_mg_bottom(const SysDev* __re
     __syncthreads();
   };
   for (int l = l0; l < lw && l < top; ++l) {
-    const LevelView<T> V(s.lev[l]);
-    const T* __restrict__ r = (const T*)s.lev[l].r;
-    const int n = (int)s.lev[l].n;
+    const LevelView<T> V(lv[l]);
+    const T* __restrict__ r = (const T*)lv[l].r;
+    const int n = (int)lv[l].n;
     for (int i = tid; i < n; i += nt) zb[l][i] = wa * V.dinv(i) * r[i];
     __syncthreads();
     sweep(l, zb[l], za[l], wb);
     // restriction of r - L z into r of level l + 1
-    const MgLevel& Lc = s.lev[l + 1];
+    const MgLevel& Lc = lv[l + 1];
     T* __restrict__ rc = (T*)Lc.r;
     const T* zl = za[l];
     auto uf = [&](long long j) -> T { return zl[j]; };
@@ -1437,15 +1487,15 @@ __global__ __launch_bounds__(kBottomThreads) void k_mg_bottom(const SysDev* __re
     if (tid < 64) {
       const int lane = tid;
       WaveLevel<T> A;
-      A.load(s.lev[lw], lane);
-      const T ra = lane < A.n ? ((const T*)s.lev[lw].r)[lane] : (T)0;
+      A.load(lv[lw], lane);
+      const T ra = lane < A.n ? ((const T*)lv[lw].r)[lane] : (T)0;
       T z;
       if (lw == top) {
         z = w * A.d * ra;
         for (int k = 1; k < sweeps; ++k) z = z + w * A.d * (ra - A.lap(z));
       } else {
-        const MgLevel& Lf = s.lev[lw];
-        const MgLevel& Lc = s.lev[lw + 1];
+        const MgLevel& Lf = lv[lw];
+        const MgLevel& Lc = lv[lw + 1];
         z = wa * A.d * ra;
         z = z + wb * A.d * (ra - A.lap(z));
         const T res = ra - A.lap(z);
@@ -1476,9 +1526,9 @@ __global__ __launch_bounds__(kBottomThreads) void k_mg_bottom(const SysDev* __re
     __syncthreads();
   } else {
     // a coarsest level of more than 64 cells: block-wide sweeps
-    const LevelView<T> V(s.lev[top]);
-    const T* __restrict__ r = (const T*)s.lev[top].r;
-    const int n = (int)s.lev[top].n;
+    const LevelView<T> V(lv[top]);
+    const T* __restrict__ r = (const T*)lv[top].r;
+    const int n = (int)lv[top].n;
     T* a = za[top];
     T* b = zb[top];
     for (int i = tid; i < n; i += nt) a[i] = w * V.dinv(i) * r[i];
@@ -1495,8 +1545,8 @@ __global__ __launch_bounds__(kBottomThreads) void k_mg_bottom(const SysDev* __re
     }
   }
   for (int l = (lw <= top ? lw : top) - 1; l >= l0; --l) {
-    const MgLevel& Lf = s.lev[l];
-    const MgLevel& Lc = s.lev[l + 1];
+    const MgLevel& Lf = lv[l];
+    const MgLevel& Lc = lv[l + 1];
     const int n = (int)Lf.n;
     for (int i = tid; i < n; i += nt) {
       const int x = i % Lf.nx, y = (i / Lf.nx) % Lf.ny, z = i / (Lf.nx * Lf.ny);
@@ -1508,8 +1558,8 @@ __global__ __launch_bounds__(kBottomThreads) void k_mg_bottom(const SysDev* __re
     for (int i = tid; i < n; i += nt) za[l][i] = zb[l][i];
     __syncthreads();
   }
-  T* out = (T*)s.lev[l0].z[l0 == top ? 0 : 1];
-  for (int i = tid; i < (int)s.lev[l0].n; i += nt) out[i] = za[l0][i];
+  T* out = (T*)s.lev[l0].z[l0 == top ? 0 : 1];  // global
+  for (int i = tid; i < (int)lv[l0].n; i += nt) out[i] = za[l0][i];
 }
 
 // (r, z) of the preconditioned CG: beta = (r,z) / (r,z)_old (0 after a start)
