@@ -369,6 +369,18 @@ __global__ __launch_bounds__(kBlock) void k_fin_pcg_spmv(const SysDev* __restric
 
 // x^ += alpha p ; r -= alpha q ; (r,r).  Decides whether a reliable update
 // (FP64 true residual) follows.
+// Cells [beg, end) of workgroup lb of a streaming launch of nblk workgroups:
+// contiguous chunks (multiples of kBlock * VX cells), which stream from HBM
+// faster than a grid-stride loop over the same workgroups.
+template <int VX>
+__device__ __forceinline__ void stream_range(long long n, int nblk, int lb, long long& beg,
+                                             long long& end) {
+  const long long unit = (long long)kBlock * VX;
+  const long long chunk = (n + (long long)nblk * unit - 1) / ((long long)nblk * unit) * unit;
+  beg = (long long)lb * chunk;
+  end = beg + chunk < n ? beg + chunk : n;
+}
+
 template <typename T, int VX>
 __global__ __launch_bounds__(kBlock) void k_pcg_update(const SysDev* __restrict__ sds,
                                                        const Rec* __restrict__ recs,
@@ -386,8 +398,9 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(const SysDev* __restrict_
   T* __restrict__ r = (T*)s.r;
   T* __restrict__ xh = (T*)s.xh;
   double acc[1] = {0.0};
-  const long long stride = (long long)rc.nblk * kBlock * VX;
-  for (long long i = ((long long)lb * kBlock + threadIdx.x) * VX; i < s.n; i += stride) {
+  long long beg, end;
+  stream_range<VX>(s.n, rc.nblk, lb, beg, end);
+  for (long long i = beg + (long long)threadIdx.x * VX; i < end; i += (long long)kBlock * VX) {
     const D pv = ldd<T, VX>(p, i);
     const D xv = rnd<T, VX>(ldd<T, VX>(xh, i) + alpha * pv);
     const D rv = rnd<T, VX>(ldd<T, VX>(r, i) - alpha * ldd<T, VX>(q, i));
@@ -441,8 +454,9 @@ __global__ __launch_bounds__(kBlock) void k_ru_accum(const SysDev* __restrict__ 
   T* __restrict__ xh = (T*)s.xh;
   double* __restrict__ x = s.x64;
   double acc[1] = {0.0};
-  const long long stride = (long long)rc.nblk * kBlock * VX;
-  for (long long i = ((long long)lb * kBlock + threadIdx.x) * VX; i < s.n; i += stride) {
+  long long beg, end;
+  stream_range<VX>(s.n, rc.nblk, lb, beg, end);
+  for (long long i = beg + (long long)threadIdx.x * VX; i < end; i += (long long)kBlock * VX) {
     const D sv = S ? ldd<T, VX>(S, i) : (D)(1.0);
     const D xv = (*reinterpret_cast<const D*>(x + i) - mp) + sv * ldd<T, VX>(xh, i);
     *reinterpret_cast<D*>(x + i) = xv;

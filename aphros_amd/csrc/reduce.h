@@ -68,12 +68,23 @@ __device__ __forceinline__ void reduce_partials(double (&v)[NV], const double* p
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
     const bool mx = (MaxMask >> k) & 1u;
-    double t = 0.0;
-    for (int i = threadIdx.x; i < nblk; i += kBlock) {
-      const double pv = partials[(long long)k * nblk + i];
-      t = mx ? fmax(t, pv) : t + pv;
+    const double* pk = partials + (long long)k * nblk;
+    // four independent accumulators (four loads in flight), combined in a
+    // fixed order
+    double t[4] = {0.0, 0.0, 0.0, 0.0};
+    int i = threadIdx.x;
+    for (; i + 3 * kBlock < nblk; i += 4 * kBlock) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const double pv = pk[i + j * kBlock];
+        t[j] = mx ? fmax(t[j], pv) : t[j] + pv;
+      }
     }
-    v[k] = t;
+    for (int j = 0; i < nblk; i += kBlock, ++j) {
+      const double pv = pk[i];
+      t[j] = mx ? fmax(t[j], pv) : t[j] + pv;
+    }
+    v[k] = mx ? fmax(fmax(t[0], t[1]), fmax(t[2], t[3])) : (t[0] + t[1]) + (t[2] + t[3]);
   }
   block_reduce<NV, MaxMask>(v);
 }
