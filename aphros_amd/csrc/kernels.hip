@@ -976,12 +976,15 @@ __global__ __launch_bounds__(kBlock) void k_mg_restrict(const SysDev* __restrict
 // planes.  Its rc.ty rows of threads are the output rows 1 .. ty-2 and one
 // halo row on each side: v is computed on all ty rows (the halo rows' v is
 // recomputed rather than exchanged between workgroups), z[1] on the output
-// rows two planes behind v.  Step k of the march forms z[1] on plane k-2 and
-// then v on plane k: both sets of loads are independent of the step's own
-// results, so one barrier per step suffices (u planes double-buffered in LDS
-// for the v sweep, v planes triple-buffered for the z[1] sweep).  The u rows
-// just outside the tile (for the halo rows' v) and the u planes before and
-// after the tile come from memory.
+// rows one plane behind v.  Step k of the march forms v on plane k and then
+// z[1] on plane k-1: the plane neighbours of v on plane k-1 are in LDS since
+// the previous step, the cell's own v on planes k-2 .. k and its coefficients
+// and r on plane k-1 (loaded by the previous step's v sweep) are in
+// registers, so one barrier per step suffices (u and v planes double-buffered
+// in LDS) and the second sweep re-reads from memory only the coefficients of
+// the neighbouring rows and columns, a plane after their first read (L2
+// hits).  The u rows just outside the tile (for the halo rows' v) and the u
+// planes before and after the tile come from memory.
 template <typename T, int VX, bool DOT, bool L0>
 __global__ __launch_bounds__(kPostBlock) void k_mg_post(const SysDev* __restrict__ sds,
                                                         const Rec* __restrict__ recs,
@@ -990,7 +993,7 @@ __global__ __launch_bounds__(kPostBlock) void k_mg_post(const SysDev* __restrict
   const int l = L0 ? 0 : l_arg;
   using V = vec_t<T, VX>;
   __shared__ T su[2][kPostBlock * VX];
-  __shared__ T sv[3][kPostBlock * VX];
+  __shared__ T sv[2][kPostBlock * VX];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   const Rec rc = recs[b2r[b]];
   const SysDev& s = sds[rc.sys];
@@ -1076,9 +1079,11 @@ __global__ __launch_bounds__(kPostBlock) void k_mg_post(const SysDev* __restrict
     return l7;
   };
   // v sweep: u at planes k-1, k and kz at plane k-1; z[1] sweep: v at planes
-  // k-3, k-2, k-1 (own cells; the plane neighbours of plane k-2 from LDS)
+  // k-2, k-1 (own cells), the cell's coefficients and r at plane k-1 and kz
+  // at plane k-2
   V um = (T)0, uc = (T)0, kzB = (T)0;
-  V v3 = (T)0, v2 = (T)0, v1 = (T)0;
+  V vm = (T)0, vc = (T)0;
+  V kx1 = (T)0, ky1 = (T)0, kz1 = (T)0, d1 = (T)0, r1 = (T)0, kzA = (T)0;
   if (active) {
     um = uv(y, zw(za - 2));
     uc = uv(y, zw(za - 1));
@@ -1088,30 +1093,11 @@ __global__ __launch_bounds__(kPostBlock) void k_mg_post(const SysDev* __restrict
   }
   __syncthreads();
   double dot = 0.0;
-  for (int k = za - 1; k <= zb + 1; ++k) {
+  for (int k = za - 1; k <= zb; ++k) {
     const int st = k - (za - 1);
-    // z[1] on plane k - 2 (output planes za .. zb-1) from v of planes k-3,
-    // k-2, k-1, all formed in earlier steps
-    if (k - 2 >= za && outrow) {
-      const int kp = zw(k - 2);
-      const long long base = (long long)kp * pl;
-      const long long ic = base + row + x0;
-      V kx1, ky1, kz1, d1;
-      co.template own<VX>(ic, kx1, ky1, kz1, d1);
-      const V kzm = co.template comp<VX>(2, (long long)zw(k - 3) * pl + row + x0);
-      const V r1 = ldv<T, VX>(r, ic);
-      auto none = [&](int) -> V { return (T)0; };  // output rows have both y neighbours
-      const V lv = lap(sv[(st + 1) % 3], base, v2, v3, v1, kx1, ky1, kz1, kzm, none);
-      V out = v2 + wa * d1 * (r1 - lv);
-      if (zsa) out += ldv<T, VX>(zsa, ic);
-      *reinterpret_cast<V*>(zo + ic) = out;
-      if constexpr (DOT) {
-#pragma unroll
-        for (int v = 0; v < VX; ++v) dot += (double)r1[v] * (double)out[v];
-      }
-    }
+    V vn = (T)0, kxc = (T)0, kyc = (T)0, kzc = (T)0, dc = (T)0, rv = (T)0;
     // v on plane k (planes za-1 .. zb)
-    if (k <= zb && active) {
+    if (active) {
       const int kk = zw(k);
       const V un = uv(y, zw(k + 1));
       if (k + 1 <= zb) {
@@ -1120,20 +1106,39 @@ __global__ __launch_bounds__(kPostBlock) void k_mg_post(const SysDev* __restrict
       }
       const long long base = (long long)kk * pl;
       const long long ic = base + row + x0;
-      V kxc, kyc, kzc, dc;
       co.template own<VX>(ic, kxc, kyc, kzc, dc);
+      rv = ldv<T, VX>(r, ic);
       auto far = [&](int yy) -> V { return uv(yy, kk); };
       const V lu = lap(su[st & 1], base, uc, um, un, kxc, kyc, kzc, kzB, far);
-      const V vn = uc + wb * dc * (ldv<T, VX>(r, ic) - lu);
+      vn = uc + wb * dc * (rv - lu);
 #pragma unroll
-      for (int v = 0; v < VX; ++v) sv[st % 3][lofs + v] = vn[v];
+      for (int v = 0; v < VX; ++v) sv[st & 1][lofs + v] = vn[v];
       kzB = kzc;
       um = uc;
       uc = un;
-      v3 = v2;
-      v2 = v1;
-      v1 = vn;
     }
+    // z[1] on plane k - 1 (output planes za .. zb-1; no wrap)
+    if (k - 1 >= za && outrow) {
+      const long long base = (long long)(k - 1) * pl;
+      const long long ic = base + row + x0;
+      auto none = [&](int) -> V { return (T)0; };  // output rows have both y neighbours
+      const V lv = lap(sv[(st + 1) & 1], base, vc, vm, vn, kx1, ky1, kz1, kzA, none);
+      V out = vc + wa * d1 * (r1 - lv);
+      if (zsa) out += ldv<T, VX>(zsa, ic);
+      *reinterpret_cast<V*>(zo + ic) = out;
+      if constexpr (DOT) {
+#pragma unroll
+        for (int v = 0; v < VX; ++v) dot += (double)r1[v] * (double)out[v];
+      }
+    }
+    vm = vc;
+    vc = vn;
+    kzA = kz1;
+    kx1 = kxc;
+    ky1 = kyc;
+    kz1 = kzc;
+    d1 = dc;
+    r1 = rv;
     __syncthreads();
   }
   if constexpr (DOT) {
@@ -1147,12 +1152,13 @@ __global__ __launch_bounds__(kPostBlock) void k_mg_post(const SysDev* __restrict
 // even row, an even number of planes from an even plane):
 //   u = wA D^-1 r;  z = u + wB D^-1 (r - L u) -> z[0] (output rows and planes);
 //   r_{l+1}[C] = sum over the 2x2x2 children c of (r - L z)_c.
-// Step k forms the residual on plane k-2 (z of planes k-3 .. k-1, from earlier
-// steps) and then z on plane k (halo rows and the planes za-1, zb included).
-// x pairs are summed in the thread, plane pairs across two steps, row pairs
-// through LDS after the next barrier (odd rows hand their sums to the even row
-// above).  One barrier per step: u planes double-buffered, z planes
-// triple-buffered in LDS.
+// Step k forms z on plane k (halo rows and the planes za-1, zb included) and
+// then the residual on plane k-1 (plane neighbours of z in LDS since the
+// previous step; the cell's own z on planes k-2 .. k and its coefficients and
+// r on plane k-1 in registers).  x pairs are summed in the thread, plane
+// pairs across two steps, row pairs through LDS after the next barrier (odd
+// rows hand their sums to the even row above).  One barrier per step; u and z
+// planes double-buffered in LDS.
 template <typename T, int VX, bool L0>
 __global__ __launch_bounds__(kPostBlock) void k_mg_pre(const SysDev* __restrict__ sds,
                                                        const Rec* __restrict__ recs,
@@ -1163,7 +1169,7 @@ __global__ __launch_bounds__(kPostBlock) void k_mg_pre(const SysDev* __restrict_
   const int l = L0 ? 0 : l_arg;
   using V = vec_t<T, VX>;
   __shared__ T su[2][kPostBlock * VX];
-  __shared__ T sz[3][kPostBlock * VX];
+  __shared__ T sz[2][kPostBlock * VX];
   __shared__ T sacc[kPostBlock * NP];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   const Rec rc = recs[b2r[b]];
@@ -1236,10 +1242,12 @@ __global__ __launch_bounds__(kPostBlock) void k_mg_pre(const SysDev* __restrict_
     return l7;
   };
   // z sweep: u at planes k-1, k, r at plane k, kz at plane k-1; residual: z at
-  // planes k-3, k-2, k-1 (own cells); the x-pair sums of the current plane
-  // pair; even rows keep their sums of the last pair for the row pair
+  // planes k-2, k-1 (own cells), the cell's coefficients and r at plane k-1,
+  // kz at plane k-2; the x-pair sums of the current plane pair; even rows
+  // keep their sums of the last pair for the row pair
   V um = (T)0, uc = (T)0, rcur = (T)0, kzB = (T)0;
-  V z3 = (T)0, z2 = (T)0, z1 = (T)0;
+  V zm = (T)0, zc0 = (T)0;
+  V kx1 = (T)0, ky1 = (T)0, kz1 = (T)0, r1 = (T)0, kzA = (T)0;
   T racc[NP], rpend[NP];
 #pragma unroll
   for (int j = 0; j < NP; ++j) racc[j] = rpend[j] = (T)0;
@@ -1254,7 +1262,7 @@ __global__ __launch_bounds__(kPostBlock) void k_mg_pre(const SysDev* __restrict_
     for (int v = 0; v < VX; ++v) su[0][lofs + v] = uc[v];
   }
   __syncthreads();
-  for (int k = za - 1; k <= zb + 1; ++k) {
+  for (int k = za - 1; k <= zb; ++k) {
     const int st = k - (za - 1);
     // row pairs of the plane pair completed in the previous step
     if (kpend >= 0) {
@@ -1264,17 +1272,40 @@ __global__ __launch_bounds__(kPostBlock) void k_mg_pre(const SysDev* __restrict_
             rpend[j] + sacc[(threadIdx.x + lx) * NP + j];
       kpend = -1;
     }
-    // residual on plane k - 2 (planes za .. zb-1)
-    if (k - 2 >= za && outrow) {
-      const int kp = k - 2;  // in [za, zb), no wrap
-      const long long base = (long long)kp * pl;
+    V zn = (T)0, kxc = (T)0, kyc = (T)0, kzc = (T)0, rk = (T)0;
+    // z on plane k (planes za-1 .. zb)
+    if (active) {
+      const int kk = zw(k);
+      const long long in = (long long)zw(k + 1) * pl + row + x0;
+      const V rn = ldv<T, VX>(r, in);
+      const V un = wa * co.template comp<VX>(3, in) * rn;
+      if (k + 1 <= zb) {
+#pragma unroll
+        for (int v = 0; v < VX; ++v) su[(st + 1) & 1][lofs + v] = un[v];
+      }
+      const long long base = (long long)kk * pl;
       const long long ic = base + row + x0;
-      V kx1, ky1, kz1, d1;
-      co.template own<VX>(ic, kx1, ky1, kz1, d1);
-      const V kzm = co.template comp<VX>(2, (long long)zw(kp - 1) * pl + row + x0);
+      V dc;
+      co.template own<VX>(ic, kxc, kyc, kzc, dc);
+      auto far = [&](int yy) -> V { return uv(yy, kk); };
+      const V lu = lap(su[st & 1], base, uc, um, un, kxc, kyc, kzc, kzB, far);
+      zn = uc + wb * dc * (rcur - lu);
+#pragma unroll
+      for (int v = 0; v < VX; ++v) sz[st & 1][lofs + v] = zn[v];
+      if (outrow && k >= za && k < zb) *reinterpret_cast<V*>(zo + ic) = zn;
+      rk = rcur;
+      kzB = kzc;
+      um = uc;
+      uc = un;
+      rcur = rn;
+    }
+    // residual on plane k - 1 (planes za .. zb-1, no wrap)
+    if (k - 1 >= za && outrow) {
+      const int kp = k - 1;
+      const long long base = (long long)kp * pl;
       auto none = [&](int) -> V { return (T)0; };
-      const V lz = lap(sz[(st + 1) % 3], base, z2, z3, z1, kx1, ky1, kz1, kzm, none);
-      const V res = ldv<T, VX>(r, ic) - lz;
+      const V lz = lap(sz[(st + 1) & 1], base, zc0, zm, zn, kx1, ky1, kz1, kzA, none);
+      const V res = r1 - lz;
       if ((kp & 1) == 0) {
 #pragma unroll
         for (int j = 0; j < NP; ++j) racc[j] = res[2 * j] + res[2 * j + 1];
@@ -1291,34 +1322,13 @@ __global__ __launch_bounds__(kPostBlock) void k_mg_pre(const SysDev* __restrict_
         }
       }
     }
-    // z on plane k (planes za-1 .. zb)
-    if (k <= zb && active) {
-      const int kk = zw(k);
-      const long long in = (long long)zw(k + 1) * pl + row + x0;
-      const V rn = ldv<T, VX>(r, in);
-      const V un = wa * co.template comp<VX>(3, in) * rn;
-      if (k + 1 <= zb) {
-#pragma unroll
-        for (int v = 0; v < VX; ++v) su[(st + 1) & 1][lofs + v] = un[v];
-      }
-      const long long base = (long long)kk * pl;
-      const long long ic = base + row + x0;
-      V kxc, kyc, kzc, dc;
-      co.template own<VX>(ic, kxc, kyc, kzc, dc);
-      auto far = [&](int yy) -> V { return uv(yy, kk); };
-      const V lu = lap(su[st & 1], base, uc, um, un, kxc, kyc, kzc, kzB, far);
-      const V zn = uc + wb * dc * (rcur - lu);
-#pragma unroll
-      for (int v = 0; v < VX; ++v) sz[st % 3][lofs + v] = zn[v];
-      if (outrow && k >= za && k < zb) *reinterpret_cast<V*>(zo + ic) = zn;
-      kzB = kzc;
-      um = uc;
-      uc = un;
-      rcur = rn;
-      z3 = z2;
-      z2 = z1;
-      z1 = zn;
-    }
+    zm = zc0;
+    zc0 = zn;
+    kzA = kz1;
+    kx1 = kxc;
+    ky1 = kyc;
+    kz1 = kzc;
+    r1 = rk;
     __syncthreads();
   }
   // the last plane pair (planes zb-2, zb-1)
