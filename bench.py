@@ -63,6 +63,18 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo) and logical CPU count."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return f"{line.split(':', 1)[1].strip()} ({os.cpu_count()} logical CPUs)"
+    except OSError:
+        pass
+    return None
+
+
 # the Jacobi-scaled form of the operator: p = S r + beta p, read r, p_in, S,
 # kx, ky, kz; write p_out, q
 SPMV_JACOBI = {"slot": 0, "vec": 8.0, "coef": 0.0, "m": 1, "name": "k_pcg_spmv<T, VX, true>"}
@@ -342,6 +354,13 @@ def main():
                         "element": "float" if main_prec == L.MIXED else "double",
                         "avg_ms": k["avg_ms"], "launches": k["launches"],
                         "plain_copy_GBps": copy_gbps}
+        # level-0 kernels of one iteration together: algorithmic bytes over
+        # their summed average durations
+        l0 = [v for v in kernels.values() if isinstance(v, dict)]
+        if l0:
+            kernels["level0_alg_GBps"] = round(
+                sum(v["alg_bytes_per_launch"] for v in l0) /
+                (sum(v["avg_ms"] for v in l0) * 1e-3) / 1e9, 1)
         if precond == L.PRECOND_MG and len(kt) > 12:
             kernels["vcycle_total_ms"] = round(kt[8][0], 5)
             kernels["mg_coarse_levels_ms_per_vcycle"] = round(kt[11][0], 5)
@@ -369,7 +388,7 @@ def main():
         dt = time.perf_counter() - t
         s_per_iter = (dt - setup) / it
         cpu = {"value": round(1.0 / (setup + s_per_iter * ref_iters), 6), "unit": "solves/s",
-               "cores": O.num_threads(), "kind": "port",
+               "cores": O.num_threads(), "kind": "port", "cpu_model": cpu_model(),
                "sample": f"{it} FP64 {name} iterations of the oracle (OpenMP) on the {n}^3 "
                          f"system ({dt:.1f}s: setup {setup:.1f}s, {s_per_iter:.3f} s/iter), "
                          f"scaled to the {ref_iters} iterations the FP64 GPU solve needs"}
