@@ -174,6 +174,49 @@ def test_mg_vcycle_large_fused_post(c2, precision, tol, w):
     assert err <= tol
 
 
+def test_mg_vcycle_fused_noncubic_mass():
+    """A 256 x 256 x 320 (z, y, x) system with a mass term: fused level-0
+    kernels with rows of 80 threads (12 thread rows per tile, 10 output rows,
+    a partial last tile in y) and the mass term in both fused kernels."""
+    shape = (256, 256, 320)
+    rng = np.random.default_rng(21)
+    rho = np.where(rng.random(shape) < 0.2, 100.0, 1.0)
+    f = rng.standard_normal(shape)
+    f -= f.mean()
+    e = _with_mass(O.assemble(rho, f))
+    _say("vcycle 320x256x256 mass, mixed, w=3")
+    s = Poisson(shape, method=L.PCG, precision=L.MIXED, resmode=L.RES_REL_TRUE, tol=1e-8,
+                maxiter=20000, precond=L.PRECOND_MG, mg_w=3)
+    s.set_system(e)
+    r = rng.standard_normal(shape)
+    z = s.precond_apply(r)
+    zo, nlev = O.mg_apply(e, r, bf16=True, w=3)
+    err = np.abs(z - zo).max() / np.abs(zo).max()
+    _say(f"  nlev={nlev} rel_err={err:.3e}")
+    assert err <= 1e-4
+
+
+def test_mg_vcycle_512_fused_w_levels():
+    """The benchmark system at 512^3 with 3 W levels: fused kernels on level 0
+    and on level 1 (256^3), whose second coarse correction ends with the
+    fused post-smoothing adding the first correction."""
+    e, _, _ = synth.config("c3_512")
+    e = e.reshape(512, 512, 512, 8)
+    _say("vcycle c3_512 mixed w=3")
+    s = Poisson(e.shape[:3], method=L.PCG, precision=L.MIXED, resmode=L.RES_REL_TRUE,
+                tol=1e-8, maxiter=20000, precond=L.PRECOND_MG, mg_w=3)
+    s.set_system(e)
+    rng = np.random.default_rng(14)
+    r = rng.standard_normal(e.shape[:3])
+    r -= r.mean()
+    z = s.precond_apply(r)
+    s.close()
+    zo, nlev = O.mg_apply(e, r, bf16=True, w=3)
+    err = np.abs(z - zo).max() / np.abs(zo).max()
+    _say(f"  nlev={nlev} rel_err={err:.3e}")
+    assert err <= 1e-4
+
+
 def test_pcg_large_fused_post_true_residual(c2):
     """The fused level-0 post-smoothing also forms (r, z) for beta: a mixed
     multigrid PCG solve at 256^3 reaches the FP64 true residual (recomputed on
