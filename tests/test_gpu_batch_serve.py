@@ -56,6 +56,26 @@ def test_batch_equals_single(precision, precond, w):
     b.close()
 
 
+@pytest.mark.parametrize("precision", [L.MIXED, L.FP64])
+def test_batch_fused_and_plain_levels(precision):
+    """A 256^3 system (fused pre/post kernels on its level 0) in one batch with
+    32^3 and 48^3 systems (separate sweeps): every level launch then runs both
+    kinds, each skipping the other's systems; results equal the single solves
+    bit for bit."""
+    systems = [_sys(256, 16, 0.05, 0.15, 1000.0, 31), _sys(*SYSTEMS[0]), _sys(*SYSTEMS[1])]
+    b = Batch(precision=precision, tol=1e-8, maxiter=50000, precond=L.PRECOND_MG)
+    slots = [b.add(E) for E in systems]
+    b.solve()
+    for E, sl in zip(systems, slots):
+        xb, ib = b.get(sl)
+        xs, is_ = _single(E, precision, precond=L.PRECOND_MG)
+        print(f"n={E.shape[0]} iter={ib['iter']} relres={ib['true_relres']:.2e}", flush=True)
+        assert ib["converged"] == 1 and ib["iter"] == is_["iter"]
+        assert np.array_equal(xb, xs)
+        assert O.true_relres(E, xb) <= 1e-8 * (1 + 1e-6)
+    b.close()
+
+
 def test_batch_warm_start_new_rhs():
     E = _sys(*SYSTEMS[2])
     b = Batch(precision=L.MIXED, tol=1e-8)
