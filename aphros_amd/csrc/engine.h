@@ -194,6 +194,10 @@ class Engine {
   hipGraphExec_t gexec_ = nullptr;
   hipGraph_t graph_ = nullptr;
   int graph_k_ = 0;
+  // event profiling: the cycle below level 0 as one graph (LaunchSet::below0)
+  hipGraphExec_t below_exec_ = nullptr;
+  hipGraph_t below_graph_ = nullptr;
+  int prof_nl_ = 0;  // events of one profiled cycle minus one
   // scratch
   void* staging_ = nullptr;
   long long staging_bytes_ = 0;

@@ -103,6 +103,8 @@ Engine::~Engine() {
   }
   if (gexec_) (void)hipGraphExecDestroy(gexec_);
   if (graph_) (void)hipGraphDestroy(graph_);
+  if (below_exec_) (void)hipGraphExecDestroy(below_exec_);
+  if (below_graph_) (void)hipGraphDestroy(below_graph_);
   if (d_sc_) (void)hipFree(d_sc_);
   if (h_sc_) (void)hipHostFree(h_sc_);
   if (d_counter_) (void)hipFree(d_counter_);
@@ -957,6 +959,14 @@ int Engine::capture() {
     APH_HIP(hipGraphDestroy(graph_));
     graph_ = nullptr;
   }
+  if (below_exec_) {
+    APH_HIP(hipGraphExecDestroy(below_exec_));
+    below_exec_ = nullptr;
+  }
+  if (below_graph_) {
+    APH_HIP(hipGraphDestroy(below_graph_));
+    below_graph_ = nullptr;
+  }
   const int nk = iter_kernels(method_, pc_ ? 1 : 0);
   // launches per iteration (the multigrid cycle counts as one kernel in nk)
   const int nl = nk + (pc_ ? vcycle_launches(L_) - 1 : 0);
@@ -1137,7 +1147,9 @@ int Engine::run() {
   while (!alldone) {
     if (profile_) {
       // same kernels as the graph, launched one by one with an event before
-      // each kernel (events cannot be timed inside a replayed graph)
+      // each kernel (HIP does not record events captured into a graph); the
+      // cycle below level 0 is one replay of its own graph, so that the
+      // event-timed solve launches only the level-0 and CG kernels singly
       const int nk = iter_kernels(method_, pc_ ? 1 : 0);
       const int nv = pc_ ? vcycle_launches(L_) + 1 : 0;
       while ((int)pev_mg_.size() < graph_k_ * nv) {
@@ -1146,13 +1158,30 @@ int Engine::run() {
         pev_mg_.push_back(e);
       }
       mg_kinds_.resize(nv);
+      LaunchSet Lp = L_;
+      if (pc_) {
+        if (!below_exec_) {
+          APH_HIP(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
+          const hipError_t e = capture_below0(fp32_, L_, st_);
+          hipGraph_t g = nullptr;
+          const hipError_t e2 = hipStreamEndCapture(st_, &g);
+          if (e != hipSuccess || e2 != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return hip(e != hipSuccess ? e : e2, "capture (cycle below level 0)");
+          }
+          below_graph_ = g;
+          APH_HIP(hipGraphInstantiate(&below_exec_, below_graph_, nullptr, nullptr, 0));
+        }
+        Lp.below0 = below_exec_;
+      }
       for (int sl = 0; sl < graph_k_; ++sl) {
         for (int j = 0; j < nk; ++j) {
           APH_HIP(hipEventRecord(pev_[(size_t)sl * nk + j], st_));
           if (j == 8 && pc_) {
             int nl = 0;
-            APH_HIP(launch_vcycle_profiled(fp32_, L_, st_, &pev_mg_[(size_t)sl * nv],
+            APH_HIP(launch_vcycle_profiled(fp32_, Lp, st_, &pev_mg_[(size_t)sl * nv],
                                            mg_kinds_.data(), &nl));
+            prof_nl_ = nl;
           } else {
             APH_HIP(launch_kernel(method_, fp32_, vx_, L_, st_, j));
           }
@@ -1187,9 +1216,9 @@ int Engine::run() {
           // three), [10] level-0 restriction, [11] coarser levels (per
           // cycle), [12] bottom-levels kernel, [13] [14] [15] level-0
           // pre-sweep, first and second post-sweep
-          const int nv = vcycle_launches(L_) + 1;
+          const int nv = vcycle_launches(L_) + 1;  // event stride per cycle
           double coarse = 0;
-          for (int i = 0; i + 1 < nv; ++i) {
+          for (int i = 0; i < prof_nl_; ++i) {
             float ms = 0;
             APH_HIP(hipEventElapsedTime(&ms, pev_mg_[(size_t)sl * nv + i],
                                         pev_mg_[(size_t)sl * nv + i + 1]));

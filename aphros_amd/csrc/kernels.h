@@ -43,6 +43,9 @@ struct LaunchSet {
   const Rec* rec_lp[kMaxLevels] = {};
   const int* b2r_lp[kMaxLevels] = {};
   int nlp[kMaxLevels] = {};
+  // event-profiled cycles only: the launches below level 0 as one replay of
+  // this graph (captured by capture_below0), nullptr: launched one by one
+  hipGraphExec_t below0 = nullptr;
 };
 
 // threads of a k_mg_post workgroup
@@ -82,6 +85,9 @@ hipError_t launch_mg_setup(int fp32, const MgLevel* levels, int nlev, const void
 hipError_t launch_vcycle_profiled(int fp32, const LaunchSet& L, hipStream_t st, hipEvent_t* ev,
                                   int* kinds, int* nlaunch);
 int vcycle_launches(const LaunchSet& L);
+// Records into the capturing stream st the launches of a cycle below level 0
+// (levels >= 1 with their W corrections), for LaunchSet::below0.
+hipError_t capture_below0(int fp32, const LaunchSet& L, hipStream_t st);
 hipError_t launch_kernel(int method, int fp32, int vx, const LaunchSet& L, hipStream_t st,
                          int j);
 hipError_t launch_iter(int method, int fp32, int vx, const LaunchSet& L, hipStream_t st);

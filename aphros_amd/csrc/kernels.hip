@@ -2292,11 +2292,17 @@ static void launch_cycle(const LaunchSet& L, hipStream_t st, int l, int wl, bool
     mark(l == 0 ? 1 : 2);
     launch_sweep<T, 3, false>(L, st, l, -1, -1, wl);
   }
-  launch_cycle<T>(L, st, l + 1, wl, false, mark);
-  if (l < L.mg_w) {
+  if (l == 0 && L.below0) {
+    // event profiling: the coarser levels as one replay (one event)
     mark(2);
-    launch_sweep<T, 4, false>(L, st, l + 1, 1, 1, l);
-    launch_cycle<T>(L, st, l + 1, l, true, mark);
+    (void)hipGraphLaunch(L.below0, st);
+  } else {
+    launch_cycle<T>(L, st, l + 1, wl, false, mark);
+    if (l < L.mg_w) {
+      mark(2);
+      launch_sweep<T, 4, false>(L, st, l + 1, 1, 1, l);
+      launch_cycle<T>(L, st, l + 1, l, true, mark);
+    }
   }
   if (L.post_fused[l]) {
     mark(l == 0 ? 5 : 2);
@@ -2337,6 +2343,28 @@ hipError_t launch_vcycle_profiled(int fp32, const LaunchSet& L, hipStream_t st, 
                                   int* kinds, int* nlaunch) {
   *nlaunch = fp32 ? launch_vcycle<float>(L, st, ev, kinds)
                   : launch_vcycle<double>(L, st, ev, kinds);
+  return hipGetLastError();
+}
+
+hipError_t capture_below0(int fp32, const LaunchSet& L, hipStream_t st) {
+  LaunchSet Lb = L;
+  Lb.below0 = nullptr;
+  auto none = [](int) {};
+  if (L.deep >= 1) {
+    if (fp32) {
+      launch_cycle<float>(Lb, st, 1, -1, false, none);
+      if (0 < Lb.mg_w) {
+        launch_sweep<float, 4, false>(Lb, st, 1, 1, 1, 0);
+        launch_cycle<float>(Lb, st, 1, 0, true, none);
+      }
+    } else {
+      launch_cycle<double>(Lb, st, 1, -1, false, none);
+      if (0 < Lb.mg_w) {
+        launch_sweep<double, 4, false>(Lb, st, 1, 1, 1, 0);
+        launch_cycle<double>(Lb, st, 1, 0, true, none);
+      }
+    }
+  }
   return hipGetLastError();
 }
 
@@ -2411,6 +2439,7 @@ hipError_t launch_start(int method, int fp32, int vx, const LaunchSet& L, hipStr
     // and its finalize are launched start-only (first == 1).
     LaunchSet Ls = L;
     Ls.start_only = 1;
+    Ls.below0 = nullptr;
     for (int j = 4; j < iter_kernels(method, L.pc); ++j) {
       const hipError_t e = launch_kernel(method, fp32, vx, Ls, st, j);
       if (e != hipSuccess) return e;

@@ -364,7 +364,8 @@ def main():
         if precond == L.PRECOND_MG and len(kt) > 12:
             kernels["vcycle_total_ms"] = round(kt[8][0], 5)
             kernels["mg_coarse_levels_ms_per_vcycle"] = round(kt[11][0], 5)
-            kernels["mg_bottom_levels_ms"] = round(kt[12][0], 5)
+            if kt[12][1] > 0:
+                kernels["mg_bottom_levels_ms"] = round(kt[12][0], 5)
         ms_it = 1e3 * total / max(1, sum(iters))
         kernels["ms_per_iter_wall"] = round(ms_it, 5)
 
