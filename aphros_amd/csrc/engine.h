@@ -164,6 +164,7 @@ class Engine {
   int set_system_staged(int slot, const double* d, long long ly, long long lz);
 
   int device_ = 0;
+  int ncu_ = 256;  // compute units of the device
   int method_ = kConjugate;
   int fp32_ = 0;
   bool pc_ = false;

@@ -52,7 +52,7 @@ Rec make_stencil_rec(int nx, int ny, int nz, int vx, int blocks_target, const Tu
 // (ty >= 8 rows).
 constexpr long long kFuseMinCells = 1LL << 24;
 
-bool make_post_rec(int nx, int ny, int nz, int vx, Rec* out) {
+bool make_post_rec(int nx, int ny, int nz, int vx, int ncu, Rec* out) {
   const long long n = (long long)nx * ny * nz;
   if (vx <= 0 || nx % vx != 0 || n < kFuseMinCells) return false;
   const int lx = nx / vx;
@@ -63,13 +63,25 @@ bool make_post_rec(int nx, int ny, int nz, int vx, Rec* out) {
   r.ty = std::min(kPostBlock / lx, ny + 2) & ~1;
   r.tiles_x = 1;
   r.tiles_y = (int)cdiv(ny, r.ty - 2);
-  // about 1024 workgroups (two resident per CU), marches of >= 8 planes
-  const long long target = std::min(1024LL, std::max(1LL, n / 16384));
-  const long long tz = std::max(1LL, target / r.tiles_y);
-  r.bz = (int)std::min((long long)nz, std::max((long long)std::min(8, nz), cdiv(nz, tz)));
-  r.bz += r.bz & 1;  // plane pairs of the restriction (nz is even on these levels)
-  r.bz = std::min(r.bz, nz);
-  r.tiles_z = (int)cdiv(nz, r.bz);
+  // One workgroup is resident per CU (1024 threads at 128 VGPRs per lane) and
+  // a workgroup's time grows with the planes it marches, bz plus two halo
+  // planes.  The z split minimises rounds * (bz + 2), rounds = workgroups / CUs
+  // rounded up; of equal costs the one with fewer workgroups (less halo) wins.
+  // bz is even: the restriction pairs planes (nz is even on these levels).
+  const long long cus = std::max(1, ncu);
+  long long best = -1;
+  for (int tz = 1; tz <= nz; ++tz) {
+    int bz = (int)cdiv(nz, tz);
+    bz = std::min(bz + (bz & 1), nz);
+    const int tiles_z = (int)cdiv(nz, bz);
+    if (tiles_z != tz) continue;
+    const long long cost = cdiv((long long)r.tiles_y * tiles_z, cus) * (bz + 2);
+    if (best < 0 || cost < best) {
+      best = cost;
+      r.bz = bz;
+      r.tiles_z = tiles_z;
+    }
+  }
   r.nblk = r.tiles_y * r.tiles_z;
   *out = r;
   return true;
@@ -139,6 +151,7 @@ int Engine::init(int device, int method, int precision, int resmode, double tol,
   if (device < 0 || device >= ndev) return fail(APHPOIS_EINVAL, "no such device");
   device_ = device;
   APH_HIP(hipSetDevice(device_));
+  APH_HIP(hipDeviceGetAttribute(&ncu_, hipDeviceAttributeMultiprocessorCount, device_));
   method_ = method;
   fp32_ = precision == 1;
   if (precond < 0 || precond > 1) return fail(APHPOIS_EINVAL, "unknown preconditioner");
@@ -364,7 +377,7 @@ int Engine::add_system(int nx, int ny, int nz, double h3, int* slot) {
       // property of the system alone, like all of its tiling)
       for (int l = 0; l < lb; ++l) {
         Rec t;
-        S.lev[l].post = make_post_rec(S.lev[l].nx, S.lev[l].ny, S.lev[l].nz, fp32_ ? 4 : 2, &t)
+        S.lev[l].post = make_post_rec(S.lev[l].nx, S.lev[l].ny, S.lev[l].nz, fp32_ ? 4 : 2, ncu_, &t)
                             ? 1
                             : 0;
       }
@@ -835,7 +848,7 @@ int Engine::build_tables() {
         if (F.post) {
           post_fused[l] = 1;
           Rec p{};
-          make_post_rec(F.nx, F.ny, F.nz, fp32_ ? 4 : 2, &p);
+          make_post_rec(F.nx, F.ny, F.nz, fp32_ ? 4 : 2, ncu_, &p);
           p.sys = k;
           p.blk0 = (int)blp[l].size();
           for (int i = 0; i < p.nblk; ++i) blp[l].push_back((int)rlp[l].size());
