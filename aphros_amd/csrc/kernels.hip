@@ -611,13 +611,15 @@ __global__ __launch_bounds__(kBlock) void k_fin_ru_resid(const SysDev* __restric
 // zero guess), post-smoothing B then A; both apply the same polynomial of
 // D^-1 L, so the cycle stays symmetric.  The coarsest level: `sweeps` sweeps
 // of weight kOmega.  Restriction = sum over the 2x2x2 children, prolongation
-// = injection (its transpose), so the V-cycle is a symmetric linear operator.
-// Arithmetic in the engine's element type T.
+// = injection (its transpose) scaled by kProlong (an over-correction of the
+// piecewise-constant interpolation; a scaled symmetric term), so the V-cycle
+// is a symmetric linear operator.  Arithmetic in the engine's element type T.
 // ===========================================================================
 constexpr double kOmega = 0.9;
 constexpr double kChebLo = 0.5, kChebHi = 1.8;
 constexpr double kOmegaA = 0.6212648748965077;  // 1 / (c + d cos(pi/4))
 constexpr double kOmegaB = 1.4484764074432;     // 1 / (c - d cos(pi/4))
+constexpr double kProlong = 1.1;
 
 template <typename T>
 struct LevelView {
@@ -784,9 +786,9 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
         // xx is a multiple of VX: the VX / 2 coarse cells are contiguous
         const vec_t<T, VX / 2> c = ldv<T, VX / 2>(zc, cb);
 #pragma unroll
-        for (int v = 0; v < VX; ++v) u[v] += c[v / 2];
+        for (int v = 0; v < VX; ++v) u[v] += (T)kProlong * c[v / 2];
       } else {
-        u[0] += zc[cb];
+        u[0] += (T)kProlong * zc[cb];
       }
       return u;
     } else {
@@ -799,7 +801,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
       return wa * co.comp1(3, i) * r[i];
     } else if constexpr (MODE == 2) {
       const long long ci = ((long long)(zz >> 1) * cny + (yy >> 1)) * cnx + (xx >> 1);
-      return zi[i] + zc[ci];
+      return zi[i] + (T)kProlong * zc[ci];
     } else {
       return zi[i];
     }
@@ -1040,9 +1042,9 @@ __global__ __launch_bounds__(kPostBlock) void k_mg_post(const SysDev* __restrict
     if constexpr (VX >= 2) {
       const vec_t<T, VX / 2> c = ldv<T, VX / 2>(zc, cb);
 #pragma unroll
-      for (int v = 0; v < VX; ++v) u[v] += c[v / 2];
+      for (int v = 0; v < VX; ++v) u[v] += (T)kProlong * c[v / 2];
     } else {
-      u[0] += zc[cb];
+      u[0] += (T)kProlong * zc[cb];
     }
     return u;
   };
@@ -1541,7 +1543,8 @@ __global__ __launch_bounds__(kBottomThreads) void k_mg_bottom(const SysDev* __re
         // prolongation of the coarsest correction, two sweeps
         const int c = lane < A.n ? lane : 0;
         const int x = c % Lf.nx, y = (c / Lf.nx) % Lf.ny, zz = c / (Lf.nx * Lf.ny);
-        const T u = z + __shfl(zc, ((zz >> 1) * Lc.ny + (y >> 1)) * Lc.nx + (x >> 1), 64);
+        const T u =
+            z + (T)kProlong * __shfl(zc, ((zz >> 1) * Lc.ny + (y >> 1)) * Lc.nx + (x >> 1), 64);
         z = u + wb * A.d * (ra - A.lap(u));
         z = z + wa * A.d * (ra - A.lap(z));
       }
@@ -1574,7 +1577,8 @@ __global__ __launch_bounds__(kBottomThreads) void k_mg_bottom(const SysDev* __re
     const int n = (int)Lf.n;
     for (int i = tid; i < n; i += nt) {
       const int x = i % Lf.nx, y = (i / Lf.nx) % Lf.ny, z = i / (Lf.nx * Lf.ny);
-      zb[l][i] = za[l][i] + za[l + 1][((long long)(z >> 1) * Lc.ny + (y >> 1)) * Lc.nx + (x >> 1)];
+      zb[l][i] = za[l][i] + (T)kProlong * za[l + 1][((long long)(z >> 1) * Lc.ny + (y >> 1)) * Lc.nx +
+                                                 (x >> 1)];
     }
     __syncthreads();
     sweep(l, zb[l], za[l], wb);

@@ -503,7 +503,8 @@ int orc_pcg_jacobi(const double* expr, int nx, int ny, int nz, const double* x0,
  *   l < top: u = a D^-1 r; z = u + b D^-1 (r - L u);
  *            r_{l+1} = sum over the 2x2x2 children of (r - L z);
  *            z_{l+1} = M r_{l+1};
- *            u = z + P z_{l+1} (P = injection to the children);
+ *            u = z + p P z_{l+1} (P = injection to the children, p =
+ *            kMgProlong, an over-correction of the constant interpolation);
  *            z = u + b D^-1 (r - L u); z = z + a D^-1 (r - L z)
  *   top:     u = w D^-1 r, then `sweeps - 1` more damped Jacobi sweeps
  * Restriction is the transpose of the injection, and pre- and post-smoothing
@@ -512,7 +513,7 @@ int orc_pcg_jacobi(const double* expr, int nx, int ny, int nz, const double* x0,
  * W levels: on the top w levels (l < w, and only where level l+1 is not the
  * coarsest: w <= nlev - 2) the coarse correction is applied twice: with
  * zs = z_{l+1}, r_{l+1} -= L_{l+1} zs, z_{l+1} = M r_{l+1}, and the
- * prolongation adds (z_{l+1} + zs).  Two symmetric corrections in sequence
+ * prolongation adds p (z_{l+1} + zs).  Two symmetric corrections in sequence
  * keep M symmetric.
  *
  * CG: textbook PCG with M, stopped on the recursive residual
@@ -534,6 +535,8 @@ typedef struct {
 static const double kMgOmega = 0.9;
 static const double kMgOmegaA = 0.6212648748965077;
 static const double kMgOmegaB = 1.4484764074432;
+/* scale of the prolonged coarse correction */
+static const double kMgProlong = 1.1;
 
 static double wall_seconds(void) {
 #ifdef _OPENMP
@@ -626,7 +629,7 @@ static void mg_vcycle(mglev_t* lev, int l, int top, int sweeps, int w) {
         const size_t c = (size_t)x + (size_t)L->nx * ((size_t)y + (size_t)L->ny * z);
         const size_t cc =
             (size_t)(x >> 1) + (size_t)C->nx * ((size_t)(y >> 1) + (size_t)C->ny * (z >> 1));
-        L->t[c] = L->z[c] + (zs ? C->z[cc] + zs[cc] : C->z[cc]);
+        L->t[c] = L->z[c] + kMgProlong * (zs ? C->z[cc] + zs[cc] : C->z[cc]);
       }
   free(zs);
   mg_sweep(L, L->t, L->z, kMgOmegaB);

@@ -23,6 +23,7 @@ OMEGA = 0.9  # coarsest level
 _C, _D = (0.5 + 1.8) / 2, (1.8 - 0.5) / 2
 OMEGA_A = 1 / (_C + _D * np.cos(np.pi / 4))
 OMEGA_B = 1 / (_C - _D * np.cos(np.pi / 4))
+PROLONG = 1.1  # scale of the prolonged coarse correction
 
 
 def _system(n, seed, ratio=1000.0, mass=0.0, sign=1, shape=None):
@@ -97,7 +98,7 @@ def _dense_vcycle(e, sweeps, scale=0.5, shift=0, w=0):
             zs = zc
             zc = vc(l + 1, rc - A[l + 1] @ zs)
             zc = zc + zs
-        z = smooth(l, z + Ps[l] @ zc, r, OMEGA_B)
+        z = smooth(l, z + PROLONG * (Ps[l] @ zc), r, OMEGA_B)
         return smooth(l, z, r, OMEGA_A)
 
     n = A[0].shape[0]
