@@ -4,6 +4,7 @@ The library is the product: HIP kernels for gfx950 behind a C ABI.  Loading
 fails loudly when it has not been built; there is no CPU fallback.
 """
 import ctypes as C
+import math
 import os
 
 import numpy as np
@@ -90,6 +91,14 @@ _ll = C.c_longlong
 
 # name -> (restype, argtypes); the exported surface of include/aphpois.h
 MG_W_DEFAULT = 3  # APHPOIS_MG_W_DEFAULT
+MG_W_AUTO = -1  # APHPOIS_MG_W_AUTO
+
+
+def mg_w_auto(n):
+    """The W levels APHPOIS_MG_W_AUTO gives a system of n cells (the engine's
+    rule, before clamping to the system's depth)."""
+    lg = math.log2(n)
+    return 0 if lg < 22.5 else (1 if lg < 24.5 else (2 if lg < 26.5 else 3))
 
 SIGNATURES = {
     "aphpois_desc_default": (None, [C.POINTER(Desc)]),

@@ -56,7 +56,7 @@ void aphpois_desc_default(aphpois_desc* d) {
   d->check_every = 0;
   d->device = 0;
   d->mem_budget = 0;
-  d->mg_w = APHPOIS_MG_W_DEFAULT;
+  d->mg_w = APHPOIS_MG_W_AUTO;
 }
 
 int aphpois_create(const aphpois_desc* desc, aphpois** out) {

@@ -158,7 +158,8 @@ int Engine::init(int device, int method, int precision, int resmode, double tol,
   if (precond == 1 && method != kPcg)
     return fail(APHPOIS_EINVAL, "the multigrid preconditioner needs method PCG");
   pc_ = precond == 1;
-  if (mg_w < 0 || mg_w > kMaxLevels) return fail(APHPOIS_EINVAL, "mg_w out of range");
+  if (mg_w < APHPOIS_MG_W_AUTO || mg_w > kMaxLevels)
+    return fail(APHPOIS_EINVAL, "mg_w out of range");
   mg_w_ = mg_w;
   resmode_ = resmode;
   tol_ = tol;
@@ -358,8 +359,15 @@ int Engine::add_system(int nx, int ny, int nz, double h3, int* slot) {
         L.z[1] = z1;
       }
       // W levels l < w (their coarse level l + 1 below the coarsest) keep the
-      // first of two coarse corrections in zs of level l + 1
-      const int w = std::max(0, std::min(mg_w_, (int)S.lev.size() - 2));
+      // first of two coarse corrections in zs of level l + 1.  Automatic:
+      // by cell count (the fastest solve measured at 64^3, 128^3, 192^3,
+      // 256^3, 384^3, 512^3 had w = 0, 0, 1, 1, 2 (3 within 1 %), 3)
+      int wsel = mg_w_;
+      if (wsel < 0) {
+        const double lg = std::log2((double)S.n);
+        wsel = lg < 22.5 ? 0 : (lg < 24.5 ? 1 : (lg < 26.5 ? 2 : 3));
+      }
+      const int w = std::max(0, std::min(wsel, (int)S.lev.size() - 2));
       S.dev.mg_w = w;
       // the bottom kernel takes the levels from the first one below every W
       // level with at most 4096 (FP32) / 2048 (FP64) cells, or the coarsest

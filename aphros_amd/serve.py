@@ -9,7 +9,8 @@ from . import _lib as L
 
 class Server:
     def __init__(self, precision=L.MIXED, tol=1e-8, maxiter=200000, delta=0.0, check_every=0,
-                 device=0, mem_budget=0, max_active=0, precond=L.PRECOND_JACOBI, mg_w=L.MG_W_DEFAULT):
+                 device=0, mem_budget=0, max_active=0, precond=L.PRECOND_JACOBI,
+                 mg_w=L.MG_W_AUTO):
         d = L.ServerDesc()
         d.precond = precond
         d.mg_w = mg_w

@@ -16,7 +16,7 @@ class Poisson:
     def __init__(self, shape, method=L.PCG, precision=L.FP64, resmode=L.RES_REL_TRUE,
                  tol=1e-8, miniter=0, maxiter=100000, h3=None, nullspace=L.NULL_AUTO,
                  delta=0.0, check_every=0, device=0, mem_budget=0, precond=L.PRECOND_JACOBI,
-                 mg_w=L.MG_W_DEFAULT):
+                 mg_w=L.MG_W_AUTO):
         nz, ny, nx = shape
         self.shape = (nz, ny, nx)
         d = L.Desc()

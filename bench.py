@@ -381,7 +381,7 @@ def main():
         t = time.perf_counter()
         if precond == L.PRECOND_MG:
             _, it, _, setup = O.mgpcg(e4, tol=0.0, maxiter=-args.cpu_iters, remove_mean=False,
-                                      w=L.MG_W_DEFAULT)
+                                      w=L.mg_w_auto(ncell))
             name = "multigrid PCG"
         else:
             _, it, _ = O.pcg_jacobi(e4, tol=0.0, maxiter=-args.cpu_iters, remove_mean=False)
@@ -394,7 +394,8 @@ def main():
                          f"system ({dt:.1f}s: setup {setup:.1f}s, {s_per_iter:.3f} s/iter), "
                          f"scaled to the {ref_iters} iterations the FP64 GPU solve needs"}
 
-    pc_name = (f"multigrid (V(2,2), W on the top {L.MG_W_DEFAULT} levels)"
+    pc_name = (f"multigrid (V(2,2), W on the top {L.mg_w_auto(ncell)} levels, "
+               f"the automatic choice at {ncell} cells)"
                if precond == L.PRECOND_MG else "Jacobi")
     form = ("mixed-precision (FP32 inner, FP64 reliable updates)" if main_prec == L.MIXED
             else "FP64")

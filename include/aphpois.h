@@ -79,10 +79,14 @@ enum {
   APHPOIS_NULL_ON = 2
 };
 
-/* W levels of the multigrid cycle by default: at 512^3 (9 levels) the
- * mixed-precision solve to 1e-8 takes 117 iterations with a V-cycle and 44
- * with three W levels, at a third more work per iteration. */
+/* W levels of the multigrid cycle on the largest grids: at 512^3 (9 levels)
+ * the mixed-precision solve to 1e-8 takes 117 iterations with a V-cycle and
+ * 35 with three W levels, at a third more work per iteration. */
 #define APHPOIS_MG_W_DEFAULT 3
+/* mg_w chosen per system from its cell count n (the default): a W level's
+ * extra coarse cycles cost launch latency, the iterations it saves cost
+ * bandwidth, so 0 for n < 2^22.5, 1 below 2^24.5, 2 below 2^26.5, else 3 */
+#define APHPOIS_MG_W_AUTO (-1)
 
 typedef struct aphpois_desc {
   int nx, ny, nz;   /* inner cells; the grid is periodic in x, y and z */
@@ -101,7 +105,7 @@ typedef struct aphpois_desc {
   int precond;      /* PCG preconditioner: APHPOIS_PRECOND_* */
   int mg_w;         /* multigrid: the top mg_w levels apply their coarse
                        correction twice (0 = V-cycle); default
-                       APHPOIS_MG_W_DEFAULT */
+                       APHPOIS_MG_W_AUTO */
 } aphpois_desc;
 
 typedef struct aphpois_info {
@@ -267,7 +271,10 @@ typedef struct aphpois_batch_desc {
   long long mem_budget; /* bytes, 0 = no limit */
   int max_systems;      /* slot capacity, 0 = 1024 */
   int precond;          /* APHPOIS_PRECOND_* */
-  int mg_w;             /* as in aphpois_desc */
+  int mg_w;             /* as in aphpois_desc (AUTO: by each system's size);
+                           a batch shares its launches among its systems, so
+                           its W levels cost little latency per system and
+                           APHPOIS_MG_W_DEFAULT suits batches of small ones */
 } aphpois_batch_desc;
 
 int aphpois_batch_create(const aphpois_batch_desc* d, aphpois_batch** out);

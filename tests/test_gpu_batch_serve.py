@@ -27,7 +27,7 @@ def _sys(n, ns, rmin, rmax, ratio, seed):
 
 
 def _single(E, precision=L.MIXED, x0=None, tol=1e-8, precond=L.PRECOND_JACOBI,
-            mg_w=L.MG_W_DEFAULT):
+            mg_w=L.MG_W_AUTO):
     s = Poisson(E.shape[:3], method=L.PCG, precision=precision, resmode=L.RES_REL_TRUE,
                 tol=tol, maxiter=50000, precond=precond, mg_w=mg_w)
     s.set_system(E)
@@ -68,7 +68,7 @@ def test_batch_fused_and_plain_levels(precision):
     b.solve()
     for E, sl in zip(systems, slots):
         xb, ib = b.get(sl)
-        xs, is_ = _single(E, precision, precond=L.PRECOND_MG)
+        xs, is_ = _single(E, precision, precond=L.PRECOND_MG, mg_w=L.MG_W_DEFAULT)
         print(f"n={E.shape[0]} iter={ib['iter']} relres={ib['true_relres']:.2e}", flush=True)
         assert ib["converged"] == 1 and ib["iter"] == is_["iter"]
         assert np.array_equal(xb, xs)

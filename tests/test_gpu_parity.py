@@ -222,7 +222,7 @@ def test_pcg_large_fused_post_true_residual(c2):
     multigrid PCG solve at 256^3 reaches the FP64 true residual (recomputed on
     the CPU) within the oracle's FP64 multigrid PCG iteration count + 2."""
     s = Poisson(c2.shape[:3], method=L.PCG, precision=L.MIXED, resmode=L.RES_REL_TRUE,
-                tol=1e-8, maxiter=20000, precond=L.PRECOND_MG)
+                tol=1e-8, maxiter=20000, precond=L.PRECOND_MG, mg_w=L.MG_W_DEFAULT)
     s.set_system(c2)
     x, info = s.solve(None)
     assert info["converged"] == 1 and info["status"] == 0
@@ -231,6 +231,27 @@ def test_pcg_large_fused_post_true_residual(c2):
     _say(f"  iter={info['iter']} oracle_iter={ito} relres={rel:.3e}")
     assert rel <= 1e-8 * (1 + 1e-6)
     assert info["iter"] <= ito + 2
+
+
+@pytest.mark.parametrize("name,w", [("c1", 0), ("c2", 1)])
+def test_mg_w_auto_picks_by_size(request, name, w):
+    """The default W levels (APHPOIS_MG_W_AUTO) are the engine's choice by
+    cell count: 0 at 64^3, 1 at 256^3; the default solve equals the solve
+    with that mg_w bit for bit."""
+    e = request.getfixturevalue(name)
+    assert L.mg_w_auto(e.shape[0] * e.shape[1] * e.shape[2]) == w
+    out = []
+    for mw in (L.MG_W_AUTO, w):
+        s = Poisson(e.shape[:3], method=L.PCG, precision=L.MIXED, resmode=L.RES_REL_TRUE,
+                    tol=1e-8, maxiter=20000, precond=L.PRECOND_MG, mg_w=mw)
+        s.set_system(e)
+        x, info = s.solve(None)
+        s.close()
+        assert info["converged"] == 1
+        out.append((x, info["iter"]))
+    _say(f"  {name}: auto iter={out[0][1]} w={w} iter={out[1][1]}")
+    assert out[0][1] == out[1][1]
+    assert np.array_equal(out[0][0], out[1][0])
 
 
 @pytest.mark.parametrize("w", [0, 2])
