@@ -649,6 +649,94 @@ struct LevelView {
   }
 };
 
+// Explicit address spaces for the bottom kernel: its level data sit in LDS
+// (or, for levels too large for it, in global memory) behind generic
+// pointers, which would otherwise compile to flat loads with 64-bit
+// addresses.
+template <typename T>
+using lds_cp = const __attribute__((address_space(3))) T*;
+template <typename T>
+using lds_p = __attribute__((address_space(3))) T*;
+template <typename T>
+using glb_cp = const __attribute__((address_space(1))) T*;
+template <typename T>
+using glb_p = __attribute__((address_space(1))) T*;
+
+// A level of the bottom kernel with 32-bit indices: the operator and
+// operation order of LevelView, periodic neighbours from the flat index.
+// CL: coefficients, mass and right-hand side in LDS (else global memory).
+template <typename T, bool CL>
+struct LdsView {
+  template <typename U>
+  using CP = std::conditional_t<CL, lds_cp<U>, glb_cp<U>>;
+  template <typename U>
+  using P = std::conditional_t<CL, lds_p<U>, glb_p<U>>;
+  using E = std::conditional_t<sizeof(T) == 4, unsigned short, T>;
+  int nx, ny, nz, pl, n;
+  CP<E> a[4];
+  CP<T> m;
+  bool hasm;
+  P<T> r;
+  __device__ explicit LdsView(const MgLevel& L)
+      : nx(L.nx), ny(L.ny), nz(L.nz), pl(L.nx * L.ny), n((int)L.n), hasm(L.m != nullptr) {
+    if constexpr (sizeof(T) == 4) {
+      a[0] = a[1] = a[2] = a[3] = (CP<E>)(const E*)L.kc;
+    } else {
+      a[0] = (CP<E>)(const E*)L.kx;
+      a[1] = (CP<E>)(const E*)L.ky;
+      a[2] = (CP<E>)(const E*)L.kz;
+      a[3] = (CP<E>)(const E*)L.dinv;
+    }
+    m = hasm ? (CP<T>)(const T*)L.m : CP<T>{};
+    r = (P<T>)(T*)L.r;
+  }
+  __device__ T k(int c, int i) const {
+    if constexpr (sizeof(T) == 4) {
+      return __uint_as_float((unsigned)a[0][((i >> 2) << 4) + 4 * c + (i & 3)] << 16);
+    } else {
+      return a[c][i];
+    }
+  }
+  __device__ T dinv(int i) const { return k(3, i); }
+  // (L u)_c of cell c = (x, y, z)
+  __device__ T lap(int c, int x, int y, int z, lds_cp<T> u) const {
+    const int xm = x == 0 ? c + nx - 1 : c - 1, xp = x == nx - 1 ? c - nx + 1 : c + 1;
+    const int ym = y == 0 ? c + pl - nx : c - nx, yp = y == ny - 1 ? c - pl + nx : c + nx;
+    const int zm = z == 0 ? c + n - pl : c - pl, zp = z == nz - 1 ? c - n + pl : c + pl;
+    const T uc = u[c];
+    T s = k(0, c) * (uc - u[xp]) + k(0, xm) * (uc - u[xm]) + k(1, c) * (uc - u[yp]) +
+          k(1, ym) * (uc - u[ym]) + k(2, c) * (uc - u[zp]) + k(2, zm) * (uc - u[zm]);
+    if (hasm) s += m[c] * uc;
+    return s;
+  }
+};
+
+// f(i, x, y, z) for the cells i = threadIdx.x + j * blockDim.x < n of an
+// (nx, ny, .) grid; the coordinates advance by the block's stride without
+// divisions.
+template <class F>
+__device__ __forceinline__ void for_cells(int n, int nx, int ny, F f) {
+  const int nt = blockDim.x, pl = nx * ny;
+  int i = threadIdx.x;
+  if (i >= n) return;
+  int z = i / pl, y = (i - z * pl) / nx, x = i - z * pl - y * nx;
+  const int sz = nt / pl, sy = (nt - sz * pl) / nx, sx = nt - sz * pl - sy * nx;
+  for (; i < n; i += nt) {
+    f(i, x, y, z);
+    x += sx;
+    y += sy;
+    z += sz;
+    if (x >= nx) {
+      x -= nx;
+      ++y;
+    }
+    if (y >= ny) {
+      y -= ny;
+      ++z;
+    }
+  }
+}
+
 // Cell of a one-cell-per-thread stencil tile of level grid (nx, ny, nz).
 struct Cell {
   int x, y, za, zb;
@@ -1473,40 +1561,53 @@ __global__ __launch_bounds__(kBottomThreads) void k_mg_bottom(const SysDev* __re
   int lw = l0;
   while (lw <= top && lv[lw].n > 64) ++lw;
   // out = u + om D^-1 (r - L u) on level l, u and out in LDS
-  auto sweep = [&](int l, const T* u, T* out, T om) {
-    const LevelView<T> V(lv[l]);
-    const T* __restrict__ r = (const T*)lv[l].r;
-    const int n = (int)lv[l].n;
-    for (int i = tid; i < n; i += nt) {
-      const int x = i % V.nx, y = (i / V.nx) % V.ny, z = i / (V.nx * V.ny);
-      auto uf = [&](long long j) -> T { return u[j]; };
-      out[i] = u[i] + om * V.dinv(i) * (r[i] - V.lap(x, y, z, uf));
+  // f(view) with the level's data in LDS or in global memory
+  auto with_view = [&](int l, auto f) {
+    if (full) {
+      f(LdsView<T, true>(lv[l]));
+    } else {
+      f(LdsView<T, false>(lv[l]));
     }
+  };
+  auto sweep = [&](int l, const T* u_, T* out_, T om) {
+    const lds_cp<T> u = (lds_cp<T>)u_;
+    const lds_p<T> out = (lds_p<T>)out_;
+    with_view(l, [&](const auto& V) {
+      for_cells(V.n, V.nx, V.ny, [&](int i, int x, int y, int z) {
+        out[i] = u[i] + om * V.dinv(i) * (V.r[i] - V.lap(i, x, y, z, u));
+      });
+    });
     __syncthreads();
   };
   for (int l = l0; l < lw && l < top; ++l) {
-    const LevelView<T> V(lv[l]);
-    const T* __restrict__ r = (const T*)lv[l].r;
-    const int n = (int)lv[l].n;
-    for (int i = tid; i < n; i += nt) zb[l][i] = wa * V.dinv(i) * r[i];
+    const lds_p<T> zal = (lds_p<T>)za[l];
+    const lds_p<T> zbl = (lds_p<T>)zb[l];
+    with_view(l, [&](const auto& V) {
+      for (int i = tid; i < V.n; i += nt) zbl[i] = wa * V.dinv(i) * V.r[i];
+    });
     __syncthreads();
     sweep(l, zb[l], za[l], wb);
-    // restriction of r - L z into r of level l + 1
-    const MgLevel& Lc = lv[l + 1];
-    T* __restrict__ rc = (T*)Lc.r;
-    const T* zl = za[l];
-    auto uf = [&](long long j) -> T { return zl[j]; };
-    for (int c = tid; c < (int)Lc.n; c += nt) {
-      const int X = c % Lc.nx, Y = (c / Lc.nx) % Lc.ny, Z = c / (Lc.nx * Lc.ny);
-      T sum = 0;
-      for (int dz = 0; dz < 2; ++dz)
-        for (int dy = 0; dy < 2; ++dy)
-          for (int dx = 0; dx < 2; ++dx) {
-            const int fx = 2 * X + dx, fy = 2 * Y + dy, fz = 2 * Z + dz;
-            sum += r[V.at(fx, fy, fz)] - V.lap(fx, fy, fz, uf);
-          }
-      rc[c] = sum;
-    }
+    // restriction of r - L z into r of level l + 1: the residual of every
+    // cell into the free second buffer, then the sums over the 2x2x2
+    // children in a fixed order
+    with_view(l, [&](const auto& V) {
+      for_cells(V.n, V.nx, V.ny, [&](int i, int x, int y, int z) {
+        zbl[i] = V.r[i] - V.lap(i, x, y, z, (lds_cp<T>)zal);
+      });
+    });
+    __syncthreads();
+    const MgLevel& Lf = lv[l];
+    with_view(l + 1, [&](const auto& C) {
+      const int fnx = Lf.nx, fpl = Lf.nx * Lf.ny;
+      for_cells(C.n, C.nx, C.ny, [&](int c, int X, int Y, int Z) {
+        const int f0 = 2 * Z * fpl + 2 * Y * fnx + 2 * X;
+        T sum = 0;
+        for (int dz = 0; dz < 2; ++dz)
+          for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx) sum += zbl[f0 + dz * fpl + dy * fnx + dx];
+        C.r[c] = sum;
+      });
+    });
     __syncthreads();
   }
   if (lw <= top) {
@@ -1553,12 +1654,13 @@ __global__ __launch_bounds__(kBottomThreads) void k_mg_bottom(const SysDev* __re
     __syncthreads();
   } else {
     // a coarsest level of more than 64 cells: block-wide sweeps
-    const LevelView<T> V(lv[top]);
-    const T* __restrict__ r = (const T*)lv[top].r;
     const int n = (int)lv[top].n;
     T* a = za[top];
     T* b = zb[top];
-    for (int i = tid; i < n; i += nt) a[i] = w * V.dinv(i) * r[i];
+    with_view(top, [&](const auto& V) {
+      const lds_p<T> al = (lds_p<T>)a;
+      for (int i = tid; i < n; i += nt) al[i] = w * V.dinv(i) * V.r[i];
+    });
     __syncthreads();
     for (int k = 1; k < sweeps; ++k) {
       sweep(top, a, b, w);
@@ -1571,23 +1673,27 @@ __global__ __launch_bounds__(kBottomThreads) void k_mg_bottom(const SysDev* __re
       __syncthreads();
     }
   }
+  // the post-smoothed correction of level l ends in zb[l] (bit l of inb;
+  // the levels solved above end in za)
+  unsigned inb = 0;
   for (int l = (lw <= top ? lw : top) - 1; l >= l0; --l) {
     const MgLevel& Lf = lv[l];
     const MgLevel& Lc = lv[l + 1];
     const int n = (int)Lf.n;
-    for (int i = tid; i < n; i += nt) {
-      const int x = i % Lf.nx, y = (i / Lf.nx) % Lf.ny, z = i / (Lf.nx * Lf.ny);
-      zb[l][i] = za[l][i] + (T)kProlong * za[l + 1][((long long)(z >> 1) * Lc.ny + (y >> 1)) * Lc.nx +
-                                                 (x >> 1)];
-    }
+    const lds_cp<T> zc = (lds_cp<T>)(((inb >> (l + 1)) & 1u) ? zb[l + 1] : za[l + 1]);
+    const lds_cp<T> zf = (lds_cp<T>)za[l];
+    const lds_p<T> u = (lds_p<T>)zb[l];
+    for_cells(n, Lf.nx, Lf.ny, [&](int i, int x, int y, int z) {
+      u[i] = zf[i] + (T)kProlong * zc[((z >> 1) * Lc.ny + (y >> 1)) * Lc.nx + (x >> 1)];
+    });
     __syncthreads();
     sweep(l, zb[l], za[l], wb);
     sweep(l, za[l], zb[l], wa);
-    for (int i = tid; i < n; i += nt) za[l][i] = zb[l][i];
-    __syncthreads();
+    inb |= 1u << l;
   }
+  const T* __restrict__ res = ((inb >> l0) & 1u) ? zb[l0] : za[l0];
   T* out = (T*)s.lev[l0].z[l0 == top ? 0 : 1];  // global
-  for (int i = tid; i < (int)lv[l0].n; i += nt) out[i] = za[l0][i];
+  for (int i = tid; i < (int)lv[l0].n; i += nt) out[i] = res[i];
 }
 
 // (r, z) of the preconditioned CG: beta = (r,z) / (r,z)_old (0 after a start)
