@@ -20,20 +20,61 @@ namespace aph {
 template <typename T, int N>
 using vec_t = T __attribute__((ext_vector_type(N)));
 
+// Global memory through pointers taken from the system tables: those are
+// generic pointers, and accesses through them would compile to flat
+// instructions (64-bit addresses in VGPRs, also counted by the LDS wait
+// counter).  The helpers below address global memory explicitly; they are
+// only used on global arrays.
+template <typename T>
+using gcp = const __attribute__((address_space(1))) T*;
+template <typename T>
+using gmp = __attribute__((address_space(1))) T*;
+
 template <typename T, int VX>
 __device__ __forceinline__ vec_t<double, VX> ldd(const T* __restrict__ p, long long i) {
-  const vec_t<T, VX> v = *reinterpret_cast<const vec_t<T, VX>*>(p + i);
+  const vec_t<T, VX> v = *(gcp<vec_t<T, VX>>)(p + i);
   return __builtin_convertvector(v, vec_t<double, VX>);
 }
 
 template <typename T, int VX>
 __device__ __forceinline__ void std_(T* __restrict__ p, long long i, vec_t<double, VX> v) {
-  *reinterpret_cast<vec_t<T, VX>*>(p + i) = __builtin_convertvector(v, vec_t<T, VX>);
+  *(gmp<vec_t<T, VX>>)(p + i) = __builtin_convertvector(v, vec_t<T, VX>);
 }
 
 template <typename T, int VX>
 __device__ __forceinline__ vec_t<T, VX> ldv(const T* __restrict__ p, long long i) {
-  return *reinterpret_cast<const vec_t<T, VX>*>(p + i);
+  return *(gcp<vec_t<T, VX>>)(p + i);
+}
+
+template <typename T, int VX>
+__device__ __forceinline__ void stv(T* __restrict__ p, long long i, vec_t<T, VX> v) {
+  *(gmp<vec_t<T, VX>>)(p + i) = v;
+}
+
+template <typename T>
+__device__ __forceinline__ T ld1(const T* __restrict__ p, long long i) {
+  return *(gcp<T>)(p + i);
+}
+
+template <typename T>
+__device__ __forceinline__ void st1(T* __restrict__ p, long long i, T v) {
+  *(gmp<T>)(p + i) = v;
+}
+
+// A copy of a table entry that no kernel writes while the reader runs
+// (system, level and launch tables), read through the constant address
+// space: uniform scalar loads into SGPRs, so the pointers it holds need no
+// VGPRs and the loads through them can take a scalar base.
+template <typename T>
+__device__ __forceinline__ T cload(const T* p) {
+  static_assert(sizeof(T) % 4 == 0 && std::is_trivially_copyable<T>::value, "dword copy");
+  T v;
+  const __attribute__((address_space(4))) unsigned* src =
+      (const __attribute__((address_space(4))) unsigned*)p;
+  unsigned* dst = reinterpret_cast<unsigned*>(&v);
+#pragma unroll
+  for (int k = 0; k < (int)(sizeof(T) / 4); ++k) dst[k] = src[k];
+  return v;
 }
 
 // bfloat16 (the upper 16 bits of an IEEE float): the V-cycle coefficients of
@@ -70,6 +111,7 @@ struct MgCoefs {
     return ldv<T, VX>(a[c], i);
   }
   __device__ T comp1(int c, long long i) const { return a[c][i]; }
+  __device__ T comp1g(int c, long long i) const { return ld1(a[c], i); }
   template <int VX>
   __device__ void own(long long i, vec_t<T, VX>& kx, vec_t<T, VX>& ky, vec_t<T, VX>& kz,
                       vec_t<T, VX>& d) const {
@@ -90,27 +132,28 @@ struct MgCoefs<float> {
     vec_t<float, VX> r;
     const unsigned short* p = kc + at(c, i);
     if constexpr (VX == 4) {
-      const uint2 w = *reinterpret_cast<const uint2*>(p);
+      const vec_t<unsigned, 2> w = *(gcp<vec_t<unsigned, 2>>)p;
       r[0] = __uint_as_float(w.x << 16);
       r[1] = __uint_as_float(w.x & 0xffff0000u);
       r[2] = __uint_as_float(w.y << 16);
       r[3] = __uint_as_float(w.y & 0xffff0000u);
     } else if constexpr (VX == 2) {
-      const unsigned w = *reinterpret_cast<const unsigned*>(p);
+      const unsigned w = *(gcp<unsigned>)p;
       r[0] = __uint_as_float(w << 16);
       r[1] = __uint_as_float(w & 0xffff0000u);
     } else {
-      r[0] = bf2f(p[0]);
+      r[0] = bf2f(ld1(p, 0));
     }
     return r;
   }
   __device__ float comp1(int c, long long i) const { return bf2f(kc[at(c, i)]); }
+  __device__ float comp1g(int c, long long i) const { return bf2f(ld1(kc, at(c, i))); }
   template <int VX>
   __device__ void own(long long i, vec_t<float, VX>& kx, vec_t<float, VX>& ky,
                       vec_t<float, VX>& kz, vec_t<float, VX>& d) const {
     if constexpr (VX == 4) {
-      const uint4* g = reinterpret_cast<const uint4*>(kc + ((i >> 2) << 4));
-      const uint4 w0 = g[0], w1 = g[1];
+      const gcp<vec_t<unsigned, 4>> g = (gcp<vec_t<unsigned, 4>>)(kc + ((i >> 2) << 4));
+      const vec_t<unsigned, 4> w0 = g[0], w1 = g[1];
       kx[0] = __uint_as_float(w0.x << 16);
       kx[1] = __uint_as_float(w0.x & 0xffff0000u);
       kx[2] = __uint_as_float(w0.y << 16);
@@ -886,7 +929,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
   auto u1 = [&](int xx, int yy, int zz) -> T {
     const long long i = (long long)zz * pl + (long long)yy * nx + xx;
     if constexpr (MODE == 1) {
-      return wa * co.comp1(3, i) * r[i];
+      return wa * co.comp1g(3, i) * ld1(r, i);
     } else if constexpr (MODE == 2) {
       const long long ci = ((long long)(zz >> 1) * cny + (yy >> 1)) * cnx + (xx >> 1);
       return zi[i] + (T)kProlong * zc[ci];
@@ -957,7 +1000,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
       {
         const T uxm = xm_in ? sp[lofs - 1] : u1(xm, y, z);
         const T uxp = xp_in ? sp[lofs + VX] : u1(xp, y, z);
-        const T kxm = co.comp1(0, base + row + xm);
+        const T kxm = co.comp1g(0, base + row + xm);
 #pragma unroll
         for (int v = 0; v < VX; ++v) {
           const T up = v + 1 < VX ? uc[(v + 1) % VX] : uxp;
@@ -982,7 +1025,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
         if constexpr (MODE == 0) {
           if (zsa) out += ldv<T, VX>(zsa, ic);
         }
-        *reinterpret_cast<V*>(zo + ic) = out;
+        stv<T, VX>(zo, ic, out);
         if constexpr (DOT) {
 #pragma unroll
           for (int v = 0; v < VX; ++v) dot += (double)rv[v] * (double)out[v];
@@ -1085,13 +1128,13 @@ __global__ __launch_bounds__(kPostBlock) void k_mg_post(const SysDev* __restrict
   __shared__ T su[2][kPostBlock * VX];
   __shared__ T sv[2][kPostBlock * VX];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const Rec rc = recs[b2r[b]];
-  const SysDev& s = sds[rc.sys];
+  const Rec rc = cload(recs + cload(b2r + b));
+  const SysDev s = cload(sds + rc.sys);
   if (s.sc->done) return;
   if (wl >= 0 && wl >= s.mg_w) return;
   if (so && !s.sc->first) return;
-  const MgLevel& Lv = s.lev[l];
-  const MgLevel& Lc = s.lev[l + 1];
+  const MgLevel Lv = cload(s.lev + l);
+  const MgLevel Lc = cload(s.lev + l + 1);
   const int nx = Lv.nx, ny = Lv.ny, nz = Lv.nz;
   const int lb = b - rc.blk0;
   const int lx = rc.lx, ty = rc.ty;  // lx * VX == nx, ty <= kPostBlock / lx
@@ -1157,7 +1200,7 @@ __global__ __launch_bounds__(kPostBlock) void k_mg_post(const SysDev* __restrict
     }
     l7 += kyc * (w - wy);
     const T wxm = sp[lxm], wxp = sp[lxp];
-    const T kxm = co.comp1(0, base + row + xm);
+    const T kxm = co.comp1g(0, base + row + xm);
 #pragma unroll
     for (int v = 0; v < VX; ++v) {
       const T up = v + 1 < VX ? w[(v + 1) % VX] : wxp;
@@ -1215,7 +1258,7 @@ __global__ __launch_bounds__(kPostBlock) void k_mg_post(const SysDev* __restrict
       const V lv = lap(sv[(st + 1) & 1], base, vc, vm, vn, kx1, ky1, kz1, kzA, none);
       V out = vc + wa * d1 * (r1 - lv);
       if (zsa) out += ldv<T, VX>(zsa, ic);
-      *reinterpret_cast<V*>(zo + ic) = out;
+      stv<T, VX>(zo, ic, out);
       if constexpr (DOT) {
 #pragma unroll
         for (int v = 0; v < VX; ++v) dot += (double)r1[v] * (double)out[v];
@@ -1262,13 +1305,13 @@ __global__ __launch_bounds__(kPostBlock) void k_mg_pre(const SysDev* __restrict_
   __shared__ T sz[2][kPostBlock * VX];
   __shared__ T sacc[kPostBlock * NP];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const Rec rc = recs[b2r[b]];
-  const SysDev& s = sds[rc.sys];
+  const Rec rc = cload(recs + cload(b2r + b));
+  const SysDev s = cload(sds + rc.sys);
   if (s.sc->done) return;
   if (wl >= 0 && wl >= s.mg_w) return;
   if (so && !s.sc->first) return;
-  const MgLevel& Lv = s.lev[l];
-  const MgLevel& Lc = s.lev[l + 1];
+  const MgLevel Lv = cload(s.lev + l);
+  const MgLevel Lc = cload(s.lev + l + 1);
   const int nx = Lv.nx, ny = Lv.ny, nz = Lv.nz;
   const int lb = b - rc.blk0;
   const int lx = rc.lx, ty = rc.ty;  // lx * VX == nx, ty even, <= kPostBlock / lx
@@ -1320,7 +1363,7 @@ __global__ __launch_bounds__(kPostBlock) void k_mg_pre(const SysDev* __restrict_
     }
     l7 += kyc * (w - wy);
     const T wxm = sp[lxm], wxp = sp[lxp];
-    const T kxm = co.comp1(0, base + row + xm);
+    const T kxm = co.comp1g(0, base + row + xm);
 #pragma unroll
     for (int v = 0; v < VX; ++v) {
       const T up = v + 1 < VX ? w[(v + 1) % VX] : wxp;
@@ -1358,8 +1401,8 @@ __global__ __launch_bounds__(kPostBlock) void k_mg_pre(const SysDev* __restrict_
     if (kpend >= 0) {
 #pragma unroll
       for (int j = 0; j < NP; ++j)
-        rco[((long long)kpend * cny + (y >> 1)) * cnx + (x0 >> 1) + j] =
-            rpend[j] + sacc[(threadIdx.x + lx) * NP + j];
+        st1(rco, ((long long)kpend * cny + (y >> 1)) * cnx + (x0 >> 1) + j,
+            rpend[j] + sacc[(threadIdx.x + lx) * NP + j]);
       kpend = -1;
     }
     V zn = (T)0, kxc = (T)0, kyc = (T)0, kzc = (T)0, rk = (T)0;
@@ -1382,7 +1425,7 @@ __global__ __launch_bounds__(kPostBlock) void k_mg_pre(const SysDev* __restrict_
       zn = uc + wb * dc * (rcur - lu);
 #pragma unroll
       for (int v = 0; v < VX; ++v) sz[st & 1][lofs + v] = zn[v];
-      if (outrow && k >= za && k < zb) *reinterpret_cast<V*>(zo + ic) = zn;
+      if (outrow && k >= za && k < zb) stv<T, VX>(zo, ic, zn);
       rk = rcur;
       kzB = kzc;
       um = uc;
@@ -1425,8 +1468,8 @@ __global__ __launch_bounds__(kPostBlock) void k_mg_pre(const SysDev* __restrict_
   if (kpend >= 0) {
 #pragma unroll
     for (int j = 0; j < NP; ++j)
-      rco[((long long)kpend * cny + (y >> 1)) * cnx + (x0 >> 1) + j] =
-          rpend[j] + sacc[(threadIdx.x + lx) * NP + j];
+      st1(rco, ((long long)kpend * cny + (y >> 1)) * cnx + (x0 >> 1) + j,
+          rpend[j] + sacc[(threadIdx.x + lx) * NP + j]);
   }
 }
 
