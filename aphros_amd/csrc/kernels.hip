@@ -30,24 +30,24 @@ using gcp = const __attribute__((address_space(1))) T*;
 template <typename T>
 using gmp = __attribute__((address_space(1))) T*;
 
-template <typename T, int VX>
-__device__ __forceinline__ vec_t<double, VX> ldd(const T* __restrict__ p, long long i) {
+template <typename T, int VX, typename P>
+__device__ __forceinline__ vec_t<double, VX> ldd(P p, long long i) {
   const vec_t<T, VX> v = *(gcp<vec_t<T, VX>>)(p + i);
   return __builtin_convertvector(v, vec_t<double, VX>);
 }
 
-template <typename T, int VX>
-__device__ __forceinline__ void std_(T* __restrict__ p, long long i, vec_t<double, VX> v) {
+template <typename T, int VX, typename P>
+__device__ __forceinline__ void std_(P p, long long i, vec_t<double, VX> v) {
   *(gmp<vec_t<T, VX>>)(p + i) = __builtin_convertvector(v, vec_t<T, VX>);
 }
 
-template <typename T, int VX>
-__device__ __forceinline__ vec_t<T, VX> ldv(const T* __restrict__ p, long long i) {
+template <typename T, int VX, typename P>
+__device__ __forceinline__ vec_t<T, VX> ldv(P p, long long i) {
   return *(gcp<vec_t<T, VX>>)(p + i);
 }
 
-template <typename T, int VX>
-__device__ __forceinline__ void stv(T* __restrict__ p, long long i, vec_t<T, VX> v) {
+template <typename T, int VX, typename P>
+__device__ __forceinline__ void stv(P p, long long i, vec_t<T, VX> v) {
   *(gmp<vec_t<T, VX>>)(p + i) = v;
 }
 
@@ -264,15 +264,15 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv(const SysDev* __restrict__ 
   const int nx = s.nx, ny = s.ny, nz = s.nz;
   const int cur = sc->cur;
   const double beta = sc->beta;
-  const T* __restrict__ r = (const T*)s.z;  // r (Jacobi-scaled) or z = M^-1 r
-  const T* __restrict__ pin = (const T*)s.p[cur];
-  T* __restrict__ pout = (T*)s.p[cur ^ 1];
-  T* __restrict__ q = (T*)s.q;
-  const T* __restrict__ kx = (const T*)s.kx;
-  const T* __restrict__ ky = (const T*)s.ky;
-  const T* __restrict__ kz = (const T*)s.kz;
-  const T* __restrict__ S = (const T*)s.S;
-  const T* __restrict__ mm = (const T*)s.m;
+  const gcp<T> __restrict__ r = (gcp<T>)s.z;  // r (Jacobi-scaled) or z = M^-1 r
+  const gcp<T> __restrict__ pin = (gcp<T>)s.p[cur];
+  const gmp<T> __restrict__ pout = (gmp<T>)s.p[cur ^ 1];
+  const gmp<T> __restrict__ q = (gmp<T>)s.q;
+  const gcp<T> __restrict__ kx = (gcp<T>)s.kx;
+  const gcp<T> __restrict__ ky = (gcp<T>)s.ky;
+  const gcp<T> __restrict__ kz = (gcp<T>)s.kz;
+  const gcp<T> __restrict__ S = (gcp<T>)s.S;
+  const gcp<T> __restrict__ mm = (gcp<T>)s.m;
   const int has_m = s.has_m;
   const int lx = rc.lx;
   const int tx = lx * VX;                 // tile width in cells (LDS row)
@@ -374,8 +374,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv(const SysDev* __restrict__ 
       }
       if (has_m) l += cva<A, VX>(ldv<T, VX>(mm, ic)) * uc;
       const V qv = __builtin_convertvector(cva<A, VX>(Sc) * l, V);
-      *reinterpret_cast<V*>(pout + ic) = pc;
-      *reinterpret_cast<V*>(q + ic) = qv;
+      stv<T, VX>(pout, ic, pc);
+      stv<T, VX>(q, ic, qv);
       const vec_t<double, VX> pcd = cvt<VX>(pc), qd = cvt<VX>(qv);
 #pragma unroll
       for (int v = 0; v < VX; ++v) dot += pcd[v] * qd[v];
@@ -542,14 +542,14 @@ __global__ __launch_bounds__(kBlock) void k_ru_resid(const SysDev* __restrict__ 
   const int nx = s.nx, ny = s.ny, nz = s.nz;
   const double mp = sc->mean_pending;
   const double sgn = s.neg ? -1.0 : 1.0;
-  const double* __restrict__ x = s.x64;
-  const double* __restrict__ bb = s.b64;
-  const double* __restrict__ e0 = s.s4[0];
-  const double* __restrict__ ex = s.s4[1];
-  const double* __restrict__ ey = s.s4[2];
-  const double* __restrict__ ez = s.s4[3];
-  const T* __restrict__ S = (const T*)s.S;
-  T* __restrict__ r = (T*)s.r;
+  const gcp<double> __restrict__ x = (gcp<double>)s.x64;
+  const gcp<double> __restrict__ bb = (gcp<double>)s.b64;
+  const gcp<double> __restrict__ e0 = (gcp<double>)s.s4[0];
+  const gcp<double> __restrict__ ex = (gcp<double>)s.s4[1];
+  const gcp<double> __restrict__ ey = (gcp<double>)s.s4[2];
+  const gcp<double> __restrict__ ez = (gcp<double>)s.s4[3];
+  const gcp<T> __restrict__ S = (gcp<T>)s.S;
+  const gmp<T> __restrict__ r = (gmp<T>)s.r;
   const int lx = rc.lx;
   const int ix = threadIdx.x % lx, iy = threadIdx.x / lx;
   const int ty = kBlock / lx;
