@@ -870,32 +870,32 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
   const bool ym_in = iy > 0;
   const bool yp_in = iy + 1 < ty && y + 1 < ny;
   const MgCoefs<T> co(Lv);
-  const T* __restrict__ mm = (const T*)Lv.m;
-  const T* __restrict__ r = (const T*)Lv.r;
-  const T* __restrict__ zi = (const T*)Lv.z[zin];
-  T* __restrict__ zo = (T*)Lv.z[zout];
-  const T* __restrict__ zc = nullptr;
-  T* __restrict__ rco = nullptr;
+  const gcp<T> __restrict__ mm = (gcp<T>)Lv.m;
+  const gcp<T> __restrict__ r = (gcp<T>)Lv.r;
+  const gcp<T> __restrict__ zi = (gcp<T>)Lv.z[zin];
+  const gmp<T> __restrict__ zo = (gmp<T>)Lv.z[zout];
+  gcp<T> __restrict__ zc = nullptr;
+  gmp<T> __restrict__ rco = nullptr;
   int cnx = 0, cny = 0;
   if constexpr (MODE == 2 || MODE == 3) {
     const MgLevel& Lc = s.lev[l + 1];
     if constexpr (MODE == 2) {
-      zc = (const T*)Lc.z[l + 1 == s.nlev - 1 ? 0 : 1];
+      zc = (gcp<T>)Lc.z[l + 1 == s.nlev - 1 ? 0 : 1];
     } else {
-      rco = (T*)Lc.r;
+      rco = (gmp<T>)Lc.r;
     }
     cnx = Lc.nx;
     cny = Lc.ny;
   }
-  T* __restrict__ rw = nullptr;   // MODE 4: residual written in place
-  T* __restrict__ zsv = nullptr;  // MODE 4: the correction saved
-  const T* __restrict__ zsa = nullptr;  // MODE 0: the first correction added
+  gmp<T> __restrict__ rw = nullptr;   // MODE 4: residual written in place
+  gmp<T> __restrict__ zsv = nullptr;  // MODE 4: the correction saved
+  gcp<T> __restrict__ zsa = nullptr;  // MODE 0: the first correction added
   if constexpr (MODE == 4) {
-    rw = (T*)Lv.r;
-    zsv = (T*)Lv.zs;
+    rw = (gmp<T>)Lv.r;
+    zsv = (gmp<T>)Lv.zs;
   }
   if constexpr (MODE == 0) {
-    if (addzs) zsa = (const T*)Lv.zs;
+    if (addzs) zsa = (gcp<T>)Lv.zs;
   }
   // MODE 1: zero guess (weight A) + sweep B; MODE 2: sweep B; MODE 0: sweep A
   const T wa = (T)kOmegaA;
@@ -1018,8 +1018,8 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
 #pragma unroll
         for (int j = 0; j < NP; ++j) racc[j] += res[2 * j] + res[2 * j + 1];
       } else if constexpr (MODE == 4) {
-        *reinterpret_cast<V*>(zsv + ic) = uc;
-        *reinterpret_cast<V*>(rw + ic) = rv - lap;
+        stv<T, VX>(zsv, ic, uc);
+        stv<T, VX>(rw, ic, rv - lap);
       } else {
         V out = uc + w * dc * (rv - lap);
         if constexpr (MODE == 0) {
