@@ -166,7 +166,9 @@ int Engine::init(int device, int method, int precision, int resmode, double tol,
   miniter_ = miniter;
   maxiter_ = maxiter;
   nullspace_ = nullspace;
-  delta_ = delta > 0 ? delta : 0.1;
+  // fewer reliable updates for the same iteration count: 512^3 mixed, 35
+  // iterations at 0.1 and 0.05 (0.166 / 0.160 s), 36 at 0.03 and 0.02
+  delta_ = delta > 0 ? delta : 0.05;
   check_every_ = check_every;
   budget_ = budget;
   capacity_ = capacity;

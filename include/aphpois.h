@@ -98,7 +98,7 @@ typedef struct aphpois_desc {
   int miniter;      /* Conf::miniter */
   int maxiter;      /* Conf::maxiter */
   int nullspace;    /* APHPOIS_NULL_* (PCG only) */
-  double delta;     /* reliable-update trigger (PCG), 0 = default 0.1 */
+  double delta;     /* reliable-update trigger (PCG), 0 = default 0.05 */
   int check_every;  /* iterations between host polls, 0 = auto */
   int device;       /* HIP device ordinal */
   long long mem_budget; /* bytes of device memory this handle may use, 0 = no limit */
