@@ -37,7 +37,7 @@ class Info(C.Structure):
     _fields_ = [("iter", C.c_int), ("residual", C.c_double), ("res_l2", C.c_double),
                 ("res_max", C.c_double), ("true_relres", C.c_double), ("mean", C.c_double),
                 ("converged", C.c_int), ("refinements", C.c_int), ("status", C.c_int),
-                ("seconds", C.c_double)]
+                ("seconds", C.c_double), ("max_diff", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -109,6 +109,7 @@ SIGNATURES = {
     "aphpois_solve": (C.c_int, [_vp, _dp, _dp, _ll, _ll, _ll, C.POINTER(Info)]),
     "aphpois_solve_device": (C.c_int, [_vp, _vp, _vp, C.POINTER(Info)]),
     "aphpois_set_conf": (C.c_int, [_vp, C.c_double, C.c_int, C.c_int]),
+    "aphpois_set_solution_ref": (C.c_int, [_vp, _dp, _ll, _ll, _ll]),
     "aphpois_device_bytes": (C.c_longlong, [_vp]),
     "aphpois_destroy": (C.c_int, [_vp]),
     "aphpois_last_error": (C.c_char_p, []),

@@ -12,11 +12,28 @@
 // inner cells into the rank-wide field of the lead block, the lead block
 // calls the C ABI once, every block copies the solution back and requests
 // the direct_one halo exchange, and Info reaches every block through
-// BcastFromLead.  The rank must own the whole periodic domain (one GPU per
-// domain); anything else is refused with fassert, like any other
-// configuration error of the reference.
+// BcastFromLead.  The rank must own the whole domain (one GPU per domain);
+// anything else is refused with fassert, like any other configuration error
+// of the reference.
+//
+// Resident coefficients: while gathering, every block compares its cells'
+// coefficients e0..e6 with those of the previous Solve bit for bit; when no
+// block saw a change, the lead uploads only the constant term e7
+// (aphpois_set_rhs: 8 instead of 64 bytes per cell) and the device keeps its
+// converted operator and multigrid levels.  linsolver_hip_reuse = 0 turns
+// this off.
+//
+// Walls: the C ABI treats every axis as periodic.  On a non-periodic axis the
+// reference's pressure system links no cell across the domain boundary (the
+// projection assembles wall and pressure boundary faces with a zero
+// coefficient for the outer cell, src/solver/proj.ipp:343-351), so the
+// periodic wrap only ever multiplies by zero and the solve is the same.  The
+// gather checks that those coefficients are zero and the solve refuses the
+// system otherwise.
 #pragma once
 
+#include <atomic>
+#include <cstring>
 #include <iostream>
 #include <memory>
 #include <stdexcept>
@@ -50,6 +67,8 @@ class SolverHip final : public Solver<M> {
     int true_relres = 0;
     int device = 0;
     long long mem_budget = 0;
+    // keep the device operator when only e7 changed since the last Solve
+    int reuse = 1;
   };
 
   SolverHip(const Conf& conf, const Extra& extra, const M& m)
@@ -73,15 +92,22 @@ class SolverHip final : public Solver<M> {
       // the lead block sizes the rank-wide fields before any block writes
       // into them (the next stage)
       if (m.IsLead()) {
+        Shared& s = *shared_;
         const size_t ns = ms.GetIndexCells().size();
-        if (shared_->fcs.size() != ns) shared_->fcs.Reinit(ms);
-        if (shared_->fcu.size() != ns) shared_->fcu.Reinit(ms, Scal(0));
+        if (s.fcs.size() != ns) {
+          s.fcs.Reinit(ms);
+          s.resident = false;
+        }
+        if (s.fce7.size() != ns) s.fce7.Reinit(ms, Scal(0));
+        if (s.fcu.size() != ns) s.fcu.Reinit(ms, Scal(0));
+        s.changed = !s.resident || !extra_.reuse;
+        s.wall_link = false;
       }
       m.BcastFromLead(&shared_);
     }
     if (sem("gather")) {
       Shared& s = *shared_;
-      ToShared(fc_system, s.fcs, m);
+      GatherSystem(fc_system, s, m);
       if (fc_init) {
         ToShared(*fc_init, s.fcu, m);
       }
@@ -93,21 +119,32 @@ class SolverHip final : public Solver<M> {
         fassert(
             size == ms.GetGlobalSize(),
             "linsolver hip: the rank must own the whole domain (one GPU per domain)");
-        for (size_t d = 0; d < M::dim; ++d) {
-          fassert(
-              ms.flags.is_periodic[d], "linsolver hip: the domain must be periodic");
-        }
+        fassert(
+            !s.wall_link.load(),
+            "linsolver hip: a coefficient links a cell across a non-periodic domain "
+            "boundary; the solver needs it to be zero there (walls and pressure "
+            "boundaries of the projection are)");
         if (!s.h) {
           Create(s, ms);
+          s.resident = false;
+          s.changed = true;
         }
         Check(aphpois_set_conf(s.h, this->conf.tol, this->conf.miniter, this->conf.maxiter));
         const auto& ics = ms.GetIndexCells();
         const long long start = ics.GetIdx(ms.GetInBlockCells().GetBegin()).raw();
         const long long lead_y = ics.GetSize()[0];
         const long long lead_z = lead_y * ics.GetSize()[1];
-        // Expr is 8 contiguous Scal per cell slot: the AoS the C ABI reads
-        const double* expr = &s.fcs.data()[0][0];
-        Check(aphpois_set_system(s.h, expr, start, lead_y, lead_z));
+        if (s.changed.load() || !s.resident) {
+          // Expr is 8 contiguous Scal per cell slot: the AoS the C ABI reads
+          s.resident = false;
+          const double* expr = &s.fcs.data()[0][0];
+          Check(aphpois_set_system(s.h, expr, start, lead_y, lead_z));
+          s.resident = true;
+          ++s.uploads;
+        } else {
+          Check(aphpois_set_rhs(s.h, s.fce7.data(), start, lead_y, lead_z));
+          ++s.rhs_uploads;
+        }
         aphpois_info info{};
         Check(aphpois_solve(
             s.h, fc_init ? s.fcu.data() : nullptr, s.fcu.data(), start, lead_y,
@@ -134,12 +171,30 @@ class SolverHip final : public Solver<M> {
   // rank-wide state, owned by the lead block
   struct Shared {
     aphpois* h = nullptr;
-    FieldCell<Expr> fcs;
+    FieldCell<Expr> fcs;   // the system of the last Solve
+    FieldCell<Scal> fce7;  // its constant terms alone
     FieldCell<Scal> fcu;
+    bool resident = false;  // the device holds the operator of fcs
+    // set by any block (blocks gather concurrently) and read by the lead
+    // in the next stage
+    std::atomic<bool> changed{true};    // some e0..e6 differs from fcs
+    std::atomic<bool> wall_link{false}; // nonzero coefficient across a wall
+    long long uploads = 0, rhs_uploads = 0;
     ~Shared() {
       if (h) aphpois_destroy(h);
     }
   };
+
+ public:
+  // Full system uploads and constant-term-only uploads so far (lead block).
+  long long GetUploads() const {
+    return shared_obj_ ? shared_obj_->uploads : 0;
+  }
+  long long GetRhsUploads() const {
+    return shared_obj_ ? shared_obj_->rhs_uploads : 0;
+  }
+
+ private:
 
   static void Check(int rc) {
     fassert(
@@ -169,6 +224,33 @@ class SolverHip final : public Solver<M> {
     d.device = extra_.device;
     d.mem_budget = extra_.mem_budget;
     Check(aphpois_create(&d, &s.h));
+  }
+
+  // The block's system into the rank-wide fields (a disjoint region per
+  // block): each cell's Expr, compared bit for bit with the one of the last
+  // Solve, and its constant term; coefficients that link across a
+  // non-periodic domain boundary must be zero.
+  void GatherSystem(const FieldCell<Expr>& fl, Shared& s, const M& m) {
+    const M& ms = m.GetShared();
+    const auto& ics = ms.GetIndexCells();
+    const MIdx gs = ms.GetGlobalSize();
+    bool changed = false, wall = false;
+    for (auto c : m.CellsM()) {
+      const MIdx w(c);
+      const size_t i = ics.GetIdx(w).raw();
+      const Expr& e = fl[c];
+      Expr& d = s.fcs.data()[i];
+      if (!changed && std::memcmp(&d[0], &e[0], 7 * sizeof(Scal)) != 0) changed = true;
+      d = e;
+      s.fce7.data()[i] = e[7];
+      for (size_t a = 0; a < M::dim; ++a) {
+        if (ms.flags.is_periodic[a]) continue;
+        if (w[a] == 0 && e[1 + 2 * a] != 0) wall = true;
+        if (w[a] == gs[a] - 1 && e[2 + 2 * a] != 0) wall = true;
+      }
+    }
+    if (changed) s.changed.store(true);
+    if (wall) s.wall_link.store(true);
   }
 
   // Inner cells of this block into / out of the rank-wide field (each block
@@ -209,6 +291,8 @@ class SolverHip final : public Solver<M> {
 //   linsolver_hip_precond                  jacobi | mg (default mg)
 //   linsolver_hip_true_relres              1: PCG stops on ||b - A x|| / ||b|| <= tol
 //   linsolver_hip_device                   HIP device ordinal (default 0)
+//   linsolver_hip_reuse                    1 (default): upload only e7 when the
+//                                          coefficients did not change
 template <class M>
 class ModuleLinearHip : public ModuleLinear<M> {
  public:
@@ -234,6 +318,7 @@ class ModuleLinearHip : public ModuleLinear<M> {
     extra.precond = pc == "mg" ? APHPOIS_PRECOND_MG : APHPOIS_PRECOND_JACOBI;
     extra.true_relres = var.Int("linsolver_hip_true_relres", 0);
     extra.device = var.Int("linsolver_hip_device", 0);
+    extra.reuse = var.Int("linsolver_hip_reuse", 1);
     return std::make_unique<SolverHip<M>>(this->GetConf(var, prefix), extra, m);
   }
 };

@@ -16,6 +16,10 @@ int set_err(int code, const std::string& msg) {
 
 }  // namespace
 
+namespace aph {
+int set_api_error(int code, const std::string& msg) { return set_err(code, msg); }
+}  // namespace aph
+
 struct aphpois {
   aph::Engine eng;
   aphpois_desc desc{};
@@ -118,6 +122,12 @@ int aphpois_solve_device(aphpois* h, const double* x0_dev, double* x_dev, aphpoi
   APH_CHECK_H(h);
   const auto& d = h->desc;
   return solve_impl(h, x0_dev, x_dev, true, 0, d.nx, (long long)d.nx * d.ny, info);
+}
+
+int aphpois_set_solution_ref(aphpois* h, const double* xref, long long start,
+                             long long lead_y, long long lead_z) {
+  APH_CHECK_H(h);
+  APH_RET(*h, h->eng.set_solution_ref(h->slot, xref, start, lead_y, lead_z));
 }
 
 int aphpois_set_conf(aphpois* h, double tol, int miniter, int maxiter) {

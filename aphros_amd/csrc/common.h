@@ -79,8 +79,10 @@ struct SysDev {
   double* x64;
   const double* b64;
   const double* s4[4];
-  // reference-mode storage
+  // reference-mode storage: component k of cell c at f7[k][c << f7sh]
+  // (f7sh 0: eight SoA arrays; 3: f7[k] = expr + k of a dense AoS Expr)
   const double* f7[8];
+  int f7sh;
   double* u[2];
   double* rr;
   double* pp[2];

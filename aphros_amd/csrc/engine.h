@@ -9,6 +9,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <memory>
 #include <string>
 #include <vector>
@@ -25,6 +26,16 @@ struct Tuning {
   int vx = 0;  // cells per thread in x (PCG), 0 = automatic
   int sm = 0;  // cells per thread of the streaming kernels, 0 = automatic
 };
+
+// Sets the message of aphpois_last_error() on the calling thread; returns code.
+int set_api_error(int code, const std::string& msg);
+
+// Launch tiling of one system: the stencil launch of the iteration kernel
+// (vx cells per thread in x for PCG, 1 otherwise), the streaming launches and
+// the one-cell-per-thread residual launches.  sys and blk0 are left to the
+// caller.
+void system_tiling(int method, int nx, int ny, int nz, int vx, const Tuning& tu, Rec* st,
+                   Rec* sm, Rec* ru);
 
 struct System {
   int nx = 0, ny = 0, nz = 0;
@@ -43,6 +54,8 @@ struct System {
   bool done_reported = true;     // the last solve's result has been collected
   std::vector<void*> allocs;
   long long bytes = 0;
+  double* xref = nullptr;    // reference solution for info.max_diff (or null)
+  double* xref_store = nullptr;  // its device storage (kept once allocated)
   void* partials = nullptr;  // reduction scratch, outside the budget
   long long partials_cap = 0;
   SysDev dev{};
@@ -61,6 +74,11 @@ class Engine {
            int maxiter, int nullspace, double delta, int check_every, long long budget,
            int capacity, int precond = 0, int mg_w = 0);
   bool multigrid() const { return pc_; }
+  // A budget shared with other engines on the same device (the server's
+  // size-class lanes): every byte counted against the budget is reserved in
+  // *used as well, so the engines' sum stays within budget.  Call before the
+  // first allocation.
+  void share_budget(std::atomic<long long>* used) { shared_ = used; }
   int add_system(int nx, int ny, int nz, double h3, int* slot);
   int set_system(int slot, const double* expr, bool on_device, long long start, long long ly,
                  long long lz);
@@ -79,6 +97,10 @@ class Engine {
   // from the resident solution.
   int prepare(int slot, const double* x0, bool on_device, long long start, long long ly,
               long long lz, bool keep_x);
+  // Reference solution of a slot (host strided field; nullptr removes it):
+  // every later solve reports max |x - xref| (mean removed when singular).
+  int set_solution_ref(int slot, const double* xref, long long start, long long ly,
+                       long long lz);
   int run();  // solves every prepared system
   int fetch(int slot, double* x, bool on_device, long long start, long long ly, long long lz,
             aphpois_info* info);
@@ -150,11 +172,16 @@ class Engine {
  private:
   int fail(int code, const std::string& msg);
   int hip(hipError_t e, const char* what);
+  // budget accounting: reserve() fails if nbytes more would exceed it
+  bool reserve(long long nbytes);
+  void release(long long nbytes);
   int alloc(System& s, void** p, long long bytes);
   int build_tables();
   int capture();
   int pull_scalars();
   void fill_info(int slot, double seconds);
+  // info.max_diff of a finished slot against its reference solution
+  int diff_info(int slot);
   bool need_start_ = false;
   int upload_strided(const double* src, bool on_device, long long start, long long ly,
                      long long lz, int nx, int ny, int nz, int comps, const double** dptr);
@@ -176,6 +203,7 @@ class Engine {
   double delta_ = 0.1;
   int check_every_ = 0;
   long long budget_ = 0;
+  std::atomic<long long>* shared_ = nullptr;
   int capacity_ = 0;
   int vx_ = 1;
   Tuning tuning_;

@@ -89,6 +89,27 @@ bool make_post_rec(int nx, int ny, int nz, int vx, int ncu, Rec* out) {
 
 }  // namespace
 
+void system_tiling(int method, int nx, int ny, int nz, int vx, const Tuning& tu, Rec* st,
+                   Rec* sm, Rec* ru) {
+  // The tiling of a system depends on that system only, so its reduction
+  // trees (and results) are the same alone and inside a batch.
+  // ~one workgroup per 2048 cells, at most 1024 (about one resident wave
+  // set of the 256 CUs: long z marches, few re-read halo planes)
+  const long long n = (long long)nx * ny * nz;
+  const long long tgt = std::min(1024LL, std::max(1LL, cdiv(n, 2048)));
+  *st = make_stencil_rec(nx, ny, nz, method == kPcg ? vx : 1, tgt, tu);
+  Tuning none;
+  *ru = make_stencil_rec(nx, ny, nz, 1, tgt, none);
+  Rec b{};
+  const int cells = tu.sm > 0 ? tu.sm : (method == kPcg ? 4 * vx : 4);
+  b.nblk = (int)std::min(16384LL, std::max(1LL, cdiv(n, (long long)kBlock * cells)));
+  b.tiles_x = b.nblk;
+  b.tiles_y = b.tiles_z = 1;
+  b.lx = kBlock;
+  b.bz = 1;
+  *sm = b;
+}
+
 int Engine::fail(int code, const std::string& msg) {
   err_ = msg;
   return code;
@@ -124,6 +145,7 @@ Engine::~Engine() {
   if (d_rec_) (void)hipFree(d_rec_);
   if (d_b2r_) (void)hipFree(d_b2r_);
   if (staging_) (void)hipFree(staging_);
+  release(staging_bytes_);
   if (d_flags_) (void)hipFree(d_flags_);
   if (d_red_) (void)hipFree(d_red_);
   if (d_red_counter_) (void)hipFree(d_red_counter_);
@@ -198,14 +220,32 @@ long long Engine::bytes() const {
   return b;
 }
 
+bool Engine::reserve(long long nbytes) {
+  if (shared_) {
+    long long cur = shared_->load();
+    do {
+      if (budget_ > 0 && cur + nbytes > budget_) return false;
+    } while (!shared_->compare_exchange_weak(cur, cur + nbytes));
+    return true;
+  }
+  return budget_ <= 0 || bytes() + nbytes <= budget_;
+}
+
+void Engine::release(long long nbytes) {
+  if (shared_) shared_->fetch_sub(nbytes);
+}
+
 int Engine::alloc(System& s, void** p, long long nbytes) {
-  if (budget_ > 0 && bytes() + nbytes > budget_) {
-    return fail(APHPOIS_EBUDGET, "memory budget exceeded (" + std::to_string(bytes() + nbytes) +
+  if (!reserve(nbytes)) {
+    return fail(APHPOIS_EBUDGET, "memory budget exceeded (" +
+                                     std::to_string((shared_ ? shared_->load() : bytes()) +
+                                                    nbytes) +
                                      " > " + std::to_string(budget_) + " bytes)");
   }
   hipError_t e = hipMalloc(p, (size_t)std::max(nbytes, 256LL));
   if (e != hipSuccess) {
     *p = nullptr;
+    release(nbytes);
     return hip(e, "hipMalloc");
   }
   s.allocs.push_back(*p);
@@ -216,6 +256,7 @@ int Engine::alloc(System& s, void** p, long long nbytes) {
 void Engine::free_system(System& s) {
   for (void* p : s.allocs) (void)hipFree(p);
   s.allocs.clear();
+  release(s.bytes);
   s.bytes = 0;
   if (s.partials) (void)hipFree(s.partials);
   s.partials = nullptr;
@@ -461,18 +502,7 @@ int Engine::upload_strided(const double* src, bool on_device, long long start, l
   const long long last = start + (long long)(nz - 1) * lz + (long long)(ny - 1) * ly + (nx - 1);
   const long long count = (last - start + 1) * comps;
   const long long need = count * 8;
-  if (need > staging_bytes_) {
-    if (staging_) {
-      APH_HIP(hipStreamSynchronize(st_));
-      APH_HIP(hipFree(staging_));
-      staging_ = nullptr;
-      staging_bytes_ = 0;
-    }
-    if (budget_ > 0 && bytes() + need > budget_)
-      return fail(APHPOIS_EBUDGET, "memory budget exceeded by the upload staging buffer");
-    APH_HIP(hipMalloc(&staging_, need));
-    staging_bytes_ = need;
-  }
+  APH_TRY(ensure_staging(need));
   APH_HIP(hipMemcpyAsync(staging_, src + start * comps, need,
                          on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st_));
   *dptr = (const double*)staging_;
@@ -496,11 +526,16 @@ int Engine::ensure_staging(long long need) {
     APH_HIP(hipStreamSynchronize(st_));
     APH_HIP(hipFree(staging_));
     staging_ = nullptr;
+    release(staging_bytes_);
     staging_bytes_ = 0;
   }
-  if (budget_ > 0 && bytes() + need > budget_)
-    return fail(APHPOIS_EBUDGET, "memory budget exceeded by the staging buffer");
-  APH_HIP(hipMalloc(&staging_, need));
+  if (!reserve(need)) return fail(APHPOIS_EBUDGET, "memory budget exceeded by the staging buffer");
+  const hipError_t e = hipMalloc(&staging_, need);
+  if (e != hipSuccess) {
+    staging_ = nullptr;
+    release(need);
+    return hip(e, "hipMalloc (staging)");
+  }
   staging_bytes_ = need;
   return APHPOIS_OK;
 }
@@ -517,14 +552,20 @@ int Engine::assemble(int slot, const double* rho, const double* fk, const double
   APH_TRY(ensure_staging(n * 8 * 8));
   const double *drho = rho, *dfk = fk, *dffv = ffv, *dsrc = src;
   void* tmp = nullptr;
+  long long tmp_bytes = 0;
   if (!on_device) {
     // inputs through a temporary device buffer: density (n) or face
     // denominators (3 n), face fluxes (3 n), source (n)
     const long long nden = rho ? n : 3 * n;
     const long long need = 8 * (nden + 3 * n + (src ? n : 0));
-    if (budget_ > 0 && bytes() + need > budget_)
+    if (!reserve(need))
       return fail(APHPOIS_EBUDGET, "memory budget exceeded by the assembly inputs");
-    APH_HIP(hipMalloc(&tmp, need));
+    const hipError_t ea = hipMalloc(&tmp, need);
+    if (ea != hipSuccess) {
+      release(need);
+      return hip(ea, "hipMalloc (assembly inputs)");
+    }
+    tmp_bytes = need;
     double* t = (double*)tmp;
     hipError_t e =
         hipMemcpyAsync(t, rho ? rho : fk, nden * 8, hipMemcpyHostToDevice, st_);
@@ -535,6 +576,7 @@ int Engine::assemble(int slot, const double* rho, const double* fk, const double
     if (e != hipSuccess) {
       (void)hipStreamSynchronize(st_);
       (void)hipFree(tmp);
+      release(need);
       return hip(e, "assemble upload");
     }
     drho = rho ? t : nullptr;
@@ -547,6 +589,7 @@ int Engine::assemble(int slot, const double* rho, const double* fk, const double
   if (tmp) {
     const hipError_t e2 = hipStreamSynchronize(st_);
     (void)hipFree(tmp);
+    release(tmp_bytes);
     if (e == hipSuccess) e = e2;
   }
   APH_HIP(e);
@@ -639,6 +682,9 @@ int Engine::set_system_staged(int slot, const double* d, long long ly, long long
   } else {
     if (method_ == kJacobi && (flags & 4u))
       return fail(APHPOIS_EINVAL, "zero diagonal coefficient (Jacobi divides by e0)");
+    // singular (vanishing row sums): only the max_diff report removes a mean
+    s.singular = nullspace_ == APHPOIS_NULL_ON ||
+                 (nullspace_ == APHPOIS_NULL_AUTO && (flags & 64u) == 0);
     APH_HIP(launch_convert_f7(d, 0, ly, lz, s.nx, s.ny, s.nz, const_cast<double*>(s.dev.f7[0]),
                               0, st_));
   }
@@ -669,6 +715,28 @@ int Engine::set_rhs(int slot, const double* e7, bool on_device, long long start,
   return APHPOIS_OK;
 }
 
+int Engine::set_solution_ref(int slot, const double* xref, long long start, long long ly,
+                             long long lz) {
+  System* sp = sys(slot);
+  if (!sp) return fail(APHPOIS_EINVAL, "no such slot");
+  System& s = *sp;
+  if (!xref) {
+    s.xref = nullptr;  // the storage stays with the system until it is removed
+    return APHPOIS_OK;
+  }
+  if (!s.xref_store) {
+    void* p = nullptr;
+    APH_TRY(alloc(s, &p, s.n * 8));
+    s.xref_store = (double*)p;
+  }
+  const double* d = nullptr;
+  APH_TRY(upload_strided(xref, false, start, ly, lz, s.nx, s.ny, s.nz, 1, &d));
+  APH_HIP(launch_gather(d, 0, ly, lz, s.nx, s.ny, s.nz, s.xref_store, 0.0, st_));
+  APH_HIP(hipStreamSynchronize(st_));
+  s.xref = s.xref_store;
+  return APHPOIS_OK;
+}
+
 int Engine::prepare(int slot, const double* x0, bool on_device, long long start, long long ly,
                     long long lz, bool keep_x) {
   System* sp = sys(slot);
@@ -685,10 +753,7 @@ int Engine::prepare(int slot, const double* x0, bool on_device, long long start,
     if (method_ == kPcg) {
       // x64 holds x + mean_pending; fold the pending mean in
       const double mp = h_sc_[slot].mean_pending;
-      if (mp != 0.0) {
-        APH_HIP(launch_gather(s.dev.x64, 0, s.nx, (long long)s.nx * s.ny, s.nx, s.ny, s.nz,
-                              s.dev.x64, mp, st_));
-      }
+      if (mp != 0.0) APH_HIP(launch_shift(s.dev.x64, n, mp, st_));
     } else if (method_ == kJacobi && h_sc_[slot].cur == 1) {
       APH_HIP(hipMemcpyAsync(s.dev.u[0], s.dev.u[1], n * 8, hipMemcpyDeviceToDevice, st_));
     }
@@ -782,31 +847,18 @@ int Engine::build_tables() {
   std::vector<std::vector<int>> blp(kMaxLevels);
   for (int k = 0; k < ns; ++k) {
     System& s = *slots_[launch_slots_[k]];
-    // The tiling of a system depends on that system only, so its reduction
-    // trees (and results) are the same alone and inside a batch.
-    // ~one workgroup per 2048 cells, at most 1024 (about one resident wave
-    // set of the 256 CUs: long z marches, few re-read halo planes)
-    const long long tgt = std::min(1024LL, std::max(1LL, cdiv(s.n, 2048)));
     // per-system tile (autotuner) over the engine-wide override
     Tuning tu = tuning_;
     if (s.tune_lx > 0) tu.lx = s.tune_lx;
     if (s.tune_bz > 0) tu.bz = s.tune_bz;
-    Rec a = make_stencil_rec(s.nx, s.ny, s.nz, method_ == kPcg ? vx_ : 1, tgt, tu);
-    a.sys = k;
+    Rec a, b, c;
+    system_tiling(method_, s.nx, s.ny, s.nz, vx_, tu, &a, &b, &c);
+    a.sys = b.sys = c.sys = k;
     a.blk0 = (int)bst.size();
-    Tuning none;
-    Rec c = make_stencil_rec(s.nx, s.ny, s.nz, 1, tgt, none);
-    c.sys = k;
-    c.blk0 = (int)bru.size();
-    Rec b{};
-    b.sys = k;
     b.blk0 = (int)bsm.size();
-    const int sm = tuning_.sm > 0 ? tuning_.sm : (method_ == kPcg ? 4 * vx_ : 4);
-    b.nblk = (int)std::min(16384LL, std::max(1LL, cdiv(s.n, (long long)kBlock * sm)));
-    b.tiles_x = b.nblk;
-    b.tiles_y = b.tiles_z = 1;
-    b.lx = kBlock;
-    b.bz = 1;
+    c.blk0 = (int)bru.size();
+    const long long tgt = std::min(1024LL, std::max(1LL, cdiv(s.n, 2048)));
+    Tuning none;
     for (int i = 0; i < a.nblk; ++i) bst.push_back((int)rst.size());
     for (int i = 0; i < b.nblk; ++i) bsm.push_back((int)rsm.size());
     for (int i = 0; i < c.nblk; ++i) bru.push_back((int)rru.size());
@@ -1057,6 +1109,28 @@ void Engine::fill_info(int slot, double seconds) {
   inf.refinements = c.nru;
   inf.status = c.status;
   inf.seconds = seconds;
+  inf.max_diff = -1.0;
+}
+
+int Engine::diff_info(int slot) {
+  System& s = *slots_[slot];
+  if (!s.xref) return APHPOIS_OK;
+  const double* x;
+  double sub = 0;
+  if (method_ == kPcg) {
+    x = s.dev.x64;
+    sub = h_sc_[slot].mean_pending;
+  } else if (method_ == kJacobi) {
+    x = s.dev.u[h_sc_[slot].cur];
+  } else {
+    x = s.dev.u[0];
+  }
+  double out[2] = {0, 0};
+  APH_HIP(launch_max_diff(x, s.xref, s.n, sub, s.singular ? 1 : 0, d_red_, d_red_ + 1024, st_));
+  APH_HIP(hipMemcpyAsync(out, d_red_ + 1024, sizeof(out), hipMemcpyDeviceToHost, st_));
+  APH_HIP(hipStreamSynchronize(st_));
+  s.info.max_diff = out[1];
+  return APHPOIS_OK;
 }
 
 int Engine::perturb_rhs(int slot, double amp, unsigned long long key) {
@@ -1144,6 +1218,7 @@ int Engine::step(int max_replays, std::vector<int>* finished) {
         s.done_reported = true;
         s.have_x = true;
         fill_info(i, 0.0);
+        APH_TRY(diff_info(i));
         finished->push_back(i);
       }
       alldone = alldone && h_sc_[i].done;
@@ -1163,7 +1238,7 @@ int Engine::run() {
   }
   APH_HIP(hipEventRecord(ev0_, st_));
   APH_HIP(launch_start(method_, fp32_, vx_, L_, st_));
-  const int maxloops = maxiter_ / std::max(1, graph_k_) + 4;
+  const int maxloops = (std::max(maxiter_, miniter_) + 1) / std::max(1, graph_k_) + 4;
   int loops = 0;
   bool alldone = false;
   int iters_before = 0;
@@ -1291,6 +1366,7 @@ int Engine::run() {
     System& s = *slots_[i];
     if (s.done_reported) continue;  // finished in an earlier call, idle since
     fill_info(i, ms * 1e-3);
+    APH_TRY(diff_info(i));
     last_run_iters_ = std::max(last_run_iters_, h_sc_[i].iter);
     s.have_x = true;
     s.done_reported = true;
@@ -1326,15 +1402,7 @@ int Engine::fetch(int slot, double* x, bool on_device, long long start, long lon
     return APHPOIS_OK;
   }
   // dense result in the staging buffer, then host copy
-  if (s.n * 8 > staging_bytes_) {
-    if (staging_) {
-      APH_HIP(hipFree(staging_));
-      staging_ = nullptr;
-      staging_bytes_ = 0;
-    }
-    APH_HIP(hipMalloc(&staging_, s.n * 8));
-    staging_bytes_ = s.n * 8;
-  }
+  APH_TRY(ensure_staging(s.n * 8));
   double* st = (double*)staging_;
   APH_HIP(launch_scatter(src, s.nx, s.ny, s.nz, st, 0, s.nx, (long long)s.nx * s.ny, sub, st_));
   if (dense) {

@@ -65,8 +65,20 @@ __device__ __forceinline__ void st1(T* __restrict__ p, long long i, T v) {
 // (system, level and launch tables), read through the constant address
 // space: uniform scalar loads into SGPRs, so the pointers it holds need no
 // VGPRs and the loads through them can take a scalar base.
+//
+// Only global memory may be read this way.  A generic pointer into LDS or
+// into a thread's private memory (a __shared__ table, a by-value copy of a
+// table entry) holds an aperture address that is not a global address;
+// reinterpreted as a constant-space pointer it would read unmapped memory
+// and fault.  So cload takes an explicitly global pointer: call sites state
+// with gtab() that the table is a hipMalloc'ed device array, and an LDS or
+// stack address does not convert to it implicitly.
 template <typename T>
-__device__ __forceinline__ T cload(const T* p) {
+__device__ __forceinline__ gcp<T> gtab(const T* p) {
+  return (gcp<T>)p;
+}
+template <typename T>
+__device__ __forceinline__ T cload(gcp<T> p) {
   static_assert(sizeof(T) % 4 == 0 && std::is_trivially_copyable<T>::value, "dword copy");
   T v;
   const __attribute__((address_space(4))) unsigned* src =
@@ -466,7 +478,7 @@ __global__ __launch_bounds__(kBlock) void k_fin_pcg_update(const SysDev* __restr
     sc->iter += 1;
     const double est = sc->gamma * sqrt(rn);
     const bool trig = !isfinite(rn) || sqrt(rn) < sc->delta * sqrt(sc->rmax2) ||
-                      est <= 0.9 * sc->tol || sc->iter >= sc->maxiter;
+                      est <= 0.9 * sc->tol || sc->iter > sc->maxiter;
     if (trig) {
       sc->ru = 1;
       sc->rho_pend = rn;
@@ -627,7 +639,9 @@ __global__ __launch_bounds__(kBlock) void k_fin_ru_resid(const SysDev* __restric
     if (!isfinite(rr) || !isfinite(rh2)) {
       sc->status = 6;
       sc->done = 1;
-    } else if ((conv && sc->iter >= sc->miniter) || sc->iter >= sc->maxiter) {
+    } else if (sc->iter >= sc->miniter && (conv || sc->iter > sc->maxiter)) {
+      // the reference's rule (src/linear/linear.ipp:109-110): at most
+      // maxiter + 1 iterations, never fewer than miniter
       sc->converged = conv ? 1 : 0;
       sc->done = 1;
     } else {
@@ -1128,13 +1142,15 @@ __global__ __launch_bounds__(kPostBlock) void k_mg_post(const SysDev* __restrict
   __shared__ T su[2][kPostBlock * VX];
   __shared__ T sv[2][kPostBlock * VX];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const Rec rc = cload(recs + cload(b2r + b));
-  const SysDev s = cload(sds + rc.sys);
+  // launch tables (recs, b2r, sds) and level tables (s.lev) are device
+  // arrays written by the host between launches only
+  const Rec rc = cload(gtab(recs + cload(gtab(b2r + b))));
+  const SysDev s = cload(gtab(sds + rc.sys));
   if (s.sc->done) return;
   if (wl >= 0 && wl >= s.mg_w) return;
   if (so && !s.sc->first) return;
-  const MgLevel Lv = cload(s.lev + l);
-  const MgLevel Lc = cload(s.lev + l + 1);
+  const MgLevel Lv = cload(gtab(s.lev + l));
+  const MgLevel Lc = cload(gtab(s.lev + l + 1));
   const int nx = Lv.nx, ny = Lv.ny, nz = Lv.nz;
   const int lb = b - rc.blk0;
   const int lx = rc.lx, ty = rc.ty;  // lx * VX == nx, ty <= kPostBlock / lx
@@ -1305,13 +1321,15 @@ __global__ __launch_bounds__(kPostBlock) void k_mg_pre(const SysDev* __restrict_
   __shared__ T sz[2][kPostBlock * VX];
   __shared__ T sacc[kPostBlock * NP];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const Rec rc = cload(recs + cload(b2r + b));
-  const SysDev s = cload(sds + rc.sys);
+  // launch tables (recs, b2r, sds) and level tables (s.lev) are device
+  // arrays written by the host between launches only
+  const Rec rc = cload(gtab(recs + cload(gtab(b2r + b))));
+  const SysDev s = cload(gtab(sds + rc.sys));
   if (s.sc->done) return;
   if (wl >= 0 && wl >= s.mg_w) return;
   if (so && !s.sc->first) return;
-  const MgLevel Lv = cload(s.lev + l);
-  const MgLevel Lc = cload(s.lev + l + 1);
+  const MgLevel Lv = cload(gtab(s.lev + l));
+  const MgLevel Lc = cload(gtab(s.lev + l + 1));
   const int nx = Lv.nx, ny = Lv.ny, nz = Lv.nz;
   const int lb = b - rc.blk0;
   const int lx = rc.lx, ty = rc.ty;  // lx * VX == nx, ty even, <= kPostBlock / lx
@@ -1848,6 +1866,7 @@ __global__ __launch_bounds__(kBlock) void k_ref_init(const SysDev* __restrict__ 
   if (sc->done) return;
   const int lb = b - rc.blk0;
   const Tile t = tile_of<1>(rc, s, lb);
+  const int sh = s.f7sh;  // 0: SoA arrays, 3: the Expr's AoS (8 doubles per cell)
   const int nx = s.nx, ny = s.ny, nz = s.nz;
   const double* __restrict__ u = s.u[0];
   double acc[1] = {0.0};
@@ -1861,13 +1880,13 @@ __global__ __launch_bounds__(kBlock) void k_ref_init(const SysDev* __restrict__ 
     for (int z = t.za; z < t.zb; ++z) {
       const long long base = (long long)z * pl;
       const long long c = base + row + xx;
-      double v = u[c] * s.f7[0][c] + s.f7[7][c];
-      v += u[base + row + xw] * s.f7[1][c];
-      v += u[base + row + xe] * s.f7[2][c];
-      v += u[base + rowm + xx] * s.f7[3][c];
-      v += u[base + rowp + xx] * s.f7[4][c];
-      v += u[(long long)wrapm(z, nz) * pl + row + xx] * s.f7[5][c];
-      v += u[(long long)wrapp(z, nz) * pl + row + xx] * s.f7[6][c];
+      double v = u[c] * s.f7[0][c << sh] + s.f7[7][c << sh];
+      v += u[base + row + xw] * s.f7[1][c << sh];
+      v += u[base + row + xe] * s.f7[2][c << sh];
+      v += u[base + rowm + xx] * s.f7[3][c << sh];
+      v += u[base + rowp + xx] * s.f7[4][c << sh];
+      v += u[(long long)wrapm(z, nz) * pl + row + xx] * s.f7[5][c << sh];
+      v += u[(long long)wrapp(z, nz) * pl + row + xx] * s.f7[6][c << sh];
       const double rv = -v;
       s.rr[c] = rv;
       s.pp[0][c] = rv;
@@ -1902,6 +1921,7 @@ __global__ __launch_bounds__(kBlock) void k_ref_spmv(const SysDev* __restrict__ 
   if (sc->done) return;
   const int lb = b - rc.blk0;
   const Tile t = tile_of<1>(rc, s, lb);
+  const int sh = s.f7sh;  // 0: SoA arrays, 3: the Expr's AoS (8 doubles per cell)
   const int nx = s.nx, ny = s.ny, nz = s.nz;
   const int cur = sc->cur;
   const int first = sc->first;
@@ -1922,13 +1942,13 @@ __global__ __launch_bounds__(kBlock) void k_ref_spmv(const SysDev* __restrict__ 
       const long long base = (long long)z * pl;
       const long long c = base + row + xx;
       const double pc = pn(c);
-      double v = pc * s.f7[0][c];
-      v += pn(base + row + xw) * s.f7[1][c];
-      v += pn(base + row + xe) * s.f7[2][c];
-      v += pn(base + rowm + xx) * s.f7[3][c];
-      v += pn(base + rowp + xx) * s.f7[4][c];
-      v += pn((long long)wrapm(z, nz) * pl + row + xx) * s.f7[5][c];
-      v += pn((long long)wrapp(z, nz) * pl + row + xx) * s.f7[6][c];
+      double v = pc * s.f7[0][c << sh];
+      v += pn(base + row + xw) * s.f7[1][c << sh];
+      v += pn(base + row + xe) * s.f7[2][c << sh];
+      v += pn(base + rowm + xx) * s.f7[3][c << sh];
+      v += pn(base + rowp + xx) * s.f7[4][c << sh];
+      v += pn((long long)wrapm(z, nz) * pl + row + xx) * s.f7[5][c << sh];
+      v += pn((long long)wrapp(z, nz) * pl + row + xx) * s.f7[6][c << sh];
       pout[c] = pc;
       s.qq[c] = v;
       acc[0] += pc * v;
@@ -2009,6 +2029,7 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(const SysDev* __restrict__ sd
   if (sc->done) return;
   const int lb = b - rc.blk0;
   const Tile t = tile_of<1>(rc, s, lb);
+  const int sh = s.f7sh;  // 0: SoA arrays, 3: the Expr's AoS (8 doubles per cell)
   const int nx = s.nx, ny = s.ny, nz = s.nz;
   const int cur = sc->cur;
   const double* __restrict__ u = s.u[cur];
@@ -2024,14 +2045,14 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(const SysDev* __restrict__ sd
     for (int z = t.za; z < t.zb; ++z) {
       const long long base = (long long)z * pl;
       const long long c = base + row + xx;
-      double nd = s.f7[7][c];
-      nd += u[base + row + xw] * s.f7[1][c];
-      nd += u[base + row + xe] * s.f7[2][c];
-      nd += u[base + rowm + xx] * s.f7[3][c];
-      nd += u[base + rowp + xx] * s.f7[4][c];
-      nd += u[(long long)wrapm(z, nz) * pl + row + xx] * s.f7[5][c];
-      nd += u[(long long)wrapp(z, nz) * pl + row + xx] * s.f7[6][c];
-      const double v = -nd / s.f7[0][c];
+      double nd = s.f7[7][c << sh];
+      nd += u[base + row + xw] * s.f7[1][c << sh];
+      nd += u[base + row + xe] * s.f7[2][c << sh];
+      nd += u[base + rowm + xx] * s.f7[3][c << sh];
+      nd += u[base + rowp + xx] * s.f7[4][c << sh];
+      nd += u[(long long)wrapm(z, nz) * pl + row + xx] * s.f7[5][c << sh];
+      nd += u[(long long)wrapp(z, nz) * pl + row + xx] * s.f7[6][c << sh];
+      const double v = -nd / s.f7[0][c << sh];
       un[c] = v;
       acc[0] = fmax(acc[0], fabs(v - u[c]));
     }
@@ -2068,6 +2089,7 @@ __global__ __launch_bounds__(kBlock) void k_true_res_f7(const SysDev* __restrict
   Scalars* sc = s.sc;
   const int lb = b - rc.blk0;
   const Tile t = tile_of<1>(rc, s, lb);
+  const int sh = s.f7sh;  // 0: SoA arrays, 3: the Expr's AoS (8 doubles per cell)
   const int nx = s.nx, ny = s.ny, nz = s.nz;
   const double* __restrict__ u = sc->method == kJacobi ? s.u[sc->cur] : s.u[0];
   double acc[3] = {0.0, 0.0, 0.0};
@@ -2081,15 +2103,15 @@ __global__ __launch_bounds__(kBlock) void k_true_res_f7(const SysDev* __restrict
     for (int z = t.za; z < t.zb; ++z) {
       const long long base = (long long)z * pl;
       const long long c = base + row + xx;
-      double v = u[c] * s.f7[0][c] + s.f7[7][c];
-      v += u[base + row + xw] * s.f7[1][c];
-      v += u[base + row + xe] * s.f7[2][c];
-      v += u[base + rowm + xx] * s.f7[3][c];
-      v += u[base + rowp + xx] * s.f7[4][c];
-      v += u[(long long)wrapm(z, nz) * pl + row + xx] * s.f7[5][c];
-      v += u[(long long)wrapp(z, nz) * pl + row + xx] * s.f7[6][c];
+      double v = u[c] * s.f7[0][c << sh] + s.f7[7][c << sh];
+      v += u[base + row + xw] * s.f7[1][c << sh];
+      v += u[base + row + xe] * s.f7[2][c << sh];
+      v += u[base + rowm + xx] * s.f7[3][c << sh];
+      v += u[base + rowp + xx] * s.f7[4][c << sh];
+      v += u[(long long)wrapm(z, nz) * pl + row + xx] * s.f7[5][c << sh];
+      v += u[(long long)wrapp(z, nz) * pl + row + xx] * s.f7[6][c << sh];
       acc[0] += v * v;
-      acc[1] += s.f7[7][c] * s.f7[7][c];
+      acc[1] += s.f7[7][c << sh] * s.f7[7][c << sh];
       acc[2] = fmax(acc[2], fabs(v));
     }
   }
@@ -2320,6 +2342,59 @@ __global__ __launch_bounds__(kBlock) void k_fill(T* __restrict__ p, long long n,
        i += (long long)gridDim.x * kBlock) {
     p[i] = v;
   }
+}
+
+// block -> record map of a one-system launch set with three records:
+// n0 workgroups of record 0, then n1 of record 1, then n2 of record 2
+__global__ __launch_bounds__(kBlock) void k_b2r3(int* __restrict__ b2r, int n0, int n1,
+                                                 int n2) {
+  const int n = n0 + n1 + n2;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+    b2r[i] = i < n0 ? 0 : (i < n0 + n1 ? 1 : 2);
+}
+
+// x -= sub in place (a warm start folding a pending mean into the solution)
+__global__ __launch_bounds__(kBlock) void k_shift(double* __restrict__ x, long long n,
+                                                  double sub) {
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (long long)gridDim.x * kBlock) {
+    x[i] -= sub;
+  }
+}
+
+// Difference of a solution from a reference solution, d = (x - sub) - xref:
+// partials of sum d (pass 0), then of max |d - mean| with the mean read from
+// stats[0] (pass 1; stats NULL: mean 0).
+__global__ __launch_bounds__(kBlock) void k_diff(const double* __restrict__ x,
+                                                 const double* __restrict__ xref, long long n,
+                                                 double sub, int pass,
+                                                 const double* __restrict__ stats,
+                                                 double* __restrict__ partials) {
+  double acc[1] = {0.0};
+  const double mean = (pass == 1 && stats) ? stats[0] : 0.0;
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (long long)gridDim.x * kBlock) {
+    const double d = (x[i] - sub) - xref[i];
+    acc[0] = pass == 0 ? acc[0] + d : fmax(acc[0], fabs(d - mean));
+  }
+  if (pass == 0) {
+    block_partial<1, 0u>(acc, partials, gridDim.x, blockIdx.x);
+  } else {
+    block_partial<1, 1u>(acc, partials, gridDim.x, blockIdx.x);
+  }
+}
+
+// out = (sum of the partials) * scale (pass 0) or their max (pass 1)
+__global__ __launch_bounds__(kBlock) void k_fin_diff(const double* __restrict__ partials,
+                                                     int nblk, int pass, double scale,
+                                                     double* __restrict__ out) {
+  double acc[1];
+  if (pass == 0) {
+    reduce_partials<1, 0u>(acc, partials, nblk);
+  } else {
+    reduce_partials<1, 1u>(acc, partials, nblk);
+  }
+  if (threadIdx.x == 0) *out = pass == 0 ? acc[0] * scale : acc[0];
 }
 
 // ===========================================================================
@@ -2695,6 +2770,30 @@ hipError_t launch_sumsq(const double* v, long long n, double* partials, unsigned
   if (g > 1024) g = 1024;
   APH_LAUNCH(k_sumsq, g, st, v, n, partials, counter, out);
   APH_LAUNCH(k_fin_sumsq, 1, st, partials, g, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_b2r3(int* b2r, int n0, int n1, int n2, hipStream_t st) {
+  APH_LAUNCH(k_b2r3, grid_for((long long)n0 + n1 + n2), st, b2r, n0, n1, n2);
+  return hipGetLastError();
+}
+
+hipError_t launch_shift(double* x, long long n, double sub, hipStream_t st) {
+  APH_LAUNCH(k_shift, grid_for(n), st, x, n, sub);
+  return hipGetLastError();
+}
+
+hipError_t launch_max_diff(const double* x, const double* xref, long long n, double sub,
+                           int remove_mean, double* partials, double* out, hipStream_t st) {
+  int g = grid_for(n);
+  if (g > 1024) g = 1024;
+  if (remove_mean) {
+    APH_LAUNCH(k_diff, g, st, x, xref, n, sub, 0, (const double*)nullptr, partials);
+    APH_LAUNCH(k_fin_diff, 1, st, partials, g, 0, 1.0 / (double)n, out);
+  }
+  APH_LAUNCH(k_diff, g, st, x, xref, n, sub, 1, remove_mean ? (const double*)out : nullptr,
+             partials);
+  APH_LAUNCH(k_fin_diff, 1, st, partials, g, 1, 1.0, out + 1);
   return hipGetLastError();
 }
 

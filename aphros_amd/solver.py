@@ -44,6 +44,15 @@ class Poisson:
         nz, ny, nx = self.shape
         L.check(L.lib().aphpois_set_rhs(self.h, L.dptr(e), 0, nx, nx * ny))
 
+    def set_solution_ref(self, xref):
+        """Reference solution (nz, ny, nx) for info["max_diff"] of later
+        solves; None removes it."""
+        nz, ny, nx = self.shape
+        x = None if xref is None else np.ascontiguousarray(xref, np.float64).reshape(-1)
+        if x is not None and x.size != nx * ny * nz:
+            raise ValueError("xref has the wrong size")
+        L.check(L.lib().aphpois_set_solution_ref(self.h, L.dptr(x), 0, nx, nx * ny))
+
     def set_conf(self, tol, miniter, maxiter):
         L.check(L.lib().aphpois_set_conf(self.h, tol, miniter, maxiter))
 

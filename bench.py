@@ -124,6 +124,129 @@ def measured_traffic(config, prec, precond, kernel, li, L):
     return None if rec is None else int(rec["fetch_bytes"] + rec["write_bytes"])
 
 
+def run_c2(args):
+    """256^3 (32 spheres r in [0.08, 0.15], density ratio 1000, seed 2): FP64
+    Jacobi-preconditioned CG, the drop-in's plain form, to a 1e-8 true
+    relative residual from a zero guess; HBM GB/s per iteration from the
+    event-timed iteration kernels."""
+    from aphros_amd import _lib as L
+    from aphros_amd import synth
+    from aphros_amd.devmem import DeviceBuffer
+    from aphros_amd.solver import Poisson
+    cfg = synth.CONFIGS["c2_256"]
+    n = cfg["n"]
+    ncell = n ** 3
+    e, _, _ = synth.config("c2_256")
+    s = Poisson((n, n, n), method=L.PCG, precision=L.FP64, resmode=L.RES_REL_TRUE, tol=1e-8,
+                maxiter=200000, precond=L.PRECOND_JACOBI)
+    s.set_system(e)
+    del e
+    xb = DeviceBuffer(ncell * 8)
+    t = time.perf_counter()
+    info0 = s.solve_device(xb.ptr)  # graph replays
+    plain = time.perf_counter() - t
+    s.set_profiling(True)
+    t = time.perf_counter()
+    info = s.solve_device(xb.ptr)  # kernels launched singly between events
+    prof = time.perf_counter() - t
+    kt = s.kernel_times()
+    s.close()
+    xb.free()
+    if not (info0["converged"] and info["converged"]):
+        raise RuntimeError("c2 solve did not converge")
+    kernels = kernel_table(kt, ncell, 8, 8, False, {}, jacobi=True)
+    b_it = sum(k["alg_bytes_per_launch"] for k in kernels.values())
+    ms_it = sum(k["avg_ms"] for k in kernels.values())
+    log(f"[bench] c2: {info0['iter']} iterations, {plain:.3f}s; iteration kernels "
+        f"{b_it / (ms_it * 1e-3) / 1e9:.0f} GB/s")
+    return {"workload": f"c2_256: {n}^3 periodic, {cfg['nspheres']} spheres r in "
+                        f"[{cfg['rmin']},{cfg['rmax']}], density ratio {cfg['ratio']:g}, seed "
+                        f"{cfg['seed']}; FP64 Jacobi-preconditioned CG to a 1e-8 true "
+                        f"relative residual, zero guess",
+            "iters": info0["iter"], "refinements": info0["refinements"],
+            "true_relres": info0["true_relres"], "s_per_solve": round(plain, 4),
+            "s_per_solve_event_timed": round(prof, 4),
+            "alg_bytes_per_iter": int(b_it), "iter_kernels_ms": round(ms_it, 5),
+            "GBps_per_iter": round(b_it / (ms_it * 1e-3) / 1e9, 1),
+            "frac_of_peak": round(b_it / (ms_it * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "GBps_whole_solve": round(b_it * info0["iter"] / plain / 1e9, 1),
+            "kernels": kernels}
+
+
+def run_c1(args):
+    """64^3 (8 spheres, density ratio 1000, seed 1) to a 1e-8 true relative
+    residual: the reference's CG restated on the GPU (CONJUGATE, FP64, its
+    stop rule with tol = 1e-8 ||b|| / sqrt(h^3)), the multigrid PCG forms on
+    the GPU, and the reference's CG on the host's cores (cpu_baseline)."""
+    from aphros_amd import _lib as L
+    from aphros_amd import synth
+    from aphros_amd.devmem import DeviceBuffer
+    from aphros_amd.solver import Poisson
+    from oracle import oracle as O
+    n = 64
+    ncell = n ** 3
+    h3 = (1.0 / n) ** 3
+    e, _, _ = synth.config("c1_64")
+    E = e.reshape(n, n, n, 8)
+    # the reference's residual is sqrt(sum r^2 / h^3): this tol is a relative
+    # residual of 1e-8 on the recursive residual
+    tol_ref = 1e-8 * float(np.linalg.norm(E[..., 7])) / np.sqrt(h3)
+    xb = DeviceBuffer(ncell * 8)
+    out = {"workload": "c1_64: 64^3 periodic, 8 spheres r in [0.08,0.15], density ratio 1000, "
+                       "seed 1, zero guess, 1e-8 true relative residual"}
+
+    def gpu(name, **kw):
+        s = Poisson((n, n, n), **kw)
+        s.set_system(E)
+        s.solve_device(xb.ptr)
+        t = time.perf_counter()
+        reps = 3
+        for _ in range(reps):
+            info = s.solve_device(xb.ptr)
+        dt = (time.perf_counter() - t) / reps
+        s.close()
+        out[name] = {"s_per_solve": round(dt, 5), "iters": info["iter"],
+                     "true_relres": info["true_relres"]}
+        log(f"[bench] c1 {name}: {dt * 1e3:.2f} ms, {info['iter']} iterations, "
+            f"relres {info['true_relres']:.2e}")
+        return info
+
+    ic = gpu("gpu_conjugate", method=L.CONJUGATE, resmode=L.RES_L2, tol=tol_ref, miniter=0,
+             maxiter=100000, h3=h3)
+    gpu("gpu_mg_pcg_mixed", method=L.PCG, precision=L.MIXED, tol=1e-8,
+        precond=L.PRECOND_MG)
+    gpu("gpu_mg_pcg_fp64", method=L.PCG, precision=L.FP64, tol=1e-8, precond=L.PRECOND_MG)
+    xb.free()
+    iters = ic["iter"]
+    cpu = None
+    if args.cpu_iters > 0:
+        k = max(1, min(args.cpu_iters, iters))
+        t = time.perf_counter()
+        _, it, _, _ = O.conjugate(E, h3, tol=0.0, miniter=0, maxiter=k - 1)
+        dt = time.perf_counter() - t
+        spi = dt / it
+        cpu = {"value": round(1.0 / (spi * iters), 5),
+               "unit": "solves/s @64^3 (SolverConjugate to a 1e-8 relative residual)",
+               "cores": O.num_threads(), "kind": "port", "cpu_model": cpu_model(),
+               "iters": iters, "s_per_solve": round(spi * iters, 3),
+               "sample": f"{it} iterations of the oracle's SolverConjugate restatement "
+                         f"(bit-identical iterates, OpenMP) on the 64^3 system in "
+                         f"{dt:.2f}s, scaled to the {iters} iterations of the solve "
+                         f"(the GPU CONJUGATE count, the same stop rule)"}
+        if args.ref_iters > 0 and O.ref_available():
+            kr = max(1, min(args.ref_iters, iters))
+            _, itr, _, sec = O.ref_solve("conjugate", E, 0.0, 0, kr - 1, timed=True)
+            cpu["reference_compiled"] = {
+                "s_per_solve": round(sec / itr * iters, 3), "cores": 1,
+                "sample": f"{itr} iterations of the reference's own SolverConjugate "
+                          f"(oracle/_ref/ref_driver: the reference's sources, one block, "
+                          f"one thread) in {sec:.2f}s, scaled to {iters} iterations"}
+        for key in ("gpu_conjugate", "gpu_mg_pcg_mixed", "gpu_mg_pcg_fp64"):
+            out[key]["speedup_vs_cpu"] = round(cpu["s_per_solve"] / out[key]["s_per_solve"], 1)
+        log(f"[bench] c1 cpu: {cpu['s_per_solve']:.2f}s per solve ({cpu['cores']} threads)")
+    return out, cpu
+
+
 def run_batched(args, precond):
     """Batched workload: independent periodic systems of uneven edges solved
     together (one set of launches per kernel for all of them), mixed
@@ -240,8 +363,19 @@ def main():
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed",
                     help="form of the timed solves (the other one is the comparison)")
     ap.add_argument("--fp64-steps", type=int, default=1)
-    ap.add_argument("--cpu-iters", type=int, default=15,
-                    help="iterations of the CPU oracle timed for cpu_baseline (0 = skip)")
+    ap.add_argument("--cpu-iters", type=int, default=2000,
+                    help="iterations of the oracle's SolverConjugate on the 64^3 system "
+                         "timed for cpu_baseline (0 = skip)")
+    ap.add_argument("--ref-iters", type=int, default=300,
+                    help="iterations of the compiled reference (oracle/_ref/ref_driver, if "
+                         "built) timed beside it (0 = skip)")
+    ap.add_argument("--cpu-mg-iters", type=int, default=15,
+                    help="iterations of the oracle's FP64 multigrid PCG at the main config "
+                         "(0 = skip)")
+    ap.add_argument("--c2", type=int, default=1,
+                    help="run the 256^3 FP64 Jacobi-PCG leg (GB/s per iteration)")
+    ap.add_argument("--c1", type=int, default=1,
+                    help="run the 64^3 reference-CG leg and the CPU baseline")
     ap.add_argument("--no-profile", action="store_true",
                     help="no per-kernel events in the timed region")
     ap.add_argument("--batch-systems", type=int, default=96,
@@ -286,13 +420,15 @@ def main():
     xbuf = DeviceBuffer(ncell * 8)
     has_m = False  # the periodic pressure system: row sums vanish
 
-    def run_form(precision, steps, warmup, profile):
+    def run_form(precision, steps, warmup, profile, xref=None):
         s = Poisson((n, n, n), method=L.PCG, precision=precision, resmode=L.RES_REL_TRUE,
                     tol=1e-8, maxiter=200000, precond=precond)
         t = time.time()
         s.set_system(expr)
         log(f"[bench] precision={precision} set_system {time.time() - t:.2f}s "
             f"device_bytes={s.device_bytes() / 1e9:.2f} GB")
+        if xref is not None:
+            s.set_solution_ref(xref.reshape(n, n, n))
         plain = []
         for i in range(warmup):
             t = time.perf_counter()
@@ -310,7 +446,7 @@ def main():
             infos.append(info)
             log(f"[bench]   step {i}: {lat[-1]:.3f}s iter={info['iter']} "
                 f"ru={info['refinements']} relres={info['true_relres']:.3e} "
-                f"converged={info['converged']}")
+                f"converged={info['converged']} max_diff={info['max_diff']:.3e}")
             if not info["converged"] or info["true_relres"] > 1e-8:
                 raise RuntimeError("solve did not converge to 1e-8")
         kt = s.kernel_times() if profile else None
@@ -319,19 +455,32 @@ def main():
         return lat, infos, kt, li, plain
 
     main_prec = L.MIXED if args.precision == "mixed" else L.FP64
-    lat, infos, kt, li, plain = run_form(main_prec, args.steps, args.warmup,
-                                         not args.no_profile)
-    total = sum(lat)
-    value = args.steps / total
-    iters = [i["iter"] for i in infos]
-
-    fp64 = None
+    # the other form first: with the FP64 solution on the host, every mixed
+    # solve reports its error against it (info.max_diff)
+    fp64, x_other = None, None
     if args.fp64_steps > 0:
         lat64, inf64, _, _, _ = run_form(L.FP64 if main_prec == L.MIXED else L.MIXED,
                                          args.fp64_steps, 0, False)
         fp64 = {"solves_per_s": round(args.fp64_steps / sum(lat64), 4),
                 "iters": [i["iter"] for i in inf64],
-                "s_per_solve": round(sum(lat64) / args.fp64_steps, 4)}
+                "s_per_solve": round(sum(lat64) / args.fp64_steps, 4),
+                "true_relres": [i["true_relres"] for i in inf64]}
+        if main_prec == L.MIXED:
+            x_other = xbuf.to_host(np.empty(ncell))
+    lat, infos, kt, li, plain = run_form(main_prec, args.steps, args.warmup,
+                                         not args.no_profile, xref=x_other)
+    total = sum(lat)
+    value = args.steps / total
+    iters = [i["iter"] for i in infos]
+    error_vs_fp64 = None
+    if x_other is not None:
+        scale = float(np.abs(x_other - x_other.mean()).max())
+        md = max(i["max_diff"] for i in infos)
+        error_vs_fp64 = {"max_abs_diff": md, "max_abs_fp64_solution": scale,
+                         "relative": md / scale,
+                         "what": "max |x_mixed - x_fp64| over the timed solves, means "
+                                 "removed (info.max_diff, computed on the GPU)"}
+        del x_other
 
     roofline, kernels = None, None
     if kt:
@@ -369,30 +518,35 @@ def main():
         ms_it = 1e3 * total / max(1, sum(iters))
         kernels["ms_per_iter_wall"] = round(ms_it, 5)
 
-    batched = run_batched(args, precond) if args.batch_systems > 0 else None
-    served = run_served(args, precond) if args.serve_requests > 0 else None
-
-    cpu = None
-    if args.cpu_iters > 0:
+    cpu_mg = None
+    if args.cpu_mg_iters > 0:
         from oracle import oracle as O
         e4 = expr.reshape(n, n, n, 8)
         # iterations of the FP64 GPU solve
         ref_iters = fp64["iters"][0] if (fp64 and main_prec == L.MIXED) else iters[0]
         t = time.perf_counter()
         if precond == L.PRECOND_MG:
-            _, it, _, setup = O.mgpcg(e4, tol=0.0, maxiter=-args.cpu_iters, remove_mean=False,
-                                      w=L.mg_w_auto(ncell))
+            _, it, _, setup = O.mgpcg(e4, tol=0.0, maxiter=-args.cpu_mg_iters,
+                                      remove_mean=False, w=L.mg_w_auto(ncell))
             name = "multigrid PCG"
         else:
-            _, it, _ = O.pcg_jacobi(e4, tol=0.0, maxiter=-args.cpu_iters, remove_mean=False)
+            _, it, _ = O.pcg_jacobi(e4, tol=0.0, maxiter=-args.cpu_mg_iters, remove_mean=False)
             setup, name = 0.0, "Jacobi-PCG"
         dt = time.perf_counter() - t
         s_per_iter = (dt - setup) / it
-        cpu = {"value": round(1.0 / (setup + s_per_iter * ref_iters), 6), "unit": "solves/s",
-               "cores": O.num_threads(), "kind": "port", "cpu_model": cpu_model(),
-               "sample": f"{it} FP64 {name} iterations of the oracle (OpenMP) on the {n}^3 "
-                         f"system ({dt:.1f}s: setup {setup:.1f}s, {s_per_iter:.3f} s/iter), "
-                         f"scaled to the {ref_iters} iterations the FP64 GPU solve needs"}
+        cpu_mg = {"value": round(1.0 / (setup + s_per_iter * ref_iters), 6),
+                  "unit": f"solves/s @{n}^3", "cores": O.num_threads(), "kind": "port",
+                  "sample": f"{it} FP64 {name} iterations of the oracle (OpenMP) on the "
+                            f"{n}^3 system ({dt:.1f}s: setup {setup:.1f}s, {s_per_iter:.3f} "
+                            f"s/iter), scaled to the {ref_iters} iterations the FP64 GPU "
+                            f"solve needs; the reference has no preconditioned CPU solver, "
+                            f"so this is this build's algorithm on the CPU"}
+    del expr, rho, f
+
+    c2 = run_c2(args) if args.c2 else None
+    c1, cpu = run_c1(args) if args.c1 else (None, None)
+    batched = run_batched(args, precond) if args.batch_systems > 0 else None
+    served = run_served(args, precond) if args.serve_requests > 0 else None
 
     pc_name = (f"multigrid (V(2,2), W on the top {L.mg_w_auto(ncell)} levels, "
                f"the automatic choice at {ncell} cells)"
@@ -426,11 +580,15 @@ def main():
         "mixed_speedup_vs_fp64": (None if not fp64 else
                                   round(value / fp64["solves_per_s"], 3) if main_prec == L.MIXED
                                   else round(fp64["solves_per_s"] / value, 3)),
+        "error_vs_fp64": error_vs_fp64,
         "roofline": roofline,
         "kernels": kernels,
+        "c2_fp64_jacobi_pcg": c2,
+        "c1_reference_cg": c1,
         "batched": batched,
         "served": served,
         "cpu_baseline": cpu,
+        "cpu_512_multigrid_port": cpu_mg,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -38,8 +38,13 @@ enum {
   APHPOIS_CONJUGATE = 0,
   /* SolverJacobi restated on the GPU (src/linear/linear.ipp:158-217). */
   APHPOIS_JACOBI = 1,
-  /* Jacobi-preconditioned CG with reliable updates in FP64 (new; symmetric
-   * systems with a same-sign diagonal). */
+  /* Preconditioned CG (Jacobi scaling or multigrid) with reliable updates in
+   * FP64 (new; symmetric systems with a same-sign diagonal).  The stop test
+   * runs at every FP64 true-residual (reliable) update: it stops once
+   * iter >= miniter and (the residual meets tol or iter > maxiter), the
+   * reference's rule, so at most maxiter + 1 iterations; unlike the
+   * reference it may stop at iteration 0 when the initial guess already
+   * meets tol and miniter is 0. */
   APHPOIS_PCG = 2
 };
 
@@ -119,6 +124,11 @@ typedef struct aphpois_info {
   int refinements;    /* PCG: FP64 true-residual (reliable) updates */
   int status;         /* APHPOIS_OK or APHPOIS_ENUMERIC on breakdown */
   double seconds;     /* wall time of the solve on the device stream */
+  double max_diff;    /* max |x - xref| against the reference solution set
+                         by aphpois_set_solution_ref, the mean of x - xref
+                         removed when the system is singular (the max_diff of
+                         the reference's linear test,
+                         src/test/linear/main.cpp:107-128); -1 if none */
 } aphpois_info;
 
 typedef struct aphpois aphpois;
@@ -157,6 +167,15 @@ int aphpois_solve(aphpois* h, const double* x0, double* x, long long start,
 /* Same with dense device arrays (x0_dev may be NULL or equal x_dev). */
 int aphpois_solve_device(aphpois* h, const double* x0_dev, double* x_dev,
                          aphpois_info* info);
+
+/* Sets the reference solution that every later solve of the handle is
+ * compared with (aphpois_info.max_diff): an FP64 solution of the resident
+ * system, e.g. of an FP64 solve or the exact solution of a test problem
+ * (the u* of src/test/linear/main.cpp:45-60).  Host scalar field with the
+ * strides of aphpois_solve; xref NULL removes it.  Kept in device memory
+ * (8 bytes per cell, counted in the budget). */
+int aphpois_set_solution_ref(aphpois* h, const double* xref, long long start,
+                             long long lead_y, long long lead_z);
 
 /* Replaces Solver::SetConf (src/linear/linear.h:48-50). */
 int aphpois_set_conf(aphpois* h, double tol, int miniter, int maxiter);

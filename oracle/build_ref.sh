@@ -30,3 +30,16 @@ if [ -f "$REPO/aphros_amd/libaphpois.so" ]; then
     -Wl,-rpath,'$ORIGIN/../../aphros_amd'
   mv "$OUT/ref_hip_driver.tmp" "$OUT/ref_hip_driver"
 fi
+
+# oracle/_ref/ref_blocks_driver: the reference's modules and the "hip" module
+# on a rank split into several blocks that share one lead (the local
+# backend's exchanges between stages, served by the harness).
+if [ -f "$REPO/aphros_amd/libaphpois.so" ]; then
+  g++ $FLAGS -I"$REPO/include" -I"$REPO/aphros_amd/adapter" -o "$OUT/ref_blocks_driver.tmp" \
+    "$HERE/ref_blocks_driver.cpp" "$REPO/aphros_amd/adapter/linear_hip.cpp" \
+    "$REF/linear/linear.cpp" "$REF/geom/mesh.cpp" "$REF/util/suspender.cpp" \
+    "$REF/parse/vars.cpp" "$REF/util/logger.cpp" "$REF/util/format.cpp" \
+    -L"$REPO/aphros_amd" -laphpois -L/opt/rocm/lib -Wl,-rpath-link,/opt/rocm/lib \
+    -Wl,-rpath,'$ORIGIN/../../aphros_amd'
+  mv "$OUT/ref_blocks_driver.tmp" "$OUT/ref_blocks_driver"
+fi

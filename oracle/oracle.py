@@ -343,8 +343,9 @@ def ref_available():
     return os.path.exists(REF_DRIVER)
 
 
-def ref_solve(name, expr, tol, miniter, maxiter, maxnorm=0, x0=None):
-    """Runs the reference's SolverConjugate/SolverJacobi (oracle/_ref)."""
+def ref_solve(name, expr, tol, miniter, maxiter, maxnorm=0, x0=None, timed=False):
+    """Runs the reference's SolverConjugate/SolverJacobi (oracle/_ref);
+    returns (x, iter, residual) and, if timed, the solve's wall seconds."""
     expr = np.ascontiguousarray(expr, np.float64)
     nz, ny, nx = expr.shape[:3]
     with tempfile.TemporaryDirectory() as d:
@@ -360,4 +361,6 @@ def ref_solve(name, expr, tol, miniter, maxiter, maxnorm=0, x0=None):
             check=True, stdout=subprocess.PIPE, text=True).stdout
         x = np.fromfile(po).reshape(nz, ny, nx)
     kv = dict(t.split("=") for t in out.split())
+    if timed:
+        return x, int(kv["iter"]), float(kv["res"]), float(kv["sec"])
     return x, int(kv["iter"]), float(kv["res"])

@@ -12,13 +12,15 @@
 // usage: ref_driver <conjugate|jacobi> nx ny nz tol miniter maxiter maxnorm
 //                   expr.bin x0.bin|- x_out.bin
 // expr.bin: nx*ny*nz*8 doubles, x-fastest cells, Expr components e0..e7.
-// Prints "iter=<int> res=<%.17e>" on stdout.
+// Prints "iter=<int> res=<%.17e> sec=<solve wall seconds>" on stdout (sec:
+// the stage loop only, without file input and output).
 //
 // Built with -DAPH_HIP_MODULE (ref_hip_driver, linked with libaphpois and the
 // "hip" module of aphros_amd/adapter): the solver is created through the
 // reference's module registry, ModuleLinear<M>::GetInstance(name)->Make(var,
 // "symm", m) (src/util/linear.ipp:9-28), so name may also be "hip"; extra
 // arguments key=value set Vars (e.g. linsolver_hip_method=pcg).
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -150,6 +152,7 @@ int main(int argc, char** argv) {
 #endif
 
   typename linear::Solver<M>::Info info{};
+  const auto t0 = std::chrono::steady_clock::now();
   do {
     info = solver->Solve(fcs, has_x0 ? &sol : nullptr, sol, m);
     for (auto& req : m.GetComm()) {
@@ -164,6 +167,8 @@ int main(int argc, char** argv) {
     m.ClearReduce();
     m.ClearBcastFromLead();  // one block: the lead's values are in place
   } while (m.Pending());
+  const double sec =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 
   std::vector<double> out(ncells);
   for (auto c : m.Cells()) {
@@ -172,6 +177,6 @@ int main(int argc, char** argv) {
   }
   std::ofstream f(argv[11], std::ios::binary);
   f.write(reinterpret_cast<const char*>(out.data()), ncells * sizeof(double));
-  std::printf("iter=%d res=%.17e\n", info.iter, info.residual);
+  std::printf("iter=%d res=%.17e sec=%.6f\n", info.iter, info.residual, sec);
   return 0;
 }

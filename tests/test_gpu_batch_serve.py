@@ -135,3 +135,50 @@ def test_server_admission_by_budget_and_cancel():
     d = srv.wait(r4, timeout=120)
     assert d["status"] == L.EBUDGET
     srv.close()
+
+
+def test_server_stats_while_running_and_destroy_with_waiter():
+    """stats() polled from another thread while requests are admitted,
+    stepped and completed; destroy() with a thread blocked in an unbounded
+    wait on an admitted request releases that thread."""
+    import ctypes as C
+    import threading
+    import time
+    srv = Server(precision=L.MIXED, tol=1e-8, precond=L.PRECOND_MG)
+    systems = [_sys(*p) for p in SYSTEMS]
+    polled = []
+    stop = threading.Event()
+
+    def poll():
+        while not stop.is_set():
+            polled.append(srv.stats())
+
+    t = threading.Thread(target=poll)
+    t.start()
+    ids = [srv.submit(E, nsteps=3, perturb=0.01, seed=60 + k) for k, E in enumerate(systems)]
+    ids.append(srv.submit(_sys(256, 16, 0.05, 0.15, 1000.0, 61), nsteps=2, seed=62))
+    res = [srv.wait(i, timeout=120) for i in ids]
+    stop.set()
+    t.join(timeout=30)
+    assert not t.is_alive()
+    assert all(r["status"] == 0 for r in res)
+    assert len(polled) > 0 and max(p["device_bytes"] for p in polled) > 0
+    assert polled[-1]["completed"] <= len(ids)
+    # a long request and a thread waiting on it without a time limit
+    rid = srv.submit(_sys(256, 16, 0.05, 0.15, 1000.0, 63), nsteps=200, perturb=0.01, seed=64)
+    out = {}
+
+    def waiter():
+        res = L.Result()
+        out["rc"] = L.lib().aphpois_server_wait(srv.h, rid, -1.0, C.byref(res))
+        out["status"] = res.status
+
+    w = threading.Thread(target=waiter)
+    w.start()
+    time.sleep(1.0)
+    srv.close()
+    w.join(timeout=60)
+    assert not w.is_alive()
+    print(f"destroy released the waiter: rc={out.get('rc')} status={out.get('status')}",
+          flush=True)
+    assert out["rc"] == L.OK and out["status"] in (L.OK, L.ECANCEL)

@@ -1,0 +1,238 @@
+"""GPU parity of the branches of the reference's solvers that the golden
+fixtures pin: the max-norm stop rule (linsolver_symm_maxnorm,
+src/linear/linear.ipp:103-104), a warm start from a given initial guess
+(fc_init, src/linear/linear.ipp:43-44), the iteration limits of the stop rule
+(src/linear/linear.ipp:109-110), the reference test's max_diff
+(src/test/linear/main.cpp:107-128) and system files
+(src/test/linear/main.cpp:181-189).
+
+The fixtures (tests/golden/ref_fixtures.npz) were written by the reference's
+own SolverConjugate compiled from its sources; the oracle reproduces them bit
+for bit (test_oracle.py).  CONJUGATE on the GPU sums its dot products in a
+different order, so iteration counts are equal and residuals agree to 1e-6
+relative, solutions to 1e-9.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from aphros_amd import _lib as L
+from aphros_amd import synth, sysio
+from aphros_amd.solver import Poisson
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_fixtures.npz")
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def G():
+    return np.load(GOLD, allow_pickle=False)
+
+
+def test_conjugate_maxnorm_matches_reference(G):
+    E = G["tlin16_expr"]
+    h3 = (1.0 / 16) ** 3
+    _say("conjugate maxnorm tlin16")
+    s = Poisson(E.shape[:3], method=L.CONJUGATE, resmode=L.RES_MAX, tol=1e-4, miniter=0,
+                maxiter=1000, h3=h3)
+    s.set_system(E)
+    x, info = s.solve(None)
+    s.close()
+    assert info["iter"] == int(G["tlin16_maxnorm_iter"])
+    ref = float(G["tlin16_maxnorm_res"])
+    assert abs(info["residual"] - ref) <= 1e-6 * ref
+    assert info["residual"] == info["res_max"]
+    xo, ito, reso, _ = O.conjugate(E, h3, tol=1e-4, miniter=0, maxiter=1000, maxnorm=True)
+    assert ito == info["iter"]
+    assert np.abs(x - xo).max() < 1e-9
+
+
+def test_conjugate_warm_start_matches_reference(G):
+    E = G["tlin16_expr"]
+    x0 = G["tlin16_x0"]
+    h3 = (1.0 / 16) ** 3
+    _say("conjugate warm start tlin16")
+    s = Poisson(E.shape[:3], method=L.CONJUGATE, resmode=L.RES_L2, tol=1e-6, miniter=0,
+                maxiter=1000, h3=h3)
+    s.set_system(E)
+    x, info = s.solve(x0)
+    assert info["iter"] == int(G["tlin16_warm_iter"])
+    ref = float(G["tlin16_warm_res"])
+    assert abs(info["residual"] - ref) <= 1e-6 * ref
+    assert np.abs(x - G["tlin16_warm_x"]).max() < 1e-9
+    # fc_init aliasing fc_sol (the reference allows it): same result
+    xa = x0.copy()
+    x2, info2 = s.solve(xa, out=xa)
+    assert x2 is xa and info2["iter"] == info["iter"]
+    assert np.array_equal(x2, x)
+    s.close()
+
+
+def test_jacobi_warm_start_bitwise():
+    E, ue, h = O.tlinear_system(16)
+    x0 = np.random.default_rng(8).uniform(-1, 1, E.shape[:3])
+    _say("jacobi warm start tlin16")
+    xo, ito, reso, _ = O.jacobi(E, x0=x0, tol=1e-5, miniter=0, maxiter=2000)
+    s = Poisson(E.shape[:3], method=L.JACOBI, resmode=L.RES_L2, tol=1e-5, miniter=0,
+                maxiter=2000, h3=h ** 3)
+    s.set_system(E)
+    x, info = s.solve(x0)
+    s.close()
+    assert info["iter"] == ito and info["residual"] == reso
+    assert np.array_equal(x, xo)
+
+
+@pytest.mark.parametrize("method", [L.CONJUGATE, L.JACOBI])
+def test_reference_iteration_limits(method):
+    """iter >= miniter && (iter > maxiter || res < tol): maxiter caps at
+    maxiter + 1 iterations, miniter forces at least miniter."""
+    E, ue, h = O.tlinear_system(16)
+    run = O.conjugate if method == L.CONJUGATE else (
+        lambda e, h3, **k: O.jacobi(e, **k))
+    for tol, miniter, maxiter in ((0.0, 0, 5), (1e-1, 40, 1000), (1e-1, 7, 3)):
+        _, ito, _, _ = run(E, h ** 3, tol=tol, miniter=miniter, maxiter=maxiter)
+        s = Poisson(E.shape[:3], method=method, resmode=L.RES_L2, tol=tol, miniter=miniter,
+                    maxiter=maxiter, h3=h ** 3)
+        s.set_system(E)
+        _, info = s.solve(None)
+        s.close()
+        _say(f"  method={method} tol={tol} miniter={miniter} maxiter={maxiter}: "
+             f"gpu {info['iter']} oracle {ito}")
+        assert info["iter"] == ito
+
+
+@pytest.mark.parametrize("precond", [L.PRECOND_JACOBI, L.PRECOND_MG])
+def test_pcg_iteration_limits(precond):
+    """PCG follows the same rule: tol 0 stops after maxiter + 1 iterations,
+    unconverged; miniter keeps a loose tolerance from stopping early."""
+    e, _, _ = synth.config("c1_64")
+    E = e.reshape(64, 64, 64, 8)
+    s = Poisson(E.shape[:3], method=L.PCG, precision=L.MIXED, resmode=L.RES_REL_TRUE,
+                tol=0.0, miniter=0, maxiter=5, precond=precond)
+    s.set_system(E)
+    _, info = s.solve(None)
+    assert info["iter"] == 6 and info["converged"] == 0
+    s.set_conf(1e-2, 0, 1000)
+    _, info = s.solve(None)
+    free = info["iter"]
+    assert info["converged"] == 1
+    s.set_conf(1e-2, free + 9, 1000)
+    _, info = s.solve(None)
+    _say(f"  precond={precond}: loose tol {free} iterations, miniter {free + 9}: "
+         f"{info['iter']}")
+    assert info["iter"] >= free + 9 and info["converged"] == 1
+    s.close()
+
+
+def test_max_diff_is_the_reference_test_measure():
+    """info.max_diff against u* of the reference's linear test equals
+    max|d - mean d|, d = x - u* (src/test/linear/main.cpp:107-128), and
+    satisfies the reference's own rule (src/test/linear/test:29)."""
+    E, ue, h = O.tlinear_system(32)
+    s = Poisson(E.shape[:3], method=L.CONJUGATE, resmode=L.RES_L2, tol=1e-5, miniter=0,
+                maxiter=1000, h3=h ** 3)
+    s.set_system(E)
+    _, info = s.solve(None)
+    assert info["max_diff"] == -1.0
+    s.set_solution_ref(ue)
+    x, info = s.solve(None)
+    want = O.max_diff_mean_removed(x, ue)
+    _say(f"  max_diff gpu {info['max_diff']:.6e} numpy {want:.6e}")
+    assert np.isclose(info["max_diff"], want, rtol=1e-10, atol=0)
+    assert info["max_diff"] < 2 * 6.926950e-03 + 1e-12
+    s.set_solution_ref(None)
+    _, info = s.solve(None)
+    assert info["max_diff"] == -1.0
+    s.close()
+
+
+def test_max_diff_mixed_against_fp64_solution():
+    e, _, _ = synth.config("c1_64")
+    E = e.reshape(64, 64, 64, 8)
+    out = {}
+    for prec in (L.FP64, L.MIXED):
+        s = Poisson(E.shape[:3], method=L.PCG, precision=prec, resmode=L.RES_REL_TRUE,
+                    tol=1e-8, precond=L.PRECOND_MG)
+        s.set_system(E)
+        if prec == L.MIXED:
+            s.set_solution_ref(out[L.FP64][0])
+        out[prec] = s.solve(None)
+        s.close()
+    x64 = out[L.FP64][0]
+    x, info = out[L.MIXED]
+    d = x - x64
+    want = np.abs(d - d.mean()).max()
+    _say(f"  mixed vs fp64: max_diff {info['max_diff']:.3e} (max|x| {np.abs(x64).max():.3e})")
+    assert np.isclose(info["max_diff"], want, rtol=1e-9, atol=1e-300)
+    assert info["max_diff"] <= 1e-5 * np.abs(x64).max()
+
+
+@pytest.mark.parametrize("method", [L.CONJUGATE, L.PCG])
+def test_system_file_round_trip_solve(tmp_path, method):
+    """The (nz, ny, nx, 8) system file written and read back through the C
+    ABI solves exactly like the in-memory system."""
+    e, _, _ = synth.config("c1_64")
+    E = e.reshape(64, 64, 64, 8)
+    path = str(tmp_path / "c1.npy")
+    sysio.save_system(path, E)
+    F = sysio.load_system(path)
+    assert F.shape == E.shape and np.array_equal(F, E)
+    res = []
+    for sysm in (E, F):
+        if method == L.CONJUGATE:
+            s = Poisson(E.shape[:3], method=L.CONJUGATE, resmode=L.RES_L2, tol=0.0, miniter=0,
+                        maxiter=60, h3=(1 / 64) ** 3)
+        else:
+            s = Poisson(E.shape[:3], method=L.PCG, precision=L.MIXED, tol=1e-8,
+                        precond=L.PRECOND_MG)
+        s.set_system(sysm)
+        res.append(s.solve(None))
+        s.close()
+    (x1, i1), (x2, i2) = res
+    assert i1["iter"] == i2["iter"] and np.array_equal(x1, x2)
+
+
+def test_python_module_hip_through_the_registry():
+    """The Python mirror of the plugin interface: MakeLinearSolver(var,
+    "symm", m) with linsolver_symm = hip, on fields over the block's index
+    box (halos included).  CONJUGATE equals the oracle's SolverConjugate;
+    a second Solve with the same coefficients uploads only e7 and equals a
+    fresh solver's result."""
+    from aphros_amd import linear
+    E, ue, h = O.tlinear_system(16)
+    m = linear.Mesh((16, 16, 16), hl=2)
+    start, ly, lz = m.strides()
+    bx, by, bz = m.box()
+
+    def boxed(a, comps):
+        f = np.zeros((bz, by, bx) + ((comps,) if comps > 1 else ()))
+        f[2:18, 2:18, 2:18] = a
+        return f
+
+    var = linear.Vars(Int={"hypre_symm_maxiter": 1000, "hypre_symm_miniter": 0},
+                      Double={"hypre_symm_tol": 1e-6}, String={"linsolver_symm": "hip"})
+    s = linear.MakeLinearSolver(var, "symm", m)
+    fs = boxed(E, 8)
+    sol = np.zeros(m.nslots())
+    info = s.Solve(fs, None, sol, m)
+    x = sol.reshape(bz, by, bx)[2:18, 2:18, 2:18]
+    xo, ito, reso, _ = O.conjugate(E, h ** 3, tol=1e-6, miniter=0, maxiter=1000)
+    assert info.iter == ito and np.abs(x - xo).max() < 1e-9
+    E2 = E.copy()
+    E2[..., 7] *= -0.5
+    fs2 = boxed(E2, 8)
+    info2 = s.Solve(fs2, None, sol, m)
+    assert (s.uploads, s.rhs_uploads) == (1, 1)
+    s2 = linear.MakeLinearSolver(var, "symm", m)
+    sol2 = np.zeros(m.nslots())
+    info3 = s2.Solve(fs2, None, sol2, m)
+    assert info2.iter == info3.iter and np.array_equal(sol, sol2)
+    s.close()
+    s2.close()

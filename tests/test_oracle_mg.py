@@ -10,8 +10,8 @@ mathematics fixes instead:
     in the stencil coarsening, the restriction or the prolongation fails it;
   * symmetry and positive definiteness of M, which CG relies on;
   * MG-PCG reaching the tolerance on the true residual (orc_true_relres is
-    pinned against the reference in test_oracle.py) and agreeing with the
-    Jacobi-PCG solution.
+    pinned against a dense numpy residual, closed forms and planted mutants
+    in test_oracle_relres.py) and agreeing with the Jacobi-PCG solution.
 """
 import numpy as np
 import pytest

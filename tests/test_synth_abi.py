@@ -101,23 +101,48 @@ def test_assembled_system_invariants():
 
 
 def test_plugin_registry_and_conf():
+    """The Python module "hip" reads the C++ module's keys with its defaults
+    (aphros_amd/adapter/linear_hip.h ModuleLinearHip::Make)."""
     names = linear.ModuleLinear.GetInstances()
-    assert {"hip", "hip_conjugate", "hip_jacobi"} <= set(names)
+    assert "hip" in names
     var = linear.Vars(Int={"hypre_symm_maxiter": 50, "linsolver_symm_maxnorm": 1},
                       Double={"hypre_symm_tol": 1e-6},
-                      String={"linsolver_symm": "hip_conjugate"})
+                      String={"linsolver_symm": "hip"})
     conf = linear.ModuleLinear.GetConf(var, "symm")
     assert (conf.tol, conf.maxiter, conf.miniter) == (1e-6, 50, 0)
     m = linear.Mesh((8, 8, 8), hl=2)
     s = linear.MakeLinearSolver(var, "symm", m)
-    assert s.method == L.CONJUGATE and s.resmode == L.RES_MAX
+    assert (s.method, s.resmode, s.precision, s.reuse) == (L.CONJUGATE, L.RES_MAX, L.FP64, True)
+    var.String["linsolver_hip_method"] = "pcg"
+    var.String["linsolver_hip_precision"] = "mixed"
+    var.Int["linsolver_hip_true_relres"] = 1
+    var.Int["linsolver_hip_reuse"] = 0
+    s = linear.MakeLinearSolver(var, "symm", m)
+    assert (s.method, s.resmode, s.precision, s.precond, s.reuse) == \
+        (L.PCG, L.RES_REL_TRUE, L.MIXED, L.PRECOND_MG, False)
+    var.String["linsolver_hip_precond"] = "jacobi"
+    assert linear.MakeLinearSolver(var, "symm", m).precond == L.PRECOND_JACOBI
     var.String["linsolver_symm"] = "nope"
     with pytest.raises(RuntimeError, match="Solver not found"):
         linear.MakeLinearSolver(var, "symm", m)
     var.String["linsolver_symm"] = "hip"
-    var.String["linsolver_symm_precision"] = "fp16"
+    var.String["linsolver_hip_precision"] = "fp16"
     with pytest.raises(RuntimeError):
         linear.MakeLinearSolver(var, "symm", m)
+    var.String["linsolver_hip_precision"] = "fp64"
+    var.String["linsolver_hip_method"] = "sor"
+    with pytest.raises(RuntimeError, match="unknown method"):
+        linear.MakeLinearSolver(var, "symm", m)
+
+
+def test_python_and_cpp_modules_read_the_same_keys():
+    """Every linsolver_hip_* key the C++ module reads is read by the Python
+    module, and no other."""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    cpp = open(os.path.join(root, "aphros_amd", "adapter", "linear_hip.h")).read()
+    py = open(os.path.join(root, "aphros_amd", "linear.py")).read()
+    keys = lambda t: set(re.findall(r'"(linsolver_hip_\w+)"', t))
+    assert keys(cpp) == keys(py) and len(keys(cpp)) >= 6
 
 
 def test_mesh_strides_and_halo_fill():
