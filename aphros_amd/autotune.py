@@ -3,14 +3,17 @@
 For a grid edge and precision it times the SpMV kernel (device events, a few
 hundred iterations on a synthetic system of that size) for each candidate
 launch shape (threads per row lx, planes per workgroup bz) and keeps the
-fastest.  Results are cached in a JSON file keyed by precision and edge
-($APHPOIS_TUNE_CACHE, default ~/.cache/aphpois/tiles.json).
+fastest.  Results are cached in a JSON file keyed by the device (its name
+and compute-unit count), the precision, the preconditioner and the edge
+($APHPOIS_TUNE_CACHE, default ~/.cache/aphpois/tiles.json), so a tile tuned on
+one GPU model is not reused on another.
 """
 import json
 import os
 
 from . import _lib as L
 from . import synth
+from .devmem import device_name
 from .solver import Poisson
 
 CANDIDATES = [(64, 0), (32, 0), (16, 0), (8, 0), (64, 8), (64, 32), (64, 64), (32, 16),
@@ -41,11 +44,12 @@ def _save(d):
         pass
 
 
-def tune_edge(n, precision=L.MIXED, iters=150, candidates=None, precond=L.PRECOND_JACOBI):
+def tune_edge(n, precision=L.MIXED, iters=150, candidates=None, precond=L.PRECOND_JACOBI,
+              device=0):
     """Returns ((lx, bz), {candidate: spmv ms})."""
     e, _, _ = synth.system(n, 2, 0.1, 0.2, 100.0, 12345)
     s = Poisson((n, n, n), method=L.PCG, precision=precision, resmode=L.RES_REL_TRUE,
-                tol=1e-30, maxiter=iters, precond=precond)
+                tol=1e-30, maxiter=iters, precond=precond, device=device)
     s.set_system(e)
     times = {}
     for lx, bz in candidates or CANDIDATES:
@@ -61,12 +65,16 @@ def tune_edge(n, precision=L.MIXED, iters=150, candidates=None, precond=L.PRECON
     return best, times
 
 
-def tile_for(n, precision=L.MIXED, retune=False, precond=L.PRECOND_JACOBI):
-    key = f"p{precision}_c{precond}_n{n}"
+def cache_key(n, precision, precond, device=0):
+    return f"{device_name(device)}|p{precision}_c{precond}_n{n}"
+
+
+def tile_for(n, precision=L.MIXED, retune=False, precond=L.PRECOND_JACOBI, device=0):
+    key = cache_key(n, precision, precond, device)
     d = _load()
     if key in d and not retune:
         return tuple(d[key]["tile"])
-    best, times = tune_edge(n, precision, precond=precond)
+    best, times = tune_edge(n, precision, precond=precond, device=device)
     d[key] = {"tile": list(best), "ms": {f"{a}x{b}": t for (a, b), t in times.items()}}
     _save(d)
     return best

@@ -20,7 +20,23 @@ def hip():
         _hip.hipFree.argtypes = [C.c_void_p]
         _hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
         _hip.hipDeviceSynchronize.argtypes = []
+        _hip.hipDeviceGetName.argtypes = [C.c_char_p, C.c_int, C.c_int]
+        _hip.hipDeviceGetAttribute.argtypes = [C.POINTER(C.c_int), C.c_int, C.c_int]
     return _hip
+
+
+def device_name(device=0):
+    """Marketing name and compute-unit count of a HIP device, e.g. 'AMD
+    Instinct MI355X/256cu' (the autotuner's cache key)."""
+    buf = C.create_string_buffer(256)
+    rc = hip().hipDeviceGetName(buf, 256, device)
+    if rc != 0:
+        raise RuntimeError(f"hipDeviceGetName({device}) failed: {rc}")
+    cus = C.c_int()
+    # hipDeviceAttributeMultiprocessorCount
+    HIP_ATTR_CU = 63
+    rc = hip().hipDeviceGetAttribute(C.byref(cus), HIP_ATTR_CU, device)
+    return f"{buf.value.decode(errors='replace').strip()}/{cus.value if rc == 0 else '?'}cu"
 
 
 class DeviceBuffer:

@@ -305,6 +305,12 @@ def run_served(args, precond):
     specs = synth.serve_specs(args.serve_requests, args.serve_rate)
     systems = [synth.spec_system(sp)[0] for sp in specs]
     srv = Server(precision=L.MIXED, tol=1e-8, mem_budget=200e9, precond=precond)
+    # reference point: a request of the largest edge alone on the idle server
+    big = max(range(len(specs)), key=lambda i: specs[i]["n"])
+    ra = srv.submit(systems[big], nsteps=10, perturb=0.01, seed=specs[big]["seed"])
+    r_alone = srv.wait(ra, timeout=120)
+    alone_s = r_alone["t_done"] - r_alone["t_submit"]
+    log(f"[bench]   served: a {specs[big]['n']}^3 request alone: {alone_s:.3f}s")
     t0 = time.perf_counter()
     ids = []
     for sp, e in zip(specs, systems):
@@ -339,6 +345,13 @@ def run_served(args, precond):
     solves = sum(r["steps_done"] for r in res)
     log(f"[bench] served: {len(res)} requests, {solves} solves in {t_last - t_first:.2f}s, "
         f"p50 {np.percentile(lat, 50):.3f}s p99 {np.percentile(lat, 99):.3f}s")
+    # latency per grid edge (size class)
+    by_edge = {}
+    for sp, r in zip(specs, res):
+        by_edge.setdefault(sp["n"], []).append(r["t_done"] - r["t_submit"])
+    per_size = {str(n): {"requests": len(v), "p50_request_s": round(float(np.percentile(v, 50)), 4),
+                         "p99_request_s": round(float(np.percentile(v, 99)), 4)}
+                for n, v in sorted(by_edge.items())}
     return {"requests": len(res), "solves": solves,
             "solves_per_s": round(solves / (t_last - t_first), 3),
             "p50_request_s": round(float(np.percentile(lat, 50)), 4),
@@ -346,6 +359,9 @@ def run_served(args, precond):
             "p50_solve_s": round(float(np.percentile(per_solve, 50)), 5),
             "p99_solve_s": round(float(np.percentile(per_solve, 99)), 5),
             "max_last_relres": max(r["last_relres"] for r in res),
+            "by_edge": per_size,
+            "alone_largest_request_s": round(alone_s, 4),
+            "p99_over_alone_largest": round(float(np.percentile(lat, 99)) / alone_s, 3),
             "wall_s": round(wall, 3),
             "workload": f"{len(res)} requests, Poisson arrivals at {args.serve_rate}/s, edges "
                         f"from {synth.SERVE_EDGES}, 8 spheres, density ratio 1000, 10 "

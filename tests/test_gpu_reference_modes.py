@@ -48,7 +48,9 @@ def test_conjugate_maxnorm_matches_reference(G):
     assert info["iter"] == int(G["tlin16_maxnorm_iter"])
     ref = float(G["tlin16_maxnorm_res"])
     assert abs(info["residual"] - ref) <= 1e-6 * ref
-    assert info["residual"] == info["res_max"]
+    # residual: max|r|/h3 of the recursive residual (the reference's
+    # Info::residual); res_max: of the true residual recomputed at the end
+    assert abs(info["residual"] - info["res_max"]) <= 1e-6 * info["residual"]
     xo, ito, reso, _ = O.conjugate(E, h3, tol=1e-4, miniter=0, maxiter=1000, maxnorm=True)
     assert ito == info["iter"]
     assert np.abs(x - xo).max() < 1e-9
