@@ -11,6 +11,7 @@
 
 #include <atomic>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -79,6 +80,13 @@ class Engine {
   // *used as well, so the engines' sum stays within budget.  Call before the
   // first allocation.
   void share_budget(std::atomic<long long>* used) { shared_ = used; }
+  // A lock shared with other engines whose threads use the same device (the
+  // server's lanes): step() holds it while it rebuilds the launch tables and
+  // captures the iteration graph, and the other threads hold it around every
+  // call that may allocate, free or synchronise outside their own stream, so
+  // that no such call runs while a stream of this device is being captured
+  // (it would invalidate the capture).  Graph replays run without it.
+  void share_api_lock(std::mutex* mu) { api_mu_ = mu; }
   int add_system(int nx, int ny, int nz, double h3, int* slot);
   int set_system(int slot, const double* expr, bool on_device, long long start, long long ly,
                  long long lz);
@@ -204,6 +212,7 @@ class Engine {
   int check_every_ = 0;
   long long budget_ = 0;
   std::atomic<long long>* shared_ = nullptr;
+  std::mutex* api_mu_ = nullptr;
   int capacity_ = 0;
   int vx_ = 1;
   Tuning tuning_;

@@ -1199,6 +1199,8 @@ int Engine::step(int max_replays, std::vector<int>* finished) {
   for (auto& s : slots_) any = any || (s && s->active);
   if (!any) return APHPOIS_OK;
   if (dirty_) {
+    std::unique_lock<std::mutex> lk;
+    if (api_mu_) lk = std::unique_lock<std::mutex>(*api_mu_);
     APH_TRY(build_tables());
     APH_TRY(capture());
   }

@@ -87,6 +87,9 @@ struct aphpois_server {
   // are destroyed
   std::atomic<long long> used{0};       // device bytes reserved by all lanes
   std::atomic<long long> mem_epoch{0};  // bumped whenever a lane frees storage
+  // held by a lane for every engine call except its graph replays, and by an
+  // engine while it captures (Engine::share_api_lock): lock order api_mu, mu
+  std::mutex api_mu;
   ServerLane lanes[kLanes];
   long long next_id = 1;
   bool stop = false;
@@ -220,16 +223,20 @@ void ServerLane::loop() {
       });
       if (srv->stop) break;
     }
-    admit();
-    // cancellation of admitted requests
-    std::vector<std::shared_ptr<Req>> cancelled;
     {
-      std::lock_guard<std::mutex> lk(srv->mu);
-      for (auto& kv : by_slot)
-        if (kv.second->cancel) cancelled.push_back(kv.second);
+      std::lock_guard<std::mutex> al(srv->api_mu);
+      admit();
+      // cancellation of admitted requests
+      std::vector<std::shared_ptr<Req>> cancelled;
+      {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        for (auto& kv : by_slot)
+          if (kv.second->cancel) cancelled.push_back(kv.second);
+      }
+      for (auto& q : cancelled) finish(q, APHPOIS_ECANCEL);
     }
-    for (auto& q : cancelled) finish(q, APHPOIS_ECANCEL);
     if (by_slot.empty()) continue;
+    // the replays run without api_mu (step takes it to rebuild and capture)
     const int rc = eng.step(1, &fin);
     if (rc != APHPOIS_OK) {
       // the device state is not trusted: fail this lane's requests and stop
@@ -252,6 +259,7 @@ void ServerLane::loop() {
       srv->cv_done.notify_all();
       break;
     }
+    std::lock_guard<std::mutex> al(srv->api_mu);
     for (int slot : fin) {
       auto it = by_slot.find(slot);
       if (it == by_slot.end()) continue;
@@ -339,6 +347,7 @@ int aphpois_server_create(const aphpois_server_desc* d, aphpois_server** out) {
     l.srv = s;
     l.index = i;
     l.eng.share_budget(&s->used);
+    l.eng.share_api_lock(&s->api_mu);
     const int rc = l.eng.init(d->device, APHPOIS_PCG, d->precision, APHPOIS_RES_REL_TRUE,
                               d->tol, 0, d->maxiter, APHPOIS_NULL_AUTO, d->delta,
                               d->check_every, d->mem_budget,
