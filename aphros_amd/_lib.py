@@ -108,6 +108,9 @@ SIGNATURES = {
     "aphpois_set_rhs": (C.c_int, [_vp, _dp, _ll, _ll, _ll]),
     "aphpois_solve": (C.c_int, [_vp, _dp, _dp, _ll, _ll, _ll, C.POINTER(Info)]),
     "aphpois_solve_device": (C.c_int, [_vp, _vp, _vp, C.POINTER(Info)]),
+    "aphpois_solve_device_noalloc": (C.c_int, [C.POINTER(Desc), _vp, _vp, _vp, _vp,
+                                               C.POINTER(Info)]),
+    "aphpois_noalloc_max_cells": (C.c_longlong, []),
     "aphpois_set_conf": (C.c_int, [_vp, C.c_double, C.c_int, C.c_int]),
     "aphpois_set_solution_ref": (C.c_int, [_vp, _dp, _ll, _ll, _ll]),
     "aphpois_device_bytes": (C.c_longlong, [_vp]),

@@ -20,7 +20,8 @@ ORACLE_SRC = os.path.join(ROOT, "oracle", "oracle.c")
 ORACLE_LIB = os.path.join(ROOT, "oracle", "liboracle.so")
 REF_SRC = "/root/reference/src"
 
-HIP_SOURCES = ["kernels.hip", "engine.hip", "api.hip", "server.hip", "synth.cpp", "sysio.cpp"]
+HIP_SOURCES = ["kernels.hip", "engine.hip", "api.hip", "server.hip", "noalloc.hip", "synth.cpp",
+               "sysio.cpp"]
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
     "-O3",

@@ -168,6 +168,26 @@ int aphpois_solve(aphpois* h, const double* x0, double* x, long long start,
 int aphpois_solve_device(aphpois* h, const double* x0_dev, double* x_dev,
                          aphpois_info* info);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): aphpois_create + aphpois_set_system_device +
+ * aphpois_solve_device in one call that allocates nothing.  The same kernels,
+ * tilings and reductions, so x and info equal that path's bit for bit.
+ * desc: method CONJUGATE or JACOBI, resmode RES_L2 or RES_MAX, nx, ny, nz,
+ * h3, tol, miniter, maxiter, device (other fields ignored).  expr_dev: dense
+ * Expr in device memory (8 doubles per cell, x fastest), read in place;
+ * x0_dev: NULL (zero guess), x_dev, or another device array; x_dev: the
+ * solution.  stream: a hipStream_t of that device, NULL for its null stream;
+ * the call returns after the solve has finished on it.  Work space (tables,
+ * partials, work vectors) is one static device array per device, held until
+ * the process ends: at most aphpois_noalloc_max_cells() = 2^21 cells
+ * (EINVAL beyond).  Calls that overlap on one device share that array and
+ * race; info.mean is 0 and info.max_diff -1. */
+int aphpois_solve_device_noalloc(const aphpois_desc* desc, const double* expr_dev,
+                                 const double* x0_dev, double* x_dev, void* stream,
+                                 aphpois_info* info);
+long long aphpois_noalloc_max_cells(void);
+
 /* Sets the reference solution that every later solve of the handle is
  * compared with (aphpois_info.max_diff): an FP64 solution of the resident
  * system, e.g. of an FP64 solve or the exact solution of a test problem

@@ -73,6 +73,33 @@ def test_invalid_descriptors_rejected():
         assert L.lib().aphpois_create(C.byref(d), C.byref(h)) in (L.EINVAL, L.EDEVICE)
 
 
+def test_noalloc_rejects_before_device_work():
+    """The allocation-free solve refuses, on the host and before any device
+    call, what its static work space or its kernels cannot take."""
+    lib = L.lib()
+    assert lib.aphpois_noalloc_max_cells() == 1 << 21
+    info = L.Info()
+    for shape, method, resmode in (((128, 128, 129), L.CONJUGATE, L.RES_L2),
+                                   ((8, 8, 8), L.PCG, L.RES_L2),
+                                   ((8, 8, 8), L.JACOBI, L.RES_REL_TRUE),
+                                   ((0, 8, 8), L.CONJUGATE, L.RES_L2)):
+        d = L.Desc()
+        lib.aphpois_desc_default(C.byref(d))
+        d.nz, d.ny, d.nx = shape
+        d.method, d.resmode = method, resmode
+        rc = lib.aphpois_solve_device_noalloc(C.byref(d), C.c_void_p(256), None,
+                                              C.c_void_p(256), None, C.byref(info))
+        assert rc == L.EINVAL, (shape, method, resmode)
+    d = L.Desc()
+    lib.aphpois_desc_default(C.byref(d))
+    d.nx = d.ny = 128
+    d.nz = 129
+    lib.aphpois_solve_device_noalloc(C.byref(d), C.c_void_p(256), None, C.c_void_p(256), None,
+                                     C.byref(info))
+    assert b"2^21" in lib.aphpois_last_error()
+    assert lib.aphpois_solve_device_noalloc(None, None, None, None, None, None) == L.EINVAL
+
+
 @pytest.mark.parametrize("n,ns,rmin,rmax,ratio,seed", [
     (32, 8, 0.08, 0.15, 1000.0, 1), (48, 5, 0.05, 0.2, 10.0, 4), (20, 30, 0.02, 0.4, 3.0, 9),
     (17, 3, 0.3, 0.45, 1000.0, 123456789)])
