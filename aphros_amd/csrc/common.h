@@ -44,8 +44,9 @@ struct Scalars {
 // Operator of every level: (L u)_c = sum_faces k_f (u_c - u_nb) + m_c u_c.
 struct MgLevel {
   int nx, ny, nz;
-  int post;  // 1: both post-smoothing sweeps fused (k_mg_post); the
-             // pre-smoothed iterate is then z[0], else z[1]
+  int post;  // 1: both post-smoothing sweeps fused (k_mg_post); 2: pre- and
+             // post-smoothing as one tile launch each (k_mg_tile); the
+             // pre-smoothed iterate is then z[0], else (0) z[1]
   long long n;
   // coefficients: FP64 engines: plus-face conductances and D^-1 as separate
   // arrays (kx, ky, kz, dinv); FP32 engines: all four in bfloat16, packed by

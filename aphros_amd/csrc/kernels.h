@@ -43,6 +43,12 @@ struct LaunchSet {
   const Rec* rec_lp[kMaxLevels] = {};
   const int* b2r_lp[kMaxLevels] = {};
   int nlp[kMaxLevels] = {};
+  // small levels whose pre- and post-smoothing run as one tile launch each
+  // (k_mg_tile, MgLevel::post == 2; 8^3 cells per workgroup, tiles_x/y/z)
+  int post_tile[kMaxLevels] = {};
+  const Rec* rec_lt[kMaxLevels] = {};
+  const int* b2r_lt[kMaxLevels] = {};
+  int nlt[kMaxLevels] = {};
   // event-profiled cycles only: the launches below level 0 as one replay of
   // this graph (captured by capture_below0), nullptr: launched one by one
   hipGraphExec_t below0 = nullptr;
@@ -50,6 +56,9 @@ struct LaunchSet {
 
 // threads of a k_mg_post workgroup
 constexpr int kPostBlock = 1024;
+// k_mg_tile: output cells per tile edge and threads per workgroup
+constexpr int kTile = 8;
+constexpr int kTileBlock = 256;
 
 // LDS of the bottom-levels kernel, per level of n cells (element size es):
 // z and a second buffer; with everything resident also the coefficients

@@ -1076,6 +1076,158 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
   }
 }
 
+// Pre- or post-smoothing of a small level (MgLevel::post == 2, FP32 engines,
+// nx % 4 == 0) in one launch: a workgroup owns an 8x8x8 tile of output cells
+// and recomputes what its two sweeps need around it, so the level costs one
+// launch and one round of global loads instead of two z-marching launches
+// with a barrier and a load round per plane.
+//   PRE:  u = wA D^-1 r (12^3 box); v = u + wB D^-1 (r - L u) (10^3 box)
+//         -> z[0] on the tile; r_{l+1}[C] = sum over the 2x2x2 children of
+//         (r - L v)
+//   POST: u = z[0] + kProlong P z_c (12^3 box); v = u + wB D^-1 (r - L u)
+//         (10^3 box); z[1] = v + wA D^-1 (r - L v) (+ zs: the last sweep of
+//         a second coarse correction) on the tile
+// The box is staged in LDS by rows of 16 cells (x0-4 .. x0+11, four aligned
+// groups of 4 cells: r as float4, the packed bfloat16 coefficients as two
+// 16-byte halves, u formed as the row is loaded), y and z from -2 to +9
+// around the tile.  Rows and groups wrap periodically (a level edge shorter
+// than the box repeats cells); tile cells past the level's edge are neither
+// stored nor restricted (edges are even, tiles start at multiples of 8).
+template <bool PRE>
+__global__ __launch_bounds__(kTileBlock) void k_mg_tile(const SysDev* __restrict__ sds,
+                                                        const Rec* __restrict__ recs,
+                                                        const int* __restrict__ b2r, int l,
+                                                        int wl, int so, int addzs) {
+  constexpr int B = kTile + 4;            // box rows per edge (y, z)
+  constexpr int W = kTile + 8;            // cells per staged row (x)
+  constexpr int NR = B * B;               // staged rows
+  constexpr int NG = W / 4;               // 4-cell groups per row
+  constexpr int NT = kTile * kTile * kTile;
+  __shared__ __align__(16) float sR[NR * W];
+  __shared__ __align__(16) float sU[NR * W];
+  __shared__ __align__(16) float sV[NR * W];
+  __shared__ __align__(16) unsigned short sK[NR * NG * 16];  // kx[4] ky[4] kz[4] dinv[4] per group
+  __shared__ float sres[PRE ? NT : 1];
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const Rec rc = recs[b2r[b]];
+  const SysDev& s = sds[rc.sys];
+  if (s.sc->done) return;
+  if (wl >= 0 && wl >= s.mg_w) return;
+  if (so && !s.sc->first) return;
+  const MgLevel& Lv = s.lev[l];
+  const MgLevel& Lc = s.lev[l + 1];
+  const int nx = Lv.nx, ny = Lv.ny, nz = Lv.nz;
+  const int lb = b - rc.blk0;
+  const int x0 = (lb % rc.tiles_x) * kTile;
+  const int t2 = lb / rc.tiles_x;
+  const int y0 = (t2 % rc.tiles_y) * kTile;
+  const int z0 = (t2 / rc.tiles_y) * kTile;
+  const int cnx = Lc.nx, cny = Lc.ny;
+  const gcp<float> r = (gcp<float>)Lv.r;
+  const gcp<float> mm = (gcp<float>)Lv.m;
+  const gcp<unsigned short> kc = (gcp<unsigned short>)Lv.kc;
+  auto wrap = [](int v, int n) {
+    v %= n;
+    return v < 0 ? v + n : v;
+  };
+  // staging: one (row, group) unit per thread and pass
+  for (int u = threadIdx.x; u < NR * NG; u += kTileBlock) {
+    const int ri = u / NG, g = u % NG;
+    const int yi = ri % B, zi = ri / B;
+    const int y = wrap(y0 - 2 + yi, ny), z = wrap(z0 - 2 + zi, nz);
+    const int gx = wrap(x0 - 4 + 4 * g, nx);  // nx % 4 == 0: a whole group
+    const int i = (z * ny + y) * nx + gx;
+    const vec_t<float, 4> rv = *(gcp<vec_t<float, 4>>)(r + i);
+    const gcp<vec_t<unsigned, 4>> kg = (gcp<vec_t<unsigned, 4>>)(kc + ((long long)(i >> 2) << 4));
+    const vec_t<unsigned, 4> k0 = kg[0], k1 = kg[1];
+    vec_t<float, 4> uv;
+    if constexpr (PRE) {
+      // u = wA * dinv * r, dinv from the group's upper half
+      const float d[4] = {__uint_as_float(k1.z << 16), __uint_as_float(k1.z & 0xffff0000u),
+                          __uint_as_float(k1.w << 16), __uint_as_float(k1.w & 0xffff0000u)};
+#pragma unroll
+      for (int v = 0; v < 4; ++v) uv[v] = (float)kOmegaA * d[v] * rv[v];
+    } else {
+      const vec_t<float, 4> zv = *(gcp<vec_t<float, 4>>)((gcp<float>)Lv.z[0] + i);
+      const vec_t<float, 2> cv = *(gcp<vec_t<float, 2>>)((gcp<float>)Lc.z[l + 1 == s.nlev - 1 ? 0 : 1] +
+                                                          ((z >> 1) * cny + (y >> 1)) * cnx + (gx >> 1));
+#pragma unroll
+      for (int v = 0; v < 4; ++v) uv[v] = zv[v] + (float)kProlong * cv[v / 2];
+    }
+    const int o = ri * W + 4 * g;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      sR[o + v] = rv[v];
+      sU[o + v] = uv[v];
+    }
+    vec_t<unsigned, 4>* kd = (vec_t<unsigned, 4>*)(sK + (ri * NG + g) * 16);
+    kd[0] = k0;
+    kd[1] = k1;
+  }
+  // coefficient c of the cell at row ri, column xi
+  auto kf = [&](int ri, int xi, int c) {
+    return __uint_as_float((unsigned)sK[(ri * NG + (xi >> 2)) * 16 + 4 * c + (xi & 3)] << 16);
+  };
+  // mass term of the cell (global; absent from the periodic pressure system)
+  auto mass = [&](int ri, int xi) {
+    const int y = wrap(y0 - 2 + ri % B, ny), z = wrap(z0 - 2 + ri / B, nz);
+    return mm[(z * ny + y) * nx + wrap(x0 - 4 + xi, nx)];
+  };
+  // sum_f k_f (w_c - w_nb) + m w_c at (ri, xi), w staged like sR: the
+  // operation order of the z-marching sweeps
+  auto lap = [&](const float* w, int ri, int xi) {
+    const int j = ri * W + xi;
+    const float wc = w[j];
+    float l7 = kf(ri, xi, 2) * (wc - w[j + B * W]) + kf(ri - B, xi, 2) * (wc - w[j - B * W]);
+    l7 += kf(ri - 1, xi, 1) * (wc - w[j - W]);
+    l7 += kf(ri, xi, 1) * (wc - w[j + W]);
+    l7 += kf(ri, xi, 0) * (wc - w[j + 1]) + kf(ri, xi - 1, 0) * (wc - w[j - 1]);
+    if (mm) l7 += mass(ri, xi) * wc;
+    return l7;
+  };
+  __syncthreads();
+  // second sweep (weight B) on the box rows 1..10, columns 3..12
+  constexpr int BV = kTile + 2;
+  for (int c = threadIdx.x; c < BV * BV * BV; c += kTileBlock) {
+    const int xi = 3 + c % BV, ri = (1 + (c / BV) % BV) + (1 + c / (BV * BV)) * B;
+    const int j = ri * W + xi;
+    sV[j] = sU[j] + (float)kOmegaB * kf(ri, xi, 3) * (sR[j] - lap(sU, ri, xi));
+  }
+  __syncthreads();
+  gmp<float> zout = (gmp<float>)(PRE ? Lv.z[0] : Lv.z[1]);
+  for (int c = threadIdx.x; c < NT; c += kTileBlock) {
+    const int a = c % kTile, bb = (c / kTile) % kTile, cz = c / (kTile * kTile);
+    const int xi = 4 + a, ri = (2 + bb) + (2 + cz) * B;
+    const int j = ri * W + xi;
+    const bool in = x0 + a < nx && y0 + bb < ny && z0 + cz < nz;
+    const int g = ((z0 + cz) * ny + y0 + bb) * nx + x0 + a;
+    if constexpr (PRE) {
+      sres[c] = sR[j] - lap(sV, ri, xi);
+      if (in) zout[g] = sV[j];
+    } else {
+      float out = sV[j] + (float)kOmegaA * kf(ri, xi, 3) * (sR[j] - lap(sV, ri, xi));
+      if (addzs && in) out += ((gcp<float>)Lv.zs)[g];
+      if (in) zout[g] = out;
+    }
+  }
+  if constexpr (PRE) {
+    __syncthreads();
+    // 64 coarse cells: x pairs, then plane pairs, then row pairs
+    constexpr int H = kTile / 2;
+    if (threadIdx.x < H * H * H) {
+      const int a = threadIdx.x % H, bb = (threadIdx.x / H) % H, cz = threadIdx.x / (H * H);
+      auto xp = [&](int dy, int dz) {
+        const int i = ((2 * cz + dz) * kTile + 2 * bb + dy) * kTile + 2 * a;
+        return sres[i] + sres[i + 1];
+      };
+      const float v = (xp(0, 0) + xp(0, 1)) + (xp(1, 0) + xp(1, 1));
+      const int cx = x0 / 2 + a, cy = y0 / 2 + bb, ccz = z0 / 2 + cz;
+      if (2 * cx < nx && 2 * cy < ny && 2 * ccz < nz)
+        ((gmp<float>)Lc.r)[(ccz * cny + cy) * cnx + cx] = v;
+    }
+  }
+}
+
 // Residual of level l restricted to level l+1: r_{l+1}[C] = sum over the
 // 2x2x2 children c of (r - L z)_c.  One thread per coarse cell.
 template <typename T, bool L0>
@@ -2520,6 +2672,13 @@ static void launch_cycle(const LaunchSet& L, hipStream_t st, int l, int wl, bool
     mark(l == 0 ? 1 : 2);
     launch_sweep<T, 3, false>(L, st, l, -1, -1, wl);
   }
+  if (L.post_tile[l]) {
+    mark(2);
+    if constexpr (sizeof(T) == 4) {
+      hipLaunchKernelGGL(k_mg_tile<true>, dim3(L.nlt[l]), dim3(kTileBlock), 0, st, L.sds,
+                         L.rec_lt[l], L.b2r_lt[l], l, wl, L.start_only, 0);
+    }
+  }
   if (l == 0 && L.below0) {
     // event profiling: the coarser levels as one replay (one event)
     mark(2);
@@ -2536,6 +2695,13 @@ static void launch_cycle(const LaunchSet& L, hipStream_t st, int l, int wl, bool
     mark(l == 0 ? 5 : 2);
     if (l == 0) launch_post_l<T, true, true>(L, st, l, wl, 0);
     else launch_post_l<T, false, false>(L, st, l, wl, second ? 1 : 0);
+  }
+  if (L.post_tile[l]) {
+    mark(2);
+    if constexpr (sizeof(T) == 4) {
+      hipLaunchKernelGGL(k_mg_tile<false>, dim3(L.nlt[l]), dim3(kTileBlock), 0, st, L.sds,
+                         L.rec_lt[l], L.b2r_lt[l], l, wl, L.start_only, second ? 1 : 0);
+    }
   }
   if (L.post_plain[l]) {
     mark(l == 0 ? 5 : 2);
@@ -2599,7 +2765,8 @@ hipError_t capture_below0(int fp32, const LaunchSet& L, hipStream_t st) {
 static int cycle_launches(const LaunchSet& L, int l) {
   int n = L.bottom_at[l] ? 1 : 0;
   if (l >= L.deep) return n;
-  n += (L.post_fused[l] ? 2 : 0) + (L.post_plain[l] ? 4 : 0) + cycle_launches(L, l + 1);
+  n += (L.post_fused[l] ? 2 : 0) + (L.post_plain[l] ? 4 : 0) + (L.post_tile[l] ? 2 : 0) +
+       cycle_launches(L, l + 1);
   if (l < L.mg_w) n += 1 + cycle_launches(L, l + 1);
   return n;
 }
