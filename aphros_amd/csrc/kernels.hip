@@ -1448,6 +1448,223 @@ __global__ __launch_bounds__(kPostBlock) void k_mg_post(const SysDev* __restrict
   }
 }
 
+// bfloat16 pair of a 32-bit word: element 0 in the low half
+__device__ __forceinline__ float bflo(unsigned w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float bfhi(unsigned w) { return __uint_as_float(w & 0xffff0000u); }
+__device__ __forceinline__ vec_t<float, 4> bf4(unsigned a, unsigned b) {
+  vec_t<float, 4> r;
+  r[0] = bflo(a);
+  r[1] = bfhi(a);
+  r[2] = bflo(b);
+  r[3] = bfhi(b);
+  return r;
+}
+
+// k_mg_post for FP32 engines with 4 cells per thread, software-pipelined:
+// every global load of step k is for step k + 1 (u on plane k + 2, the
+// packed coefficients, r, the y-minus row's ky and the x-minus cell's kx on
+// plane k + 1, the halo row's u), so its latency overlaps the barrier and
+// the compute of a step instead of preceding it (the barrier waits for LDS
+// only).  The coefficients stay packed (bfloat16 pairs) in registers; the
+// second sweep reuses the first sweep's ky and kx of the neighbouring cells
+// instead of loading them again.  Same arithmetic as k_mg_post.
+template <bool DOT, bool L0>
+__global__ __launch_bounds__(kPostBlock) void k_mg_post4(const SysDev* __restrict__ sds,
+                                                         const Rec* __restrict__ recs,
+                                                         const int* __restrict__ b2r,
+                                                         int l_arg, int wl, int so, int addzs) {
+  using V = vec_t<float, 4>;
+  using U4 = vec_t<unsigned, 4>;
+  using U2 = vec_t<unsigned, 2>;
+  using V2 = vec_t<float, 2>;
+  constexpr int VX = 4;
+  const int l = L0 ? 0 : l_arg;
+  __shared__ float su[2][kPostBlock * VX];
+  __shared__ float sv[2][kPostBlock * VX];
+  // plane k-1's packed coefficients of each thread's cells, kept for the
+  // second sweep (each thread reads and writes only its own slot)
+  __shared__ U4 sq[2][kPostBlock];
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const Rec rc = cload(gtab(recs + cload(gtab(b2r + b))));
+  const SysDev s = cload(gtab(sds + rc.sys));
+  if (s.sc->done) return;
+  if (wl >= 0 && wl >= s.mg_w) return;
+  if (so && !s.sc->first) return;
+  const MgLevel Lv = cload(gtab(s.lev + l));
+  const MgLevel Lc = cload(gtab(s.lev + l + 1));
+  const int nx = Lv.nx, ny = Lv.ny, nz = Lv.nz;
+  const int lb = b - rc.blk0;
+  const int lx = rc.lx, ty = rc.ty;
+  const int tiy = lb % rc.tiles_y, tiz = lb / rc.tiles_y;
+  const int ix = threadIdx.x % lx, iy = threadIdx.x / lx;
+  const bool active = iy < ty;
+  const int y0 = tiy * (ty - 2);
+  const int yr = y0 - 1 + iy;
+  const int y = yr < 0 ? yr + ny : (yr >= ny ? yr - ny : yr);
+  const bool outrow = active && iy >= 1 && iy <= ty - 2 && yr < ny;
+  const int za = tiz * rc.bz, zb = min(za + rc.bz, nz);
+  const int x0 = ix * VX;
+  const int lofs = iy * nx + x0;
+  const int lxm = iy * nx + (ix == 0 ? nx - 1 : x0 - 1);
+  const int lxp = iy * nx + (ix == lx - 1 ? 0 : x0 + VX);
+  const int xm = x0 == 0 ? nx - 1 : x0 - 1;
+  const int ym = wrapm(y, ny), yp = wrapp(y, ny);
+  const bool far = iy == 0 || iy + 1 == ty;
+  const int yfar = iy == 0 ? ym : yp;
+  // per-thread offsets within a plane (32-bit: a plane has < 2^31 cells);
+  // plane bases are uniform over the workgroup
+  const int oc = y * nx + x0;                     // the thread's cells
+  const int oym = ym * nx + x0;                   // the y-minus row
+  const int oxm = y * nx + xm;                    // the x-minus cell
+  const int ofar = yfar * nx + x0;                // the halo row (edge thread rows)
+  const int cnx = Lc.nx, cny = Lc.ny;
+  const int occ = (y >> 1) * cnx + (x0 >> 1);     // coarse cells under oc
+  const int ocf = (yfar >> 1) * cnx + (x0 >> 1);  // coarse cells under ofar
+  const long long pl = (long long)nx * ny;
+  const long long cpl = (long long)cnx * cny;
+  const gcp<unsigned short> kc = (gcp<unsigned short>)Lv.kc;
+  const gcp<float> mm = (gcp<float>)Lv.m;
+  const gcp<float> r = (gcp<float>)Lv.r;
+  const gcp<float> zi = (gcp<float>)Lv.z[0];
+  const gmp<float> zo = (gmp<float>)Lv.z[1];
+  const gcp<float> zc = (gcp<float>)Lc.z[l + 1 == s.nlev - 1 ? 0 : 1];
+  const gcp<float> zsa = addzs ? (gcp<float>)Lv.zs : nullptr;
+  const float wa = (float)kOmegaA, wb = (float)kOmegaB;
+  auto zw = [&](int k) { return k < 0 ? k + nz : (k >= nz ? k - nz : k); };
+  // u = z[0] + P z_c at the 4 cells at offset o (coarse offset co) of plane zz
+  auto uv = [&](int zz, int o, int co) -> V {
+    V u = *(gcp<V>)(zi + (long long)zz * pl + o);
+    const V2 c = *(gcp<V2>)(zc + (long long)(zz >> 1) * cpl + co);
+#pragma unroll
+    for (int v = 0; v < VX; ++v) u[v] += (float)kProlong * c[v / 2];
+    return u;
+  };
+  // the packed coefficients of plane kk: group of the cells at offset o
+  // (o % 4 == 0) at kcp + 4 o
+  auto kplane = [&](int kk) { return kc + 4 * (long long)kk * pl; };
+  // the laplacian of k_mg_post with ky of the y-minus row and kx of the
+  // x-minus cell given
+  auto lap = [&](const float* sp, long long base, V w, V wzm, V wzp, V kxc, V kyc, V kzc,
+                 V kzm, V kym, float kxm, V wfar) -> V {
+    V l7 = kzc * (w - wzp) + kzm * (w - wzm);
+    V wy;
+    if (iy > 0) {
+#pragma unroll
+      for (int v = 0; v < VX; ++v) wy[v] = sp[lofs - nx + v];
+    } else {
+      wy = wfar;
+    }
+    l7 += kym * (w - wy);
+    if (iy + 1 < ty) {
+#pragma unroll
+      for (int v = 0; v < VX; ++v) wy[v] = sp[lofs + nx + v];
+    } else {
+      wy = wfar;
+    }
+    l7 += kyc * (w - wy);
+    const float wxm = sp[lxm], wxp = sp[lxp];
+#pragma unroll
+    for (int v = 0; v < VX; ++v) {
+      const float up = v + 1 < VX ? w[(v + 1) % VX] : wxp;
+      const float uw = v > 0 ? w[(v + VX - 1) % VX] : wxm;
+      const float kw = v > 0 ? kxc[(v + VX - 1) % VX] : kxm;
+      l7[v] += kxc[v] * (w[v] - up) + kw * (w[v] - uw);
+    }
+    if (mm) l7 += *(gcp<V>)(mm + base + oc) * w;
+    return l7;
+  };
+  // prefetched operands of the first sweep on plane k: u on plane k + 1,
+  // the packed coefficients, r, ky of row ym, kx of cell xm and the halo
+  // row's u (first and last thread rows) on plane k
+  V pu = 0.0f, pr = 0.0f, pfar = 0.0f;
+  U4 pk0 = 0u, pk1 = 0u;
+  U2 pky = 0u;
+  unsigned short pkx = 0;
+  auto prefetch = [&](int k) {
+    const int kk = zw(k);
+    const gcp<unsigned short> kp = kplane(kk);
+    pu = uv(zw(k + 1), oc, occ);
+    const gcp<U4> g = (gcp<U4>)(kp + 4 * oc);
+    pk0 = g[0];
+    pk1 = g[1];
+    pr = *(gcp<V>)(r + (long long)kk * pl + oc);
+    pky = *(gcp<U2>)(kp + 4 * oym + 4);  // ky[4]: the second quarter of the group
+    pkx = kp[((oxm >> 2) << 4) + (oxm & 3)];
+    if (far) pfar = uv(kk, ofar, ocf);
+  };
+  // state: u on planes k-1 (um), k (uc); v on planes k-2 (vm), k-1 (vc);
+  // packed kz of planes k-1 (zB) and k-2 (zA); plane k-1's packed
+  // coefficients, r, ky of row ym and kx of cell xm for the second sweep
+  V um = 0.0f, uc = 0.0f, vm = 0.0f, vc = 0.0f, r1 = 0.0f;
+  U2 zA = 0u, zB = 0u, ky1 = 0u;
+  unsigned short kx1 = 0;
+  if (active) {
+    um = uv(zw(za - 2), oc, occ);
+    uc = uv(zw(za - 1), oc, occ);
+    zB = *(gcp<U2>)(kplane(zw(za - 2)) + 4 * oc + 8);  // kz[4]
+#pragma unroll
+    for (int v = 0; v < VX; ++v) su[0][lofs + v] = uc[v];
+    prefetch(za - 1);
+  }
+  __syncthreads();
+  double dot = 0.0;
+  for (int k = za - 1; k <= zb; ++k) {
+    const int st = k - (za - 1);
+    // this step's operands (loaded during the previous step)
+    const V un = pu, rv = pr, wfar = pfar;
+    const U4 c0 = pk0, c1 = pk1;
+    const U2 cy = pky;
+    const unsigned short cx = pkx;
+    if (active && k + 1 <= zb) prefetch(k + 1);
+    V vn = 0.0f;
+    // v on plane k (planes za-1 .. zb)
+    if (active) {
+      const int kk = zw(k);
+      if (k + 1 <= zb) {
+#pragma unroll
+        for (int v = 0; v < VX; ++v) su[(st + 1) & 1][lofs + v] = un[v];
+      }
+      const V lu = lap(su[st & 1], (long long)kk * pl, uc, um, un, bf4(c0.x, c0.y),
+                       bf4(c0.z, c0.w), bf4(c1.x, c1.y), bf4(zB.x, zB.y), bf4(cy.x, cy.y),
+                       bflo(cx), wfar);
+      vn = uc + wb * bf4(c1.z, c1.w) * (rv - lu);
+#pragma unroll
+      for (int v = 0; v < VX; ++v) sv[st & 1][lofs + v] = vn[v];
+      um = uc;
+      uc = un;
+    }
+    // z[1] on plane k - 1 (output planes za .. zb-1; no wrap)
+    if (k - 1 >= za && outrow) {
+      const long long base = (long long)(k - 1) * pl;
+      const U4 q0 = sq[0][threadIdx.x], q1 = sq[1][threadIdx.x];
+      const V lv = lap(sv[(st + 1) & 1], base, vc, vm, vn, bf4(q0.x, q0.y), bf4(q0.z, q0.w),
+                       bf4(q1.x, q1.y), bf4(zA.x, zA.y), bf4(ky1.x, ky1.y), bflo(kx1),
+                       V(0.0f));
+      V out = vc + wa * bf4(q1.z, q1.w) * (r1 - lv);
+      if (zsa) out += *(gcp<V>)(zsa + base + oc);
+      *(gmp<V>)(zo + base + oc) = out;
+      if constexpr (DOT) {
+#pragma unroll
+        for (int v = 0; v < VX; ++v) dot += (double)r1[v] * (double)out[v];
+      }
+    }
+    vm = vc;
+    vc = vn;
+    zA = zB;
+    zB = U2{c1.x, c1.y};
+    sq[0][threadIdx.x] = c0;
+    sq[1][threadIdx.x] = c1;
+    r1 = rv;
+    ky1 = cy;
+    kx1 = cx;
+    __syncthreads();
+  }
+  if constexpr (DOT) {
+    double acc[1] = {dot};
+    block_partial<1, 0u, kPostBlock>(acc, s.partials, rc.nblk, lb);
+  }
+}
+
 // Pre-smoothing and restriction of a level in one z march (the levels with
 // post = 1, on the tiles of k_mg_post: an even number of output rows from an
 // even row, an even number of planes from an even plane):
@@ -1643,6 +1860,232 @@ __global__ __launch_bounds__(kPostBlock) void k_mg_pre(const SysDev* __restrict_
   }
 }
 
+// k_mg_pre for FP32 engines with 4 cells per thread, software-pipelined like
+// k_mg_post4: the loads of step k (r and D^-1 on plane k + 2, the packed
+// coefficients, ky of the y-minus row, kx of the x-minus cell and the halo
+// row's r and D^-1 on plane k + 1) are for step k + 1; plane k-1's packed
+// coefficients wait in LDS for the residual.  Same arithmetic as k_mg_pre.
+template <bool L0>
+__global__ __launch_bounds__(kPostBlock) void k_mg_pre4(const SysDev* __restrict__ sds,
+                                                        const Rec* __restrict__ recs,
+                                                        const int* __restrict__ b2r, int l_arg,
+                                                        int wl, int so) {
+  using V = vec_t<float, 4>;
+  using U4 = vec_t<unsigned, 4>;
+  using U2 = vec_t<unsigned, 2>;
+  constexpr int VX = 4;
+  constexpr int NP = VX / 2;
+  const int l = L0 ? 0 : l_arg;
+  __shared__ float su[2][kPostBlock * VX];
+  __shared__ float sz[2][kPostBlock * VX];
+  __shared__ float sacc[kPostBlock * NP];
+  // each thread's plane k-1 data for the residual (own slots only): packed
+  // coefficients, r, and {ky of row ym, kz of plane k-2}
+  __shared__ U4 sq[2][kPostBlock];
+  __shared__ V sr[kPostBlock];
+  __shared__ U4 sm[kPostBlock];
+  __shared__ unsigned skx[kPostBlock];               // kx of cell xm, plane k-1
+  __shared__ vec_t<float, NP> spend[kPostBlock];     // even rows' pending sums
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const Rec rc = cload(gtab(recs + cload(gtab(b2r + b))));
+  const SysDev s = cload(gtab(sds + rc.sys));
+  if (s.sc->done) return;
+  if (wl >= 0 && wl >= s.mg_w) return;
+  if (so && !s.sc->first) return;
+  const MgLevel Lv = cload(gtab(s.lev + l));
+  const MgLevel Lc = cload(gtab(s.lev + l + 1));
+  const int nx = Lv.nx, ny = Lv.ny, nz = Lv.nz;
+  const int lb = b - rc.blk0;
+  const int lx = rc.lx, ty = rc.ty;
+  const int tiy = lb % rc.tiles_y, tiz = lb / rc.tiles_y;
+  const int ix = threadIdx.x % lx, iy = threadIdx.x / lx;
+  const bool active = iy < ty;
+  const int y0 = tiy * (ty - 2);
+  const int yr = y0 - 1 + iy;
+  const int y = yr < 0 ? yr + ny : (yr >= ny ? yr - ny : yr);
+  const bool outrow = active && iy >= 1 && iy <= ty - 2 && yr < ny;
+  const int za = tiz * rc.bz, zb = min(za + rc.bz, nz);
+  const int x0 = ix * VX;
+  const int lofs = iy * nx + x0;
+  const int lxm = iy * nx + (ix == 0 ? nx - 1 : x0 - 1);
+  const int lxp = iy * nx + (ix == lx - 1 ? 0 : x0 + VX);
+  const int xm = x0 == 0 ? nx - 1 : x0 - 1;
+  const int ym = wrapm(y, ny), yp = wrapp(y, ny);
+  const bool far = iy == 0 || iy + 1 == ty;
+  const int yfar = iy == 0 ? ym : yp;
+  const int oc = y * nx + x0;
+  const int oym = ym * nx + x0;
+  const int oxm = y * nx + xm;
+  const int ofar = yfar * nx + x0;
+  const long long pl = (long long)nx * ny;
+  const gcp<unsigned short> kc = (gcp<unsigned short>)Lv.kc;
+  const gcp<float> mm = (gcp<float>)Lv.m;
+  const gcp<float> r = (gcp<float>)Lv.r;
+  const gmp<float> zo = (gmp<float>)Lv.z[0];
+  const gmp<float> rco = (gmp<float>)Lc.r;
+  const int cnx = Lc.nx, cny = Lc.ny;
+  const float wa = (float)kOmegaA, wb = (float)kOmegaB;
+  auto zw = [&](int k) { return k < 0 ? k + nz : (k >= nz ? k - nz : k); };
+  auto kplane = [&](int kk) { return kc + 4 * (long long)kk * pl; };
+  // u = wA D^-1 r from r and the packed D^-1 pair of 4 cells
+  auto u_of = [&](V rr, U2 d) -> V { return wa * bf4(d.x, d.y) * rr; };
+  auto lap = [&](const float* sp, long long base, V w, V wzm, V wzp, V kxc, V kyc, V kzc,
+                 V kzm, V kym, float kxm, V wfar) -> V {
+    V l7 = kzc * (w - wzp) + kzm * (w - wzm);
+    V wy;
+    if (iy > 0) {
+#pragma unroll
+      for (int v = 0; v < VX; ++v) wy[v] = sp[lofs - nx + v];
+    } else {
+      wy = wfar;
+    }
+    l7 += kym * (w - wy);
+    if (iy + 1 < ty) {
+#pragma unroll
+      for (int v = 0; v < VX; ++v) wy[v] = sp[lofs + nx + v];
+    } else {
+      wy = wfar;
+    }
+    l7 += kyc * (w - wy);
+    const float wxm = sp[lxm], wxp = sp[lxp];
+#pragma unroll
+    for (int v = 0; v < VX; ++v) {
+      const float up = v + 1 < VX ? w[(v + 1) % VX] : wxp;
+      const float uw = v > 0 ? w[(v + VX - 1) % VX] : wxm;
+      const float kw = v > 0 ? kxc[(v + VX - 1) % VX] : kxm;
+      l7[v] += kxc[v] * (w[v] - up) + kw * (w[v] - uw);
+    }
+    if (mm) l7 += *(gcp<V>)(mm + base + oc) * w;
+    return l7;
+  };
+  // prefetched operands of the z sweep on plane k: r and D^-1 on plane k + 1
+  // (u there, and r of the next step), the packed coefficients, ky of row
+  // ym, kx of cell xm and the halo row's r and D^-1 on plane k
+  V pr = 0.0f, pfr = 0.0f;
+  U2 pd = 0u, pfd = 0u, pky = 0u;
+  U4 pk0 = 0u, pk1 = 0u;
+  unsigned short pkx = 0;
+  auto prefetch = [&](int k) {
+    const int kk = zw(k), kn = zw(k + 1);
+    const gcp<unsigned short> kp = kplane(kk);
+    pr = *(gcp<V>)(r + (long long)kn * pl + oc);
+    pd = *(gcp<U2>)(kplane(kn) + 4 * oc + 12);  // dinv[4]: the last quarter
+    const gcp<U4> g = (gcp<U4>)(kp + 4 * oc);
+    pk0 = g[0];
+    pk1 = g[1];
+    pky = *(gcp<U2>)(kp + 4 * oym + 4);
+    pkx = kp[((oxm >> 2) << 4) + (oxm & 3)];
+    if (far) {
+      pfr = *(gcp<V>)(r + (long long)kk * pl + ofar);
+      pfd = *(gcp<U2>)(kp + 4 * ofar + 12);
+    }
+  };
+  // state: u on planes k-1 (um), k (uc), r on plane k (rcur); z on planes
+  // k-2 (zm), k-1 (zc0); packed kz of planes k-1 (zB), k-2 (zA); r, ky of
+  // row ym and kx of cell xm on plane k-1 (r1, ky1, kx1; its packed
+  // coefficients in sq); the restriction sums
+  V um = 0.0f, uc = 0.0f, rcur = 0.0f, zm = 0.0f, zc0 = 0.0f;
+  U2 zB = 0u;
+  float racc[NP];
+#pragma unroll
+  for (int j = 0; j < NP; ++j) racc[j] = 0.0f;
+  int kpend = -1;  // coarse plane of rpend (even rows), -1: none
+  if (active) {
+    const int k2 = zw(za - 2), k1 = zw(za - 1);
+    um = u_of(*(gcp<V>)(r + (long long)k2 * pl + oc), *(gcp<U2>)(kplane(k2) + 4 * oc + 12));
+    rcur = *(gcp<V>)(r + (long long)k1 * pl + oc);
+    uc = u_of(rcur, *(gcp<U2>)(kplane(k1) + 4 * oc + 12));
+    zB = *(gcp<U2>)(kplane(k2) + 4 * oc + 8);  // kz[4]
+#pragma unroll
+    for (int v = 0; v < VX; ++v) su[0][lofs + v] = uc[v];
+    prefetch(za - 1);
+  }
+  __syncthreads();
+  for (int k = za - 1; k <= zb; ++k) {
+    const int st = k - (za - 1);
+    // row pairs of the plane pair completed in the previous step
+    if (kpend >= 0) {
+      const vec_t<float, NP> rp = spend[threadIdx.x];
+#pragma unroll
+      for (int j = 0; j < NP; ++j)
+        rco[((long long)kpend * cny + (y >> 1)) * cnx + (x0 >> 1) + j] =
+            rp[j] + sacc[(threadIdx.x + lx) * NP + j];
+      kpend = -1;
+    }
+    const V rn = pr, fr = pfr;
+    const U2 dn = pd, fd = pfd, cy = pky;
+    const U4 c0 = pk0, c1 = pk1;
+    const unsigned short cx = pkx;
+    if (active && k + 1 <= zb) prefetch(k + 1);
+    V zn = 0.0f, rk = 0.0f;
+    // z on plane k (planes za-1 .. zb)
+    if (active) {
+      const int kk = zw(k);
+      const V un = u_of(rn, dn);
+      if (k + 1 <= zb) {
+#pragma unroll
+        for (int v = 0; v < VX; ++v) su[(st + 1) & 1][lofs + v] = un[v];
+      }
+      const long long base = (long long)kk * pl;
+      const V lu = lap(su[st & 1], base, uc, um, un, bf4(c0.x, c0.y), bf4(c0.z, c0.w),
+                       bf4(c1.x, c1.y), bf4(zB.x, zB.y), bf4(cy.x, cy.y), bflo(cx),
+                       u_of(fr, fd));
+      zn = uc + wb * bf4(c1.z, c1.w) * (rcur - lu);
+#pragma unroll
+      for (int v = 0; v < VX; ++v) sz[st & 1][lofs + v] = zn[v];
+      if (outrow && k >= za && k < zb) *(gmp<V>)(zo + base + oc) = zn;
+      rk = rcur;
+      um = uc;
+      uc = un;
+      rcur = rn;
+    }
+    // residual on plane k - 1 (planes za .. zb-1, no wrap)
+    if (k - 1 >= za && outrow) {
+      const int kp = k - 1;
+      const long long base = (long long)kp * pl;
+      const U4 q0 = sq[0][threadIdx.x], q1 = sq[1][threadIdx.x], qm = sm[threadIdx.x];
+      const V lz = lap(sz[(st + 1) & 1], base, zc0, zm, zn, bf4(q0.x, q0.y), bf4(q0.z, q0.w),
+                       bf4(q1.x, q1.y), bf4(qm.z, qm.w), bf4(qm.x, qm.y), bflo(skx[threadIdx.x]),
+                       V(0.0f));
+      const V res = sr[threadIdx.x] - lz;
+      if ((kp & 1) == 0) {
+#pragma unroll
+        for (int j = 0; j < NP; ++j) racc[j] = res[2 * j] + res[2 * j + 1];
+      } else {
+#pragma unroll
+        for (int j = 0; j < NP; ++j) racc[j] += res[2 * j] + res[2 * j + 1];
+        if (y & 1) {
+#pragma unroll
+          for (int j = 0; j < NP; ++j) sacc[threadIdx.x * NP + j] = racc[j];
+        } else {
+          vec_t<float, NP> rp;
+#pragma unroll
+          for (int j = 0; j < NP; ++j) rp[j] = racc[j];
+          spend[threadIdx.x] = rp;
+          kpend = kp >> 1;
+        }
+      }
+    }
+    zm = zc0;
+    zc0 = zn;
+    sm[threadIdx.x] = U4{cy.x, cy.y, zB.x, zB.y};
+    zB = U2{c1.x, c1.y};
+    sq[0][threadIdx.x] = c0;
+    sq[1][threadIdx.x] = c1;
+    sr[threadIdx.x] = rk;
+    skx[threadIdx.x] = cx;
+    __syncthreads();
+  }
+  // the last plane pair (planes zb-2, zb-1)
+  if (kpend >= 0) {
+    const vec_t<float, NP> rp = spend[threadIdx.x];
+#pragma unroll
+    for (int j = 0; j < NP; ++j)
+      rco[((long long)kpend * cny + (y >> 1)) * cnx + (x0 >> 1) + j] =
+          rp[j] + sacc[(threadIdx.x + lx) * NP + j];
+  }
+}
+
 // A level of at most 64 cells held by the lanes of one wavefront (cell i on
 // lane i): neighbour values by lane shuffles, no LDS and no barriers.  Lanes
 // past the level's cells carry zero coefficients and read themselves.
@@ -1702,7 +2145,7 @@ template <typename T>
 __global__ __launch_bounds__(kBottomThreads) void k_mg_bottom(const SysDev* __restrict__ sds,
                                                               int sweeps, int l0, int wl,
                                                               int so) {
-  extern __shared__ unsigned char smem[];
+  extern __shared__ __align__(16) unsigned char smem[];
   __shared__ MgLevel lv[kMaxLevels];  // the levels, with LDS pointers if resident
   const SysDev& s = sds[blockIdx.x];
   if (s.nlev <= 0 || s.sc->done || s.mg_lb != l0) return;
@@ -1729,14 +2172,27 @@ __global__ __launch_bounds__(kBottomThreads) void k_mg_bottom(const SysDev* __re
   if (tid == 0) {
     for (int l = l0; l <= top; ++l) lv[l] = s.lev[l];
   }
+  // the copies into LDS as one table of segments (16-byte units; a tail of
+  // 4-byte words per segment), so that every thread issues all of its loads
+  // before it waits for any
+  constexpr int kSeg = 6 * kMaxLevels;
+  __shared__ const uint4* cs_src[kSeg];
+  __shared__ uint4* cs_dst[kSeg];
+  __shared__ int cs_beg[kSeg + 1];
+  __shared__ int cs_tail[kSeg];
+  int nseg = 0;
   if (full) {
-    // copy `bytes` (a multiple of 4) from global memory to LDS
+    // record a copy of `bytes` (a multiple of 4) from global memory to LDS
     auto copy = [&](void* dst, const void* src, long long bytes) {
-      const long long n16 = bytes >> 4;
-      for (long long i = tid; i < n16; i += nt) ((uint4*)dst)[i] = ((const uint4*)src)[i];
-      for (long long i = n16 * 4 + tid; i < (bytes >> 2); i += nt)
-        ((unsigned*)dst)[i] = ((const unsigned*)src)[i];
+      if (tid == 0) {
+        cs_src[nseg] = (const uint4*)src;
+        cs_dst[nseg] = (uint4*)dst;
+        cs_beg[nseg + 1] = cs_beg[nseg] + (int)(bytes >> 4);
+        cs_tail[nseg] = (int)((bytes & 15) >> 2);
+      }
+      ++nseg;
     };
+    if (tid == 0) cs_beg[0] = 0;
     for (int l = l0; l <= top; ++l) {
       const MgLevel& G = s.lev[l];
       const long long n = G.n;
@@ -1767,6 +2223,35 @@ __global__ __launch_bounds__(kBottomThreads) void k_mg_bottom(const SysDev* __re
       if (tid == 0) lv[l].r = p;
       p += lds_al16(n * es);
     }
+    __syncthreads();
+    const int total = cs_beg[nseg];
+    using W4 = vec_t<unsigned, 4>;
+    for (int b0 = tid; b0 < total; b0 += 4 * nt) {
+      W4 v0 = 0u, v1 = 0u, v2 = 0u, v3 = 0u;
+      int s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+      auto load = [&](int u, int& sg, W4& v) {
+        if (u < total) {
+          while (cs_beg[sg + 1] <= u) ++sg;
+          v = *(glb_cp<W4>)((const W4*)cs_src[sg] + (u - cs_beg[sg]));
+        }
+      };
+      auto store = [&](int u, int sg, W4 v) {
+        if (u < total) *(lds_p<W4>)((W4*)cs_dst[sg] + (u - cs_beg[sg])) = v;
+      };
+      load(b0, s0, v0);
+      load(b0 + nt, s1, v1);
+      load(b0 + 2 * nt, s2, v2);
+      load(b0 + 3 * nt, s3, v3);
+      store(b0, s0, v0);
+      store(b0 + nt, s1, v1);
+      store(b0 + 2 * nt, s2, v2);
+      store(b0 + 3 * nt, s3, v3);
+    }
+    for (int k = 0; k < nseg; ++k) {
+      const int n16 = cs_beg[k + 1] - cs_beg[k];
+      if (tid < cs_tail[k])
+        ((unsigned*)(cs_dst[k] + n16))[tid] = ((const unsigned*)(cs_src[k] + n16))[tid];
+    }
   }
   __syncthreads();
   // levels lw..top go to one wavefront (a level below one of <= 64 cells has
@@ -1792,9 +2277,83 @@ __global__ __launch_bounds__(kBottomThreads) void k_mg_bottom(const SysDev* __re
     });
     __syncthreads();
   };
+  // FP32 levels resident in LDS with nx % 4 == 0: groups of 4 cells per
+  // thread (the packed coefficients of a group in two 16-byte reads, the
+  // neighbouring rows and planes in 16-byte reads), the operation order of
+  // LdsView::lap per cell
+  using V4 = vec_t<float, 4>;
+  using U4 = vec_t<unsigned, 4>;
+  using U2 = vec_t<unsigned, 2>;
+  auto vec_ok = [&](int l) { return sizeof(T) == 4 && full && lv[l].nx % 4 == 0; };
+  // f(g, i, x0, y, z) for the groups g = i / 4 of level l
+  auto for_groups = [&](int l, auto f) {
+    const int nx = lv[l].nx, ny = lv[l].ny, ng = (int)lv[l].n / 4;
+    for (int g = tid; g < ng; g += nt) {
+      const int i = 4 * g;
+      const int x0 = i % nx, t = i / nx;
+      f(g, i, x0, t % ny, t / ny);
+    }
+  };
+  // (L u) of the 4 cells from i = 4 g (x0, y, z) of level l, u in LDS
+  auto lap4 = [&](int l, const float* u_, int g, int i, int x0, int y, int z) -> V4 {
+    const lds_cp<float> u = (lds_cp<float>)u_;
+    const lds_cp<unsigned short> kc = (lds_cp<unsigned short>)(const unsigned short*)lv[l].kc;
+    const int nx = lv[l].nx, ny = lv[l].ny, nz = lv[l].nz, pl = nx * ny, n = (int)lv[l].n;
+    const U4 c0 = *(lds_cp<U4>)(kc + 16 * g), c1 = *(lds_cp<U4>)(kc + 16 * g + 8);
+    const int ixm = x0 == 0 ? i + nx - 1 : i - 1, ixp = x0 + 4 == nx ? i - nx + 4 : i + 4;
+    const int iym = y == 0 ? i + pl - nx : i - nx, iyp = y == ny - 1 ? i - pl + nx : i + nx;
+    const int izm = z == 0 ? i + n - pl : i - pl, izp = z == nz - 1 ? i - n + pl : i + pl;
+    const V4 w = *(lds_cp<V4>)(u + i);
+    const V4 wym = *(lds_cp<V4>)(u + iym), wyp = *(lds_cp<V4>)(u + iyp);
+    const V4 wzm = *(lds_cp<V4>)(u + izm), wzp = *(lds_cp<V4>)(u + izp);
+    const float wxm = u[ixm], wxp = u[ixp];
+    const U2 ym2 = *(lds_cp<U2>)(kc + 16 * (iym >> 2) + 4);
+    const U2 zm2 = *(lds_cp<U2>)(kc + 16 * (izm >> 2) + 8);
+    const float kxm = bflo(kc[16 * (ixm >> 2) + (ixm & 3)]);
+    const V4 kx = bf4(c0.x, c0.y), ky = bf4(c0.z, c0.w), kz = bf4(c1.x, c1.y);
+    const V4 kym = bf4(ym2.x, ym2.y), kzm = bf4(zm2.x, zm2.y);
+    V4 s4;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const float up = v < 3 ? w[v + 1] : wxp, uw = v > 0 ? w[v - 1] : wxm;
+      const float kw = v > 0 ? kx[v - 1] : kxm;
+      s4[v] = kx[v] * (w[v] - up) + kw * (w[v] - uw) + ky[v] * (w[v] - wyp[v]) +
+              kym[v] * (w[v] - wym[v]) + kz[v] * (w[v] - wzp[v]) + kzm[v] * (w[v] - wzm[v]);
+    }
+    if (lv[l].m) s4 += *(lds_cp<V4>)((const float*)lv[l].m + i) * w;
+    return s4;
+  };
+  auto dinv4 = [&](int l, int g) {
+    const U2 d = *(lds_cp<U2>)((lds_cp<unsigned short>)(const unsigned short*)lv[l].kc +
+                               16 * g + 12);
+    return bf4(d.x, d.y);
+  };
+  // out = u + om D^-1 (r - L u) on level l by groups, then a barrier
+  auto sweep4 = [&](int l, const T* u, T* out, T om) {
+    for_groups(l, [&](int g, int i, int x0, int y, int z) {
+      const V4 uc = *(lds_cp<V4>)((const float*)u + i);
+      const V4 rr = *(lds_cp<V4>)((const float*)lv[l].r + i);
+      *(lds_p<V4>)((float*)out + i) =
+          uc + (float)om * dinv4(l, g) * (rr - lap4(l, (const float*)u, g, i, x0, y, z));
+    });
+    __syncthreads();
+  };
   for (int l = l0; l < lw && l < top; ++l) {
     const lds_p<T> zal = (lds_p<T>)za[l];
     const lds_p<T> zbl = (lds_p<T>)zb[l];
+    if (vec_ok(l)) {
+      for_groups(l, [&](int g, int i, int, int, int) {
+        *(lds_p<V4>)((float*)zb[l] + i) =
+            (float)wa * dinv4(l, g) * *(lds_cp<V4>)((const float*)lv[l].r + i);
+      });
+      __syncthreads();
+      sweep4(l, zb[l], za[l], wb);
+      for_groups(l, [&](int g, int i, int x0, int y, int z) {
+        *(lds_p<V4>)((float*)zb[l] + i) = *(lds_cp<V4>)((const float*)lv[l].r + i) -
+                                          lap4(l, (const float*)za[l], g, i, x0, y, z);
+      });
+      __syncthreads();
+    } else {
     with_view(l, [&](const auto& V) {
       for (int i = tid; i < V.n; i += nt) zbl[i] = wa * V.dinv(i) * V.r[i];
     });
@@ -1809,6 +2368,7 @@ __global__ __launch_bounds__(kBottomThreads) void k_mg_bottom(const SysDev* __re
       });
     });
     __syncthreads();
+    }
     const MgLevel& Lf = lv[l];
     with_view(l + 1, [&](const auto& C) {
       const int fnx = Lf.nx, fpl = Lf.nx * Lf.ny;
@@ -1900,8 +2460,13 @@ __global__ __launch_bounds__(kBottomThreads) void k_mg_bottom(const SysDev* __re
       u[i] = zf[i] + (T)kProlong * zc[((z >> 1) * Lc.ny + (y >> 1)) * Lc.nx + (x >> 1)];
     });
     __syncthreads();
-    sweep(l, zb[l], za[l], wb);
-    sweep(l, za[l], zb[l], wa);
+    if (vec_ok(l)) {
+      sweep4(l, zb[l], za[l], wb);
+      sweep4(l, za[l], zb[l], wa);
+    } else {
+      sweep(l, zb[l], za[l], wb);
+      sweep(l, za[l], zb[l], wa);
+    }
     inb |= 1u << l;
   }
   const T* __restrict__ res = ((inb >> l0) & 1u) ? zb[l0] : za[l0];
@@ -2592,8 +3157,8 @@ static void launch_pre_l(const LaunchSet& L, hipStream_t st, int l, int wl) {
   const dim3 g(L.nlp[l]), t(kPostBlock);
   if (L.vx_p[l] == 4) {
     if constexpr (sizeof(T) == 4) {
-      hipLaunchKernelGGL((k_mg_pre<T, 4, L0>), g, t, 0, st, L.sds, L.rec_lp[l], L.b2r_lp[l], l,
-                         wl, L.start_only);
+      hipLaunchKernelGGL((k_mg_pre4<L0>), g, t, 0, st, L.sds, L.rec_lp[l], L.b2r_lp[l], l, wl,
+                         L.start_only);
     }
   } else {
     hipLaunchKernelGGL((k_mg_pre<T, 2, L0>), g, t, 0, st, L.sds, L.rec_lp[l], L.b2r_lp[l], l, wl,
@@ -2610,7 +3175,7 @@ static void launch_post_l(const LaunchSet& L, hipStream_t st, int l, int wl, int
   switch (L.vx_p[l]) {
     case 4:
       if constexpr (sizeof(T) == 4) {
-        hipLaunchKernelGGL((k_mg_post<T, 4, DOT, L0>), g, t, 0, st, L.sds, L.rec_lp[l],
+        hipLaunchKernelGGL((k_mg_post4<DOT, L0>), g, t, 0, st, L.sds, L.rec_lp[l],
                            L.b2r_lp[l], l, wl, L.start_only, addzs);
       }
       break;
