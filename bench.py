@@ -80,7 +80,20 @@ def cpu_model():
 SPMV_JACOBI = {"slot": 0, "vec": 8.0, "coef": 0.0, "m": 1, "name": "k_pcg_spmv<T, VX, true>"}
 
 
-def kernel_table(kt, ncell, elem, coef, has_m, per_iter_launches, jacobi=False):
+def launched_name(name, elem, vx):
+    """The kernel instance the engine launches for a table entry: the
+    template arguments of the precision and cells per thread filled in; FP32
+    engines with 4 cells per thread run the software-pipelined fused level
+    kernels k_mg_pre4 / k_mg_post4."""
+    if elem == 4 and vx == 4:
+        if name == "k_mg_post<T, VX, true, true>":
+            return "k_mg_post4<true, true>"
+        if name == "k_mg_pre<T, VX, true>":
+            return "k_mg_pre4<true>"
+    return name.replace("T, VX", f"{'float' if elem == 4 else 'double'}, {vx}")
+
+
+def kernel_table(kt, ncell, elem, coef, has_m, per_iter_launches, jacobi=False, vx=1):
     """Per-kernel average duration and achieved algorithmic bandwidth."""
     out = {}
     table = dict(KERNELS)
@@ -100,7 +113,8 @@ def kernel_table(kt, ncell, elem, coef, has_m, per_iter_launches, jacobi=False):
             continue
         per_cell = (k["vec"] + (k["m"] if has_m else 0)) * elem + k["coef"] * coef
         b = per_cell * ncell
-        out[key] = {"kernel": k["name"], "avg_ms": round(ms, 5), "launches": cnt,
+        out[key] = {"kernel": launched_name(k["name"], elem, vx), "avg_ms": round(ms, 5),
+                    "launches": cnt,
                     "alg_bytes_per_cell": per_cell, "alg_bytes_per_launch": int(b),
                     "GBps": round(b / (ms * 1e-3) / 1e9, 1),
                     "ms_per_iter": round(ms * per_iter_launches.get(key, 1), 5)}
@@ -119,8 +133,7 @@ def measured_traffic(config, prec, precond, kernel, li, L):
             table = json.load(f)["kernels"]
     except (OSError, ValueError, KeyError):
         return None
-    name = kernel.replace("T, VX", f"float, {li['vx']}")
-    rec = table.get(name)
+    rec = table.get(kernel)
     return None if rec is None else int(rec["fetch_bytes"] + rec["write_bytes"])
 
 
@@ -150,11 +163,12 @@ def run_c2(args):
     info = s.solve_device(xb.ptr)  # kernels launched singly between events
     prof = time.perf_counter() - t
     kt = s.kernel_times()
+    li = s.launch_info()
     s.close()
     xb.free()
     if not (info0["converged"] and info["converged"]):
         raise RuntimeError("c2 solve did not converge")
-    kernels = kernel_table(kt, ncell, 8, 8, False, {}, jacobi=True)
+    kernels = kernel_table(kt, ncell, 8, 8, False, {}, jacobi=True, vx=li["vx"])
     b_it = sum(k["alg_bytes_per_launch"] for k in kernels.values())
     ms_it = sum(k["avg_ms"] for k in kernels.values())
     log(f"[bench] c2: {info0['iter']} iterations, {plain:.3f}s; iteration kernels "
@@ -505,7 +519,7 @@ def main():
         elem = 4 if main_prec == L.MIXED else 8
         coef = 2 if precond == L.PRECOND_MG and main_prec == L.MIXED else elem
         kernels = kernel_table(kt, ncell, elem, coef, has_m, per_iter,
-                               jacobi=precond != L.PRECOND_MG)
+                               jacobi=precond != L.PRECOND_MG, vx=li["vx"])
         if kernels:
             # the dominant kernel: the most device time per iteration
             key = max(kernels, key=lambda k: kernels[k]["ms_per_iter"])
