@@ -99,7 +99,7 @@ bool make_post_rec(int nx, int ny, int nz, int vx, int ncu, Rec* out) {
 }  // namespace
 
 void system_tiling(int method, int nx, int ny, int nz, int vx, const Tuning& tu, Rec* st,
-                   Rec* sm, Rec* ru) {
+                   Rec* sm, Rec* ru, int vx_ru) {
   // The tiling of a system depends on that system only, so its reduction
   // trees (and results) are the same alone and inside a batch.
   // ~one workgroup per 2048 cells, at most 1024 (about one resident wave
@@ -111,7 +111,7 @@ void system_tiling(int method, int nx, int ny, int nz, int vx, const Tuning& tu,
   // lanes; the neighbouring rows' halo is served from L2 either way)
   *st = make_stencil_rec(nx, ny, nz, method == kPcg ? vx : 1, tgt, tu, method == kPcg ? 128 : 64);
   Tuning none;
-  *ru = make_stencil_rec(nx, ny, nz, 1, tgt, none);
+  *ru = make_stencil_rec(nx, ny, nz, vx_ru, tgt, none);
   Rec b{};
   const int cells = tu.sm > 0 ? tu.sm : (method == kPcg ? 4 * vx : 4);
   b.nblk = (int)std::min(16384LL, std::max(1LL, cdiv(n, (long long)kBlock * cells)));
@@ -831,6 +831,9 @@ int Engine::build_tables() {
   } else {
     vx_ = 1;
   }
+  // the PCG FP64 residual: 2 cells per thread when every row has an even
+  // length (the reference modes' residual stays at 1)
+  const int vx_ru = method_ == kPcg && all2 ? 2 : 1;
   const int ns = (int)launch_slots_.size();
   std::vector<Rec> rst, rsm, rru;
   std::vector<int> bst, bsm, bru;
@@ -865,7 +868,7 @@ int Engine::build_tables() {
     if (s.tune_lx > 0) tu.lx = s.tune_lx;
     if (s.tune_bz > 0) tu.bz = s.tune_bz;
     Rec a, b, c;
-    system_tiling(method_, s.nx, s.ny, s.nz, vx_, tu, &a, &b, &c);
+    system_tiling(method_, s.nx, s.ny, s.nz, vx_, tu, &a, &b, &c, vx_ru);
     a.sys = b.sys = c.sys = k;
     a.blk0 = (int)bst.size();
     b.blk0 = (int)bsm.size();
@@ -1028,6 +1031,7 @@ int Engine::build_tables() {
   L_.nst = (int)bst.size();
   L_.nsm = (int)bsm.size();
   L_.nru = (int)bru.size();
+  L_.vx_ru = vx_ru;
   L_.ns = ns;
   L_.pc = pc_ ? 1 : 0;
   L_.nlev_max = nlev_max;

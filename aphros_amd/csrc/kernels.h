@@ -15,6 +15,7 @@ struct LaunchSet {
   const Rec* rec_ru = nullptr;  // one-cell-per-thread stencil launches (FP64 residuals)
   const int* b2r_ru = nullptr;
   int nst = 0, nsm = 0, nru = 0;
+  int vx_ru = 1;  // cells per thread of the PCG FP64 residual launch (rec_ru)
   int ns = 0;  // systems (one workgroup each in the finalize kernels)
   // multigrid preconditioner (pc = 1): per level l < nlev_max - 1, the
   // one-cell-per-thread tiling of level l (rec_lf) and of level l+1 (rec_lc)

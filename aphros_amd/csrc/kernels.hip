@@ -619,6 +619,119 @@ __global__ __launch_bounds__(kBlock) void k_ru_resid(const SysDev* __restrict__ 
   block_partial<3, 4u>(acc, s.partials, rc.nblk, lb);
 }
 
+// k_ru_resid with 2 cells per thread (even nx): 16-byte FP64 loads of x, b
+// and the coefficients; the same per-cell operation order and partials.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_ru_resid2(const SysDev* __restrict__ sds,
+                                                      const Rec* __restrict__ recs,
+                                                      const int* __restrict__ b2r) {
+  using D2 = vec_t<double, 2>;
+  __shared__ double su[2][kBlock * 2];
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const Rec rc = recs[b2r[b]];
+  const SysDev& s = sds[rc.sys];
+  Scalars* sc = s.sc;
+  if (sc->done || !sc->ru) return;
+  const int lb = b - rc.blk0;
+  const Tile t = tile_of<2>(rc, s, lb);
+  const int nx = s.nx, ny = s.ny, nz = s.nz;
+  const double mp = sc->mean_pending;
+  const double sgn = s.neg ? -1.0 : 1.0;
+  const gcp<double> __restrict__ x = (gcp<double>)s.x64;
+  const gcp<double> __restrict__ bb = (gcp<double>)s.b64;
+  const gcp<double> __restrict__ e0 = (gcp<double>)s.s4[0];
+  const gcp<double> __restrict__ ex = (gcp<double>)s.s4[1];
+  const gcp<double> __restrict__ ey = (gcp<double>)s.s4[2];
+  const gcp<double> __restrict__ ez = (gcp<double>)s.s4[3];
+  const gcp<T> __restrict__ S = (gcp<T>)s.S;
+  const gmp<T> __restrict__ r = (gmp<T>)s.r;
+  const int lx = rc.lx;
+  const int ix = threadIdx.x % lx, iy = threadIdx.x / lx;
+  const int ty = kBlock / lx;
+  const int tx = lx * 2;
+  const int lofs = iy * tx + ix * 2;
+  const bool xm_in = ix > 0;
+  const bool xp_in = ix + 1 < lx && t.x0 + 2 < nx;
+  const bool ym_in = iy > 0;
+  const bool yp_in = iy + 1 < ty && t.y + 1 < ny;
+  const long long pl = (long long)nx * ny;
+  const long long row = (long long)t.y * nx;
+  const long long rowm = (long long)wrapm(t.y, ny) * nx;
+  const long long rowp = (long long)wrapp(t.y, ny) * nx;
+  const int xw = wrapm(t.x0, nx), xe = t.x0 + 2 == nx ? 0 : t.x0 + 2;
+  auto ld2 = [&](gcp<double> a, long long i) { return *(gcp<D2>)(a + i); };
+  double acc[3] = {0.0, 0.0, 0.0};
+  D2 um = 0.0, uc = 0.0, ezm = 0.0;
+  if (t.active) {
+    const long long im = (long long)wrapm(t.za, nz) * pl + row + t.x0;
+    um = ld2(x, im) - mp;
+    ezm = ld2(ez, im);
+    uc = ld2(x, (long long)t.za * pl + row + t.x0) - mp;
+    su[0][lofs] = uc[0];
+    su[0][lofs + 1] = uc[1];
+  }
+  __syncthreads();
+  for (int z = t.za; z < t.zb; ++z) {
+    const int buf = (z - t.za) & 1;
+    if (t.active) {
+      const long long base = (long long)z * pl;
+      const long long c = base + row + t.x0;
+      const D2 un = ld2(x, (long long)wrapp(z, nz) * pl + row + t.x0) - mp;
+      if (z + 1 < t.zb) {
+        su[buf ^ 1][lofs] = un[0];
+        su[buf ^ 1][lofs + 1] = un[1];
+      }
+      const double* sp = su[buf];
+      const double uxm = xm_in ? sp[lofs - 1] : x[base + row + xw] - mp;
+      const double uxp = xp_in ? sp[lofs + 2] : x[base + row + xe] - mp;
+      D2 uym, uyp;
+      if (ym_in) {
+        uym[0] = sp[lofs - tx];
+        uym[1] = sp[lofs - tx + 1];
+      } else {
+        uym = ld2(x, base + rowm + t.x0) - mp;
+      }
+      if (yp_in) {
+        uyp[0] = sp[lofs + tx];
+        uyp[1] = sp[lofs + tx + 1];
+      } else {
+        uyp = ld2(x, base + rowp + t.x0) - mp;
+      }
+      const D2 e0c = ld2(e0, c), exc = ld2(ex, c), eyc = ld2(ey, c), ezc = ld2(ez, c);
+      const D2 eym = ld2(ey, base + rowm + t.x0), bc = ld2(bb, c);
+      const double exw = ex[base + row + xw];
+      D2 r64;
+#pragma unroll
+      for (int v = 0; v < 2; ++v) {
+        const double wm = v == 0 ? uxm : uc[0], wp = v == 0 ? uc[1] : uxp;
+        const double kw = v == 0 ? exw : exc[0];
+        double ax = e0c[v] * uc[v];
+        ax += kw * wm;
+        ax += exc[v] * wp;
+        ax += eym[v] * uym[v];
+        ax += eyc[v] * uyp[v];
+        ax += ezm[v] * um[v];
+        ax += ezc[v] * un[v];
+        r64[v] = bc[v] - ax;
+      }
+      vec_t<T, 2> rh;
+#pragma unroll
+      for (int v = 0; v < 2; ++v) {
+        rh[v] = (T)(sgn * (S ? (double)S[c + v] : 1.0) * r64[v]);
+        acc[0] += (double)rh[v] * (double)rh[v];
+        acc[1] += r64[v] * r64[v];
+        acc[2] = fmax(acc[2], fabs(r64[v]));
+      }
+      *(gmp<vec_t<T, 2>>)(r + c) = rh;
+      ezm = ezc;
+      um = uc;
+      uc = un;
+    }
+    __syncthreads();
+  }
+  block_partial<3, 4u>(acc, s.partials, rc.nblk, lb);
+}
+
 __global__ __launch_bounds__(kBlock) void k_fin_ru_resid(const SysDev* __restrict__ sds) {
   const SysDev& s = sds[blockIdx.x];
   Scalars* sc = s.sc;
@@ -3351,7 +3464,10 @@ static void launch_pcg_kernel(const LaunchSet& L, hipStream_t st, int j) {
     case 3: APH_LAUNCH(k_fin_pcg_update, L.ns, st, L.sds); break;
     case 4: APH_LAUNCH((k_ru_accum<T, VX>), L.nsm, st, L.sds, L.rec_sm, L.b2r_sm); break;
     case 5: APH_LAUNCH(k_fin_ru_accum, L.ns, st, L.sds); break;
-    case 6: APH_LAUNCH((k_ru_resid<T, 1>), L.nru, st, L.sds, L.rec_ru, L.b2r_ru); break;
+    case 6:
+      if (L.vx_ru == 2) APH_LAUNCH(k_ru_resid2<T>, L.nru, st, L.sds, L.rec_ru, L.b2r_ru);
+      else APH_LAUNCH((k_ru_resid<T, 1>), L.nru, st, L.sds, L.rec_ru, L.b2r_ru);
+      break;
     default: APH_LAUNCH(k_fin_ru_resid, L.ns, st, L.sds); break;
   }
 }
