@@ -1574,13 +1574,15 @@ __device__ __forceinline__ vec_t<float, 4> bf4(unsigned a, unsigned b) {
 }
 
 // k_mg_post for FP32 engines with 4 cells per thread, software-pipelined:
-// every global load of step k is for step k + 1 (u on plane k + 2, the
-// packed coefficients, r, the y-minus row's ky and the x-minus cell's kx on
-// plane k + 1, the halo row's u), so its latency overlaps the barrier and
-// the compute of a step instead of preceding it (the barrier waits for LDS
-// only).  The coefficients stay packed (bfloat16 pairs) in registers; the
-// second sweep reuses the first sweep's ky and kx of the neighbouring cells
-// instead of loading them again.  Same arithmetic as k_mg_post.
+// the loads that go to HBM are issued one step ahead (u on plane k + 2, the
+// packed coefficients, r and the halo row's u on plane k + 1), so their
+// latency overlaps the barrier and the compute of a step instead of
+// preceding it (the barrier waits for LDS only); the y-minus row's ky and the
+// x-minus cell's kx, which neighbouring threads of the workgroup load as
+// their own, come from L2 within the step.  The coefficients stay packed
+// (bfloat16 pairs) in registers; the second sweep reuses the first sweep's
+// ky and kx of the neighbouring cells instead of loading them again.  Same
+// arithmetic as k_mg_post.
 template <bool DOT, bool L0>
 __global__ __launch_bounds__(kPostBlock) void k_mg_post4(const SysDev* __restrict__ sds,
                                                          const Rec* __restrict__ recs,
@@ -1594,9 +1596,6 @@ __global__ __launch_bounds__(kPostBlock) void k_mg_post4(const SysDev* __restric
   const int l = L0 ? 0 : l_arg;
   __shared__ float su[2][kPostBlock * VX];
   __shared__ float sv[2][kPostBlock * VX];
-  // plane k-1's packed coefficients of each thread's cells, kept for the
-  // second sweep (each thread reads and writes only its own slot)
-  __shared__ U4 sq[2][kPostBlock];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   const Rec rc = cload(gtab(recs + cload(gtab(b2r + b))));
   const SysDev s = cload(gtab(sds + rc.sys));
@@ -1691,8 +1690,6 @@ __global__ __launch_bounds__(kPostBlock) void k_mg_post4(const SysDev* __restric
   // row's u (first and last thread rows) on plane k
   V pu = 0.0f, pr = 0.0f, pfar = 0.0f;
   U4 pk0 = 0u, pk1 = 0u;
-  U2 pky = 0u;
-  unsigned short pkx = 0;
   auto prefetch = [&](int k) {
     const int kk = zw(k);
     const gcp<unsigned short> kp = kplane(kk);
@@ -1701,8 +1698,6 @@ __global__ __launch_bounds__(kPostBlock) void k_mg_post4(const SysDev* __restric
     pk0 = g[0];
     pk1 = g[1];
     pr = *(gcp<V>)(r + (long long)kk * pl + oc);
-    pky = *(gcp<U2>)(kp + 4 * oym + 4);  // ky[4]: the second quarter of the group
-    pkx = kp[((oxm >> 2) << 4) + (oxm & 3)];
     if (far) pfar = uv(kk, ofar, ocf);
   };
   // state: u on planes k-1 (um), k (uc); v on planes k-2 (vm), k-1 (vc);
@@ -1710,6 +1705,7 @@ __global__ __launch_bounds__(kPostBlock) void k_mg_post4(const SysDev* __restric
   // coefficients, r, ky of row ym and kx of cell xm for the second sweep
   V um = 0.0f, uc = 0.0f, vm = 0.0f, vc = 0.0f, r1 = 0.0f;
   U2 zA = 0u, zB = 0u, ky1 = 0u;
+  U4 q0 = 0u, q1 = 0u;
   unsigned short kx1 = 0;
   if (active) {
     um = uv(zw(za - 2), oc, occ);
@@ -1726,8 +1722,13 @@ __global__ __launch_bounds__(kPostBlock) void k_mg_post4(const SysDev* __restric
     // this step's operands (loaded during the previous step)
     const V un = pu, rv = pr, wfar = pfar;
     const U4 c0 = pk0, c1 = pk1;
-    const U2 cy = pky;
-    const unsigned short cx = pkx;
+    U2 cy = 0u;
+    unsigned short cx = 0;
+    if (active) {
+      const gcp<unsigned short> kp = kplane(zw(k));
+      cy = *(gcp<U2>)(kp + 4 * oym + 4);
+      cx = kp[((oxm >> 2) << 4) + (oxm & 3)];
+    }
     if (active && k + 1 <= zb) prefetch(k + 1);
     V vn = 0.0f;
     // v on plane k (planes za-1 .. zb)
@@ -1749,7 +1750,6 @@ __global__ __launch_bounds__(kPostBlock) void k_mg_post4(const SysDev* __restric
     // z[1] on plane k - 1 (output planes za .. zb-1; no wrap)
     if (k - 1 >= za && outrow) {
       const long long base = (long long)(k - 1) * pl;
-      const U4 q0 = sq[0][threadIdx.x], q1 = sq[1][threadIdx.x];
       const V lv = lap(sv[(st + 1) & 1], base, vc, vm, vn, bf4(q0.x, q0.y), bf4(q0.z, q0.w),
                        bf4(q1.x, q1.y), bf4(zA.x, zA.y), bf4(ky1.x, ky1.y), bflo(kx1),
                        V(0.0f));
@@ -1765,8 +1765,8 @@ __global__ __launch_bounds__(kPostBlock) void k_mg_post4(const SysDev* __restric
     vc = vn;
     zA = zB;
     zB = U2{c1.x, c1.y};
-    sq[0][threadIdx.x] = c0;
-    sq[1][threadIdx.x] = c1;
+    q0 = c0;
+    q1 = c1;
     r1 = rv;
     ky1 = cy;
     kx1 = cx;
@@ -1974,10 +1974,10 @@ __global__ __launch_bounds__(kPostBlock) void k_mg_pre(const SysDev* __restrict_
 }
 
 // k_mg_pre for FP32 engines with 4 cells per thread, software-pipelined like
-// k_mg_post4: the loads of step k (r and D^-1 on plane k + 2, the packed
-// coefficients, ky of the y-minus row, kx of the x-minus cell and the halo
-// row's r and D^-1 on plane k + 1) are for step k + 1; plane k-1's packed
-// coefficients wait in LDS for the residual.  Same arithmetic as k_mg_pre.
+// k_mg_post4: the HBM loads of step k (r and D^-1 on plane k + 2, the packed
+// coefficients and the halo row's r and D^-1 on plane k + 1) are for step
+// k + 1; plane k-1's packed coefficients, r, ky and kx wait in LDS slots for
+// the residual.  Same arithmetic as k_mg_pre.
 template <bool L0>
 __global__ __launch_bounds__(kPostBlock) void k_mg_pre4(const SysDev* __restrict__ sds,
                                                         const Rec* __restrict__ recs,
@@ -2075,9 +2075,8 @@ __global__ __launch_bounds__(kPostBlock) void k_mg_pre4(const SysDev* __restrict
   // (u there, and r of the next step), the packed coefficients, ky of row
   // ym, kx of cell xm and the halo row's r and D^-1 on plane k
   V pr = 0.0f, pfr = 0.0f;
-  U2 pd = 0u, pfd = 0u, pky = 0u;
+  U2 pd = 0u, pfd = 0u;
   U4 pk0 = 0u, pk1 = 0u;
-  unsigned short pkx = 0;
   auto prefetch = [&](int k) {
     const int kk = zw(k), kn = zw(k + 1);
     const gcp<unsigned short> kp = kplane(kk);
@@ -2086,8 +2085,6 @@ __global__ __launch_bounds__(kPostBlock) void k_mg_pre4(const SysDev* __restrict
     const gcp<U4> g = (gcp<U4>)(kp + 4 * oc);
     pk0 = g[0];
     pk1 = g[1];
-    pky = *(gcp<U2>)(kp + 4 * oym + 4);
-    pkx = kp[((oxm >> 2) << 4) + (oxm & 3)];
     if (far) {
       pfr = *(gcp<V>)(r + (long long)kk * pl + ofar);
       pfd = *(gcp<U2>)(kp + 4 * ofar + 12);
@@ -2126,9 +2123,15 @@ __global__ __launch_bounds__(kPostBlock) void k_mg_pre4(const SysDev* __restrict
       kpend = -1;
     }
     const V rn = pr, fr = pfr;
-    const U2 dn = pd, fd = pfd, cy = pky;
+    const U2 dn = pd, fd = pfd;
     const U4 c0 = pk0, c1 = pk1;
-    const unsigned short cx = pkx;
+    U2 cy = 0u;
+    unsigned short cx = 0;
+    if (active) {
+      const gcp<unsigned short> kp = kplane(zw(k));
+      cy = *(gcp<U2>)(kp + 4 * oym + 4);  // ky[4] of the y-minus row
+      cx = kp[((oxm >> 2) << 4) + (oxm & 3)];
+    }
     if (active && k + 1 <= zb) prefetch(k + 1);
     V zn = 0.0f, rk = 0.0f;
     // z on plane k (planes za-1 .. zb)
