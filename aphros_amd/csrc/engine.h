@@ -33,10 +33,10 @@ int set_api_error(int code, const std::string& msg);
 
 // Launch tiling of one system: the stencil launch of the iteration kernel
 // (vx cells per thread in x for PCG, 1 otherwise), the streaming launches and
-// the residual launches (vx_ru cells per thread).  sys and blk0 are left to
-// the caller.
+// the residual launches (vx_ru cells per thread); fp64: the engine's
+// vectors are FP64.  sys and blk0 are left to the caller.
 void system_tiling(int method, int nx, int ny, int nz, int vx, const Tuning& tu, Rec* st,
-                   Rec* sm, Rec* ru, int vx_ru = 1);
+                   Rec* sm, Rec* ru, int vx_ru = 1, int fp64 = 1);
 
 struct System {
   int nx = 0, ny = 0, nz = 0;

@@ -99,17 +99,20 @@ bool make_post_rec(int nx, int ny, int nz, int vx, int ncu, Rec* out) {
 }  // namespace
 
 void system_tiling(int method, int nx, int ny, int nz, int vx, const Tuning& tu, Rec* st,
-                   Rec* sm, Rec* ru, int vx_ru) {
+                   Rec* sm, Rec* ru, int vx_ru, int fp64) {
   // The tiling of a system depends on that system only, so its reduction
   // trees (and results) are the same alone and inside a batch.
   // ~one workgroup per 2048 cells, at most 1024 (about one resident wave
   // set of the 256 CUs: long z marches, few re-read halo planes)
   const long long n = (long long)nx * ny * nz;
   const long long tgt = std::min(1024LL, std::max(1LL, cdiv(n, 2048)));
-  // the PCG operator: rows of up to 128 lanes (whole rows at 512^3 FP32 and
-  // 256^3 FP64: 0.722 -> 0.718 ms and 0.249 -> 0.240 ms per launch against 64
-  // lanes; the neighbouring rows' halo is served from L2 either way)
-  *st = make_stencil_rec(nx, ny, nz, method == kPcg ? vx : 1, tgt, tu, method == kPcg ? 128 : 64);
+  // the FP64 PCG operator: rows of up to 128 lanes (whole rows at 256^3:
+  // 0.249 -> 0.240 ms per launch against 64 lanes).  The FP32 operator keeps
+  // 64: whole rows at 512^3 took 0.718 against 0.722 ms but fetched 3.51
+  // instead of 3.15 GB per launch (two output rows per block re-read two halo
+  // rows of z and p)
+  *st = make_stencil_rec(nx, ny, nz, method == kPcg ? vx : 1, tgt, tu,
+                         method == kPcg && fp64 ? 128 : 64);
   Tuning none;
   *ru = make_stencil_rec(nx, ny, nz, vx_ru, tgt, none);
   Rec b{};
@@ -868,7 +871,7 @@ int Engine::build_tables() {
     if (s.tune_lx > 0) tu.lx = s.tune_lx;
     if (s.tune_bz > 0) tu.bz = s.tune_bz;
     Rec a, b, c;
-    system_tiling(method_, s.nx, s.ny, s.nz, vx_, tu, &a, &b, &c, vx_ru);
+    system_tiling(method_, s.nx, s.ny, s.nz, vx_, tu, &a, &b, &c, vx_ru, fp32_ ? 0 : 1);
     a.sys = b.sys = c.sys = k;
     a.blk0 = (int)bst.size();
     b.blk0 = (int)bsm.size();
