@@ -57,6 +57,34 @@ def test_batch_equals_single(precision, precond, w):
 
 
 @pytest.mark.parametrize("precision", [L.MIXED, L.FP64])
+def test_batch_odd_row_length(precision):
+    """An odd row length (33^3) takes the one-cell-per-thread CG kernels and
+    FP64 residual.  Cells per thread are chosen for the whole launch, so a
+    batch that holds a 33^3 system runs its 32^3 neighbour at one cell per
+    thread too: the 33^3 result equals its lone solve bit for bit, the 32^3
+    one meets the same tolerance with another summation order."""
+    e33 = _sys(33, 4, 0.1, 0.2, 1000.0, 31)
+    e32 = _sys(32, 3, 0.1, 0.2, 1000.0, 21)
+    b = Batch(precision=precision, tol=1e-8, maxiter=50000, precond=L.PRECOND_JACOBI)
+    s33, s32 = b.add(e33), b.add(e32)
+    b.solve()
+    for E, sl in ((e33, s33), (e32, s32)):
+        xb, ib = b.get(sl)
+        xs, is_ = _single(E, precision, precond=L.PRECOND_JACOBI)
+        print(f"n={E.shape[0]} batch iter={ib['iter']} alone iter={is_['iter']}", flush=True)
+        assert ib["converged"] == 1 and is_["converged"] == 1
+        assert O.true_relres(E, xb) <= 1e-8 * (1 + 1e-6)
+        assert O.true_relres(E, xs) <= 1e-8 * (1 + 1e-6)
+        if E.shape[0] == 33:
+            assert ib["iter"] == is_["iter"] and np.array_equal(xb, xs)
+        else:
+            assert abs(ib["iter"] - is_["iter"]) <= 2
+            scale = np.abs(xs - xs.mean()).max()
+            assert np.abs((xb - xb.mean()) - (xs - xs.mean())).max() <= 1e-5 * scale
+    b.close()
+
+
+@pytest.mark.parametrize("precision", [L.MIXED, L.FP64])
 def test_batch_fused_and_plain_levels(precision):
     """A 256^3 system (fused pre/post kernels on its level 0) in one batch with
     32^3 and 48^3 systems (separate sweeps): every level launch then runs both
