@@ -64,6 +64,9 @@ struct System {
   double seconds = 0;
 };
 
+// per-kernel timing slots of a multigrid PCG engine
+constexpr int kProfSlots = 17;
+
 class Engine {
  public:
   Engine() = default;
@@ -165,15 +168,16 @@ class Engine {
     prof_reset();
   }
   void prof_reset() {
-    for (int j = 0; j < 16; ++j) {
+    for (int j = 0; j < kProfSlots; ++j) {
       prof_ms_[j] = 0;
       prof_n_[j] = 0;
     }
   }
   // iteration kernels, plus (multigrid) [9] level-0 sweep, [10] level-0
   // restriction, [11] coarser levels per cycle, [12] bottom levels,
-  // [13] [14] [15] level-0 pre-sweep, first and second post-sweep
-  int prof_kernels() const { return pc_ ? 16 : iter_kernels(method_, 0); }
+  // [13] [14] [15] level-0 pre-sweep, first and second post-sweep, [16]
+  // x^ += alpha p (launched apart from the update: xh_split)
+  int prof_kernels() const { return pc_ ? kProfSlots : iter_kernels(method_, 0); }
   double prof_avg_ms(int j) const { return prof_n_[j] ? prof_ms_[j] / prof_n_[j] : 0.0; }
   long long prof_count(int j) const { return prof_n_[j]; }
 
@@ -247,8 +251,8 @@ class Engine {
   std::vector<hipEvent_t> pev_;
   std::vector<hipEvent_t> pev_mg_;  // V-cycle launches (profiling)
   std::vector<int> mg_kinds_;
-  double prof_ms_[16] = {};
-  long long prof_n_[16] = {};
+  double prof_ms_[kProfSlots] = {};
+  long long prof_n_[kProfSlots] = {};
   double last_run_ms_ = 0;
   int last_run_iters_ = 0;
   std::string err_;

@@ -412,6 +412,7 @@ __global__ __launch_bounds__(kBlock) void k_fin_pcg_spmv(const SysDev* __restric
   reduce_partials<1, 0u>(acc, s.partials, s.nblk_st);
   if (threadIdx.x == 0) {
     const double pq = acc[0];
+    sc->xh_fold = 0;
     if (!(pq != 0.0) || !isfinite(pq)) {
       sc->status = 6;
       sc->done = 1;
@@ -436,7 +437,9 @@ __device__ __forceinline__ void stream_range(long long n, int nblk, int lb, long
   end = beg + chunk < n ? beg + chunk : n;
 }
 
-template <typename T, int VX>
+// XH = false (xh_split): r only; k_pcg_xh adds alpha p to x^ later in the
+// iteration.
+template <typename T, int VX, bool XH>
 __global__ __launch_bounds__(kBlock) void k_pcg_update(const SysDev* __restrict__ sds,
                                                        const Rec* __restrict__ recs,
                                                        const int* __restrict__ b2r) {
@@ -456,15 +459,43 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(const SysDev* __restrict_
   long long beg, end;
   stream_range<VX>(s.n, rc.nblk, lb, beg, end);
   for (long long i = beg + (long long)threadIdx.x * VX; i < end; i += (long long)kBlock * VX) {
-    const D pv = ldd<T, VX>(p, i);
-    const D xv = rnd<T, VX>(ldd<T, VX>(xh, i) + alpha * pv);
+    if constexpr (XH) {
+      const D pv = ldd<T, VX>(p, i);
+      const D xv = rnd<T, VX>(ldd<T, VX>(xh, i) + alpha * pv);
+      std_<T, VX>(xh, i, xv);
+    }
     const D rv = rnd<T, VX>(ldd<T, VX>(r, i) - alpha * ldd<T, VX>(q, i));
-    std_<T, VX>(xh, i, xv);
     std_<T, VX>(r, i, rv);
 #pragma unroll
     for (int v = 0; v < VX; ++v) acc[0] += rv[v] * rv[v];
   }
   block_partial<1, 0u>(acc, s.partials, rc.nblk, lb);
+}
+
+// x^ += alpha p of the iteration (xh_split), on the update's streaming
+// tiling, launched after the level-0 pre-smoothing; skipped when the
+// iteration's reliable update already folded alpha p into x64 (xh_fold) or
+// the system is done.
+template <typename T, int VX>
+__global__ __launch_bounds__(kBlock) void k_pcg_xh(const SysDev* __restrict__ sds,
+                                                   const Rec* __restrict__ recs,
+                                                   const int* __restrict__ b2r) {
+  using D = vec_t<double, VX>;
+  const int b = blockIdx.x;
+  const Rec rc = recs[b2r[b]];
+  const SysDev& s = sds[rc.sys];
+  Scalars* sc = s.sc;
+  if (sc->done || sc->xh_fold) return;
+  const int lb = b - rc.blk0;
+  const double alpha = sc->alpha;
+  const T* __restrict__ p = (const T*)s.p[sc->cur];
+  T* __restrict__ xh = (T*)s.xh;
+  long long beg, end;
+  stream_range<VX>(s.n, rc.nblk, lb, beg, end);
+  for (long long i = beg + (long long)threadIdx.x * VX; i < end; i += (long long)kBlock * VX) {
+    const D xv = rnd<T, VX>(ldd<T, VX>(xh, i) + alpha * ldd<T, VX>(p, i));
+    std_<T, VX>(xh, i, xv);
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void k_fin_pcg_update(const SysDev* __restrict__ sds) {
@@ -492,11 +523,13 @@ __global__ __launch_bounds__(kBlock) void k_fin_pcg_update(const SysDev* __restr
 }
 
 // Reliable update, part 1: x64 = (x64 - mean_pending) + S * x^ ; x^ = 0 ;
-// mean of the new x64 (nullspace removal).
+// mean of the new x64 (nullspace removal).  fold (xh_split): x^ lacks this
+// iteration's alpha p (k_pcg_xh runs after the reliable update), so alpha p
+// is added here and k_pcg_xh skips the iteration (xh_fold).
 template <typename T, int VX>
 __global__ __launch_bounds__(kBlock) void k_ru_accum(const SysDev* __restrict__ sds,
                                                      const Rec* __restrict__ recs,
-                                                     const int* __restrict__ b2r) {
+                                                     const int* __restrict__ b2r, int fold) {
   using D = vec_t<double, VX>;
   const int b = blockIdx.x;
   const Rec rc = recs[b2r[b]];
@@ -508,12 +541,16 @@ __global__ __launch_bounds__(kBlock) void k_ru_accum(const SysDev* __restrict__ 
   const T* __restrict__ S = (const T*)s.S;
   T* __restrict__ xh = (T*)s.xh;
   double* __restrict__ x = s.x64;
+  const double fa = fold ? sc->alpha : 0.0;  // 0 at a start
+  const T* __restrict__ p = (const T*)s.p[sc->cur];
   double acc[1] = {0.0};
   long long beg, end;
   stream_range<VX>(s.n, rc.nblk, lb, beg, end);
   for (long long i = beg + (long long)threadIdx.x * VX; i < end; i += (long long)kBlock * VX) {
     const D sv = S ? ldd<T, VX>(S, i) : (D)(1.0);
-    const D xv = (*reinterpret_cast<const D*>(x + i) - mp) + sv * ldd<T, VX>(xh, i);
+    D h = ldd<T, VX>(xh, i);
+    if (fa != 0.0) h += fa * ldd<T, VX>(p, i);
+    const D xv = (*reinterpret_cast<const D*>(x + i) - mp) + sv * h;
     *reinterpret_cast<D*>(x + i) = xv;
     std_<T, VX>(xh, i, (D)(0.0));
 #pragma unroll
@@ -530,6 +567,7 @@ __global__ __launch_bounds__(kBlock) void k_fin_ru_accum(const SysDev* __restric
   reduce_partials<1, 0u>(acc, s.partials, s.nblk_sm);
   if (threadIdx.x == 0) {
     sc->mean_pending = sc->singular ? acc[0] / (double)s.n : 0.0;
+    sc->xh_fold = 1;
   }
 }
 
@@ -3313,6 +3351,19 @@ static void launch_sweep(const LaunchSet& L, hipStream_t st, int l, int zin, int
   else launch_sweep_l<T, MODE, DOT, false>(L, st, l, zin, zout, wl, addzs);
 }
 
+// x^ += alpha p of the iteration (k_pcg_xh) with the CG's cells per thread
+template <typename T>
+static void launch_xh(const LaunchSet& L, hipStream_t st) {
+  if (L.vx_cg == 4) {
+    if constexpr (sizeof(T) == 4)
+      APH_LAUNCH((k_pcg_xh<T, 4>), L.nsm, st, L.sds, L.rec_sm, L.b2r_sm);
+  } else if (L.vx_cg == 2) {
+    APH_LAUNCH((k_pcg_xh<T, 2>), L.nsm, st, L.sds, L.rec_sm, L.b2r_sm);
+  } else {
+    APH_LAUNCH((k_pcg_xh<T, 1>), L.nsm, st, L.sds, L.rec_sm, L.b2r_sm);
+  }
+}
+
 // The cycle from level l down, recursively.  Per level above the coarsest:
 // the zero-guess sweep fused with the second pre-smoothing sweep (-> z[1]),
 // restriction of r - L z[1], the coarse correction (twice on W levels: the
@@ -3329,7 +3380,7 @@ static void launch_sweep(const LaunchSet& L, hipStream_t st, int l, int zin, int
 // `mark(kind)` precedes every launch (profiling): 0 = level-0 pre-smoothing
 // sweep, 1 = level-0 restriction, 2 = coarser-level launch, 3 = bottom
 // levels (k_mg_bottom), 5 and 6 = level-0 first and second post-smoothing
-// sweeps (5 alone: both, fused).
+// sweeps (5 alone: both, fused), 7 = the CG's x^ += alpha p (k_pcg_xh).
 template <typename T, typename Mark>
 static void launch_cycle(const LaunchSet& L, hipStream_t st, int l, int wl, bool second,
                          Mark& mark) {
@@ -3359,6 +3410,11 @@ static void launch_cycle(const LaunchSet& L, hipStream_t st, int l, int wl, bool
       hipLaunchKernelGGL(k_mg_tile<true>, dim3(L.nlt[l]), dim3(kTileBlock), 0, st, L.sds,
                          L.rec_lt[l], L.b2r_lt[l], l, wl, L.start_only, 0);
     }
+  }
+  if (l == 0 && L.xh_split) {
+    // x^ += alpha p of the iteration, once the pre-smoothing has read r
+    mark(7);
+    launch_xh<T>(L, st);
   }
   if (l == 0 && L.below0) {
     // event profiling: the coarser levels as one replay (one event)
@@ -3446,6 +3502,7 @@ hipError_t capture_below0(int fp32, const LaunchSet& L, hipStream_t st) {
 static int cycle_launches(const LaunchSet& L, int l) {
   int n = L.bottom_at[l] ? 1 : 0;
   if (l >= L.deep) return n;
+  if (l == 0 && L.xh_split) n += 1;
   n += (L.post_fused[l] ? 2 : 0) + (L.post_plain[l] ? 4 : 0) + (L.post_tile[l] ? 2 : 0) +
        cycle_launches(L, l + 1);
   if (l < L.mg_w) n += 1 + cycle_launches(L, l + 1);
@@ -3463,9 +3520,14 @@ static void launch_pcg_kernel(const LaunchSet& L, hipStream_t st, int j) {
       break;
     case 8: launch_vcycle<T>(L, st); break;
     case 1: APH_LAUNCH(k_fin_pcg_spmv, L.ns, st, L.sds); break;
-    case 2: APH_LAUNCH((k_pcg_update<T, VX>), L.nsm, st, L.sds, L.rec_sm, L.b2r_sm); break;
+    case 2:
+      if (L.xh_split) APH_LAUNCH((k_pcg_update<T, VX, false>), L.nsm, st, L.sds, L.rec_sm, L.b2r_sm);
+      else APH_LAUNCH((k_pcg_update<T, VX, true>), L.nsm, st, L.sds, L.rec_sm, L.b2r_sm);
+      break;
     case 3: APH_LAUNCH(k_fin_pcg_update, L.ns, st, L.sds); break;
-    case 4: APH_LAUNCH((k_ru_accum<T, VX>), L.nsm, st, L.sds, L.rec_sm, L.b2r_sm); break;
+    case 4:
+      APH_LAUNCH((k_ru_accum<T, VX>), L.nsm, st, L.sds, L.rec_sm, L.b2r_sm, L.xh_split);
+      break;
     case 5: APH_LAUNCH(k_fin_ru_accum, L.ns, st, L.sds); break;
     case 6:
       if (L.vx_ru == 2) APH_LAUNCH(k_ru_resid2<T>, L.nru, st, L.sds, L.rec_ru, L.b2r_ru);
@@ -3519,6 +3581,7 @@ hipError_t launch_start(int method, int fp32, int vx, const LaunchSet& L, hipStr
     LaunchSet Ls = L;
     Ls.start_only = 1;
     Ls.below0 = nullptr;
+    Ls.xh_split = 0;  // no iteration yet: x^ stays zero
     for (int j = 4; j < iter_kernels(method, L.pc); ++j) {
       const hipError_t e = launch_kernel(method, fp32, vx, Ls, st, j);
       if (e != hipSuccess) return e;

@@ -34,8 +34,12 @@ KERNELS = {
     # p = z + beta p fused: read z, p_in, kx, ky, kz (the CG's FP32 operator);
     # write p_out, q
     "spmv": {"slot": 0, "vec": 7.0, "coef": 0.0, "m": 1, "name": "k_pcg_spmv<T, VX, false>"},
-    # read p, xh, r, q; write xh, r
-    "update": {"slot": 2, "vec": 6.0, "coef": 0.0, "m": 0, "name": "k_pcg_update<T, VX>"},
+    # read p, xh, r, q; write xh, r (the split form: r, q -> r alone, and
+    # "xh" below)
+    "update": {"slot": 2, "vec": 6.0, "coef": 0.0, "m": 0, "name": "k_pcg_update<T, VX, true>"},
+    # x^ += alpha p in a launch of its own (multigrid PCG launches whose
+    # vectors exceed 256 MiB): read p, xh; write xh
+    "xh": {"slot": 16, "vec": 3.0, "coef": 0.0, "m": 0, "name": "k_pcg_xh<T, VX>"},
     # u = w D^-1 r fused: read r, kx, ky, kz, dinv; write z
     "mg_presweep_l0": {"slot": 13, "vec": 2.0, "coef": 4.0, "m": 1,
                        "name": "k_mg_sweep<T, VX, 1, false, true>"},
@@ -99,6 +103,8 @@ def kernel_table(kt, ncell, elem, coef, has_m, per_iter_launches, jacobi=False, 
     table = dict(KERNELS)
     if jacobi:
         table = {"spmv": SPMV_JACOBI, "update": KERNELS["update"]}
+    if len(kt) > 16 and kt[16][1] > 0:
+        table["update"] = dict(KERNELS["update"], vec=3.0, name="k_pcg_update<T, VX, false>")
     fused = len(kt) > 15 and kt[15][1] == 0
     fused_pre = len(kt) > 10 and kt[10][1] == 0
     skip = {"mg_postsweep1_l0" if fused else "mg_post_l0"}
