@@ -65,7 +65,7 @@ struct System {
 };
 
 // per-kernel timing slots of a multigrid PCG engine
-constexpr int kProfSlots = 17;
+constexpr int kProfSlots = 16;
 
 class Engine {
  public:
@@ -175,8 +175,7 @@ class Engine {
   }
   // iteration kernels, plus (multigrid) [9] level-0 sweep, [10] level-0
   // restriction, [11] coarser levels per cycle, [12] bottom levels,
-  // [13] [14] [15] level-0 pre-sweep, first and second post-sweep, [16]
-  // x^ += alpha p (launched apart from the update: xh_split)
+  // [13] [14] [15] level-0 pre-sweep, first and second post-sweep
   int prof_kernels() const { return pc_ ? kProfSlots : iter_kernels(method_, 0); }
   double prof_avg_ms(int j) const { return prof_n_[j] ? prof_ms_[j] / prof_n_[j] : 0.0; }
   long long prof_count(int j) const { return prof_n_[j]; }

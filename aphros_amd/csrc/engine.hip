@@ -60,12 +60,6 @@ constexpr long long kFuseMinCells = 1LL << 24;
 // halos (4.5x the level's bytes) miss L2 and the tiles took 37 + 38 us
 // against 22 + 22 us, so 128^3 keeps the sweeps.
 constexpr long long kTileMaxCells = 1LL << 18;
-// Multigrid PCG launches whose CG vectors (cells x element size, summed over
-// the systems) exceed this split the update (xh_split).  512^3 FP32 (537 MB a
-// vector): 0.1455 -> 0.1437 s per solve; 256^3 FP32 (67 MB, the vectors stay
-// in the Infinity Cache either way): 0.0221 -> 0.0223 s, so smaller launches
-// keep the fused update.
-constexpr long long kXhSplitBytes = 256LL << 20;
 
 bool make_post_rec(int nx, int ny, int nz, int vx, int ncu, Rec* out) {
   const long long n = (long long)nx * ny * nz;
@@ -1041,16 +1035,6 @@ int Engine::build_tables() {
   L_.nsm = (int)bsm.size();
   L_.nru = (int)bru.size();
   L_.vx_ru = vx_ru;
-  {
-    // multigrid PCG whose vectors exceed the 256 MB Infinity Cache: the
-    // update writes r alone, so more of r is still cached when the level-0
-    // pre-smoothing reads it, and x^ += alpha p follows that pre-smoothing
-    // (the same bytes in two launches); smaller launches keep the fused update
-    long long ncells = 0;
-    for (int i : launch_slots_) ncells += slots_[i]->n;
-    L_.xh_split = pc_ && ncells * (fp32_ ? 4 : 8) > kXhSplitBytes ? 1 : 0;
-    L_.vx_cg = vx_;
-  }
   L_.ns = ns;
   L_.pc = pc_ ? 1 : 0;
   L_.nlev_max = nlev_max;
@@ -1236,10 +1220,7 @@ int Engine::precond_apply(int slot, const double* r, double* z) {
     e = launch_cast(d, 8, const_cast<void*>(s.lev[0].r), es, n, st_);
     if (e == hipSuccess) e = hipStreamSynchronize(st_);
     if (e == hipSuccess) e = hipMemcpy(d_sc_ + slot, &sc, sizeof(Scalars), hipMemcpyHostToDevice);
-    // the cycle alone: no x^ update
-    LaunchSet Lq = L_;
-    Lq.xh_split = 0;
-    if (e == hipSuccess) e = launch_kernel(method_, fp32_, vx_, Lq, st_, 8);
+    if (e == hipSuccess) e = launch_kernel(method_, fp32_, vx_, L_, st_, 8);
     // z of level 0 back through the staging buffer (free again after the upload)
     if (e == hipSuccess) e = launch_cast(s.lev[0].z[1], es, staging_, 8, n, st_);
     if (e == hipSuccess) e = hipMemcpyAsync(z, staging_, n * 8, hipMemcpyDeviceToHost, st_);
@@ -1376,7 +1357,7 @@ int Engine::run() {
           // cycle launches by kind: [9] level-0 sweep (per launch, all
           // three), [10] level-0 restriction, [11] coarser levels (per
           // cycle), [12] bottom-levels kernel, [13] [14] [15] level-0
-          // pre-sweep, first and second post-sweep, [16] x^ += alpha p
+          // pre-sweep, first and second post-sweep
           const int nv = vcycle_launches(L_) + 1;  // event stride per cycle
           double coarse = 0;
           for (int i = 0; i < prof_nl_; ++i) {
@@ -1392,8 +1373,6 @@ int Engine::run() {
               slot1 = 10;
             } else if (kd == 3) {
               slot1 = 12;
-            } else if (kd == 7) {
-              slot1 = 16;
             } else if (kd == 2) {
               coarse += ms;
             }

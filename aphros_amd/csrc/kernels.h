@@ -16,10 +16,6 @@ struct LaunchSet {
   const int* b2r_ru = nullptr;
   int nst = 0, nsm = 0, nru = 0;
   int vx_ru = 1;  // cells per thread of the PCG FP64 residual launch (rec_ru)
-  // multigrid PCG with xh_split: the update kernel does r -= alpha q alone
-  // and k_pcg_xh (vx_cg cells per thread, the streaming tiling) x^ += alpha
-  // p after the level-0 pre-smoothing
-  int xh_split = 0, vx_cg = 1;
   int ns = 0;  // systems (one workgroup each in the finalize kernels)
   // multigrid preconditioner (pc = 1): per level l < nlev_max - 1, the
   // one-cell-per-thread tiling of level l (rec_lf) and of level l+1 (rec_lc)

@@ -257,7 +257,11 @@ __device__ __forceinline__ Tile tile_of(const Rec& rc, const SysDev& s, int lb) 
 //
 // SC = false (multigrid mode): S is 1 and the direction is built from the
 // preconditioned residual z: p = z + beta p_in, q = L p.
-template <typename T, int VX, bool SC>
+// XH (multigrid mode): x^ += alpha p_in of the previous iteration, on the
+// tile's own cells where p_in is read (planes za .. zb-1), instead of a pass
+// of the update kernel; skipped when that iteration's reliable update folded
+// alpha p into x64 (xh_fold).
+template <typename T, int VX, bool SC, bool XH = false>
 __global__ __launch_bounds__(kBlock) void k_pcg_spmv(const SysDev* __restrict__ sds,
                                                      const Rec* __restrict__ recs,
                                                      const int* __restrict__ b2r) {
@@ -307,6 +311,16 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv(const SysDev* __restrict__ 
   auto pnew_st = [&](long long i) -> V {
     return __builtin_convertvector(ldd<T, VX>(r, i) + beta * ldd<T, VX>(pin, i), V);
   };
+  const double xa = XH && !sc->xh_fold ? sc->alpha : 0.0;
+  const gmp<T> __restrict__ xh = (gmp<T>)s.xh;
+  // pnew_st, and x^ += xa p_in on cells the tile owns
+  auto pnew_own = [&](long long i) -> V {
+    const vec_t<double, VX> po = ldd<T, VX>(pin, i);
+    if constexpr (XH) {
+      if (xa != 0.0) std_<T, VX>(xh, i, ldd<T, VX>(xh, i) + xa * po);
+    }
+    return __builtin_convertvector(ldd<T, VX>(r, i) + beta * po, V);
+  };
   auto sld = [&](long long i) -> V {
     if constexpr (SC) return ldv<T, VX>(S, i);
     return (T)1;
@@ -329,7 +343,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv(const SysDev* __restrict__ 
     um = unew(i);
     kzm = ldv<T, VX>(kz, i);
     i = (long long)t.za * pl + row + t.x0;
-    pc = pnew_st(i);
+    pc = pnew_own(i);
     Sc = sld(i);
     uc = cva<A, VX>(Sc) * cva<A, VX>(pc);
 #pragma unroll
@@ -343,7 +357,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv(const SysDev* __restrict__ 
       const long long base = (long long)z * pl;
       const long long ic = base + row + t.x0;
       const long long in = (long long)wrapp(z, nz) * pl + row + t.x0;
-      const V pn = pnew_st(in);
+      const V pn = z + 1 < t.zb ? pnew_own(in) : pnew_st(in);
       const V Sn = sld(in);
       const D un = cva<A, VX>(Sn) * cva<A, VX>(pn);
       if (z + 1 < t.zb) {
@@ -437,8 +451,8 @@ __device__ __forceinline__ void stream_range(long long n, int nblk, int lb, long
   end = beg + chunk < n ? beg + chunk : n;
 }
 
-// XH = false (xh_split): r only; k_pcg_xh adds alpha p to x^ later in the
-// iteration.
+// XH = false (multigrid mode): r only; the CG operator of the next iteration
+// adds alpha p to x^ where it reads p (k_pcg_spmv, XH).
 template <typename T, int VX, bool XH>
 __global__ __launch_bounds__(kBlock) void k_pcg_update(const SysDev* __restrict__ sds,
                                                        const Rec* __restrict__ recs,
@@ -472,32 +486,6 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(const SysDev* __restrict_
   block_partial<1, 0u>(acc, s.partials, rc.nblk, lb);
 }
 
-// x^ += alpha p of the iteration (xh_split), on the update's streaming
-// tiling, launched after the level-0 pre-smoothing; skipped when the
-// iteration's reliable update already folded alpha p into x64 (xh_fold) or
-// the system is done.
-template <typename T, int VX>
-__global__ __launch_bounds__(kBlock) void k_pcg_xh(const SysDev* __restrict__ sds,
-                                                   const Rec* __restrict__ recs,
-                                                   const int* __restrict__ b2r) {
-  using D = vec_t<double, VX>;
-  const int b = blockIdx.x;
-  const Rec rc = recs[b2r[b]];
-  const SysDev& s = sds[rc.sys];
-  Scalars* sc = s.sc;
-  if (sc->done || sc->xh_fold) return;
-  const int lb = b - rc.blk0;
-  const double alpha = sc->alpha;
-  const T* __restrict__ p = (const T*)s.p[sc->cur];
-  T* __restrict__ xh = (T*)s.xh;
-  long long beg, end;
-  stream_range<VX>(s.n, rc.nblk, lb, beg, end);
-  for (long long i = beg + (long long)threadIdx.x * VX; i < end; i += (long long)kBlock * VX) {
-    const D xv = rnd<T, VX>(ldd<T, VX>(xh, i) + alpha * ldd<T, VX>(p, i));
-    std_<T, VX>(xh, i, xv);
-  }
-}
-
 __global__ __launch_bounds__(kBlock) void k_fin_pcg_update(const SysDev* __restrict__ sds) {
   const SysDev& s = sds[blockIdx.x];
   Scalars* sc = s.sc;
@@ -523,9 +511,9 @@ __global__ __launch_bounds__(kBlock) void k_fin_pcg_update(const SysDev* __restr
 }
 
 // Reliable update, part 1: x64 = (x64 - mean_pending) + S * x^ ; x^ = 0 ;
-// mean of the new x64 (nullspace removal).  fold (xh_split): x^ lacks this
-// iteration's alpha p (k_pcg_xh runs after the reliable update), so alpha p
-// is added here and k_pcg_xh skips the iteration (xh_fold).
+// mean of the new x64 (nullspace removal).  fold (multigrid mode): x^ lacks
+// this iteration's alpha p (the next iteration's CG operator adds it), so
+// alpha p is added here and the operator skips it (xh_fold).
 template <typename T, int VX>
 __global__ __launch_bounds__(kBlock) void k_ru_accum(const SysDev* __restrict__ sds,
                                                      const Rec* __restrict__ recs,
@@ -3351,19 +3339,6 @@ static void launch_sweep(const LaunchSet& L, hipStream_t st, int l, int zin, int
   else launch_sweep_l<T, MODE, DOT, false>(L, st, l, zin, zout, wl, addzs);
 }
 
-// x^ += alpha p of the iteration (k_pcg_xh) with the CG's cells per thread
-template <typename T>
-static void launch_xh(const LaunchSet& L, hipStream_t st) {
-  if (L.vx_cg == 4) {
-    if constexpr (sizeof(T) == 4)
-      APH_LAUNCH((k_pcg_xh<T, 4>), L.nsm, st, L.sds, L.rec_sm, L.b2r_sm);
-  } else if (L.vx_cg == 2) {
-    APH_LAUNCH((k_pcg_xh<T, 2>), L.nsm, st, L.sds, L.rec_sm, L.b2r_sm);
-  } else {
-    APH_LAUNCH((k_pcg_xh<T, 1>), L.nsm, st, L.sds, L.rec_sm, L.b2r_sm);
-  }
-}
-
 // The cycle from level l down, recursively.  Per level above the coarsest:
 // the zero-guess sweep fused with the second pre-smoothing sweep (-> z[1]),
 // restriction of r - L z[1], the coarse correction (twice on W levels: the
@@ -3380,7 +3355,7 @@ static void launch_xh(const LaunchSet& L, hipStream_t st) {
 // `mark(kind)` precedes every launch (profiling): 0 = level-0 pre-smoothing
 // sweep, 1 = level-0 restriction, 2 = coarser-level launch, 3 = bottom
 // levels (k_mg_bottom), 5 and 6 = level-0 first and second post-smoothing
-// sweeps (5 alone: both, fused), 7 = the CG's x^ += alpha p (k_pcg_xh).
+// sweeps (5 alone: both, fused).
 template <typename T, typename Mark>
 static void launch_cycle(const LaunchSet& L, hipStream_t st, int l, int wl, bool second,
                          Mark& mark) {
@@ -3410,11 +3385,6 @@ static void launch_cycle(const LaunchSet& L, hipStream_t st, int l, int wl, bool
       hipLaunchKernelGGL(k_mg_tile<true>, dim3(L.nlt[l]), dim3(kTileBlock), 0, st, L.sds,
                          L.rec_lt[l], L.b2r_lt[l], l, wl, L.start_only, 0);
     }
-  }
-  if (l == 0 && L.xh_split) {
-    // x^ += alpha p of the iteration, once the pre-smoothing has read r
-    mark(7);
-    launch_xh<T>(L, st);
   }
   if (l == 0 && L.below0) {
     // event profiling: the coarser levels as one replay (one event)
@@ -3502,7 +3472,6 @@ hipError_t capture_below0(int fp32, const LaunchSet& L, hipStream_t st) {
 static int cycle_launches(const LaunchSet& L, int l) {
   int n = L.bottom_at[l] ? 1 : 0;
   if (l >= L.deep) return n;
-  if (l == 0 && L.xh_split) n += 1;
   n += (L.post_fused[l] ? 2 : 0) + (L.post_plain[l] ? 4 : 0) + (L.post_tile[l] ? 2 : 0) +
        cycle_launches(L, l + 1);
   if (l < L.mg_w) n += 1 + cycle_launches(L, l + 1);
@@ -3515,18 +3484,18 @@ template <typename T, int VX>
 static void launch_pcg_kernel(const LaunchSet& L, hipStream_t st, int j) {
   switch (j) {
     case 0:
-      if (L.pc) APH_LAUNCH((k_pcg_spmv<T, VX, false>), L.nst, st, L.sds, L.rec_st, L.b2r_st);
+      if (L.pc) APH_LAUNCH((k_pcg_spmv<T, VX, false, true>), L.nst, st, L.sds, L.rec_st, L.b2r_st);
       else APH_LAUNCH((k_pcg_spmv<T, VX, true>), L.nst, st, L.sds, L.rec_st, L.b2r_st);
       break;
     case 8: launch_vcycle<T>(L, st); break;
     case 1: APH_LAUNCH(k_fin_pcg_spmv, L.ns, st, L.sds); break;
     case 2:
-      if (L.xh_split) APH_LAUNCH((k_pcg_update<T, VX, false>), L.nsm, st, L.sds, L.rec_sm, L.b2r_sm);
+      if (L.pc) APH_LAUNCH((k_pcg_update<T, VX, false>), L.nsm, st, L.sds, L.rec_sm, L.b2r_sm);
       else APH_LAUNCH((k_pcg_update<T, VX, true>), L.nsm, st, L.sds, L.rec_sm, L.b2r_sm);
       break;
     case 3: APH_LAUNCH(k_fin_pcg_update, L.ns, st, L.sds); break;
     case 4:
-      APH_LAUNCH((k_ru_accum<T, VX>), L.nsm, st, L.sds, L.rec_sm, L.b2r_sm, L.xh_split);
+      APH_LAUNCH((k_ru_accum<T, VX>), L.nsm, st, L.sds, L.rec_sm, L.b2r_sm, L.pc);
       break;
     case 5: APH_LAUNCH(k_fin_ru_accum, L.ns, st, L.sds); break;
     case 6:
@@ -3581,7 +3550,6 @@ hipError_t launch_start(int method, int fp32, int vx, const LaunchSet& L, hipStr
     LaunchSet Ls = L;
     Ls.start_only = 1;
     Ls.below0 = nullptr;
-    Ls.xh_split = 0;  // no iteration yet: x^ stays zero
     for (int j = 4; j < iter_kernels(method, L.pc); ++j) {
       const hipError_t e = launch_kernel(method, fp32, vx, Ls, st, j);
       if (e != hipSuccess) return e;

@@ -79,8 +79,8 @@ class Poisson:
         L.check(L.lib().aphpois_set_profiling(self.h, int(on)))
 
     def kernel_times(self):
-        avg = (C.c_double * 17)()
-        cnt = (C.c_longlong * 17)()
+        avg = (C.c_double * 16)()
+        cnt = (C.c_longlong * 16)()
         nk = C.c_int()
         L.check(L.lib().aphpois_kernel_times(self.h, avg, cnt, C.byref(nk)))
         return [(avg[j], cnt[j]) for j in range(nk.value)]

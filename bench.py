@@ -31,15 +31,14 @@ HBM_PEAK_GBS = 8000.0
 # non-singular system.  slot: index in Poisson.kernel_times(); name: the
 # kernel instance in profiles.
 KERNELS = {
-    # p = z + beta p fused: read z, p_in, kx, ky, kz (the CG's FP32 operator);
-    # write p_out, q
-    "spmv": {"slot": 0, "vec": 7.0, "coef": 0.0, "m": 1, "name": "k_pcg_spmv<T, VX, false>"},
-    # read p, xh, r, q; write xh, r (the split form: r, q -> r alone, and
-    # "xh" below)
-    "update": {"slot": 2, "vec": 6.0, "coef": 0.0, "m": 0, "name": "k_pcg_update<T, VX, true>"},
-    # x^ += alpha p in a launch of its own (multigrid PCG launches whose
-    # vectors exceed 256 MiB): read p, xh; write xh
-    "xh": {"slot": 16, "vec": 3.0, "coef": 0.0, "m": 0, "name": "k_pcg_xh<T, VX>"},
+    # p = z + beta p: read z, p_in, kx, ky, kz (the CG's FP32 operator); write
+    # p_out, q; and x^ += alpha p_in of the previous iteration (xh_vec: read and
+    # write x^) in the launches that carry it (not the first iteration's nor
+    # the one after a reliable update, which folds it into x64)
+    "spmv": {"slot": 0, "vec": 7.0, "xh_vec": 2.0, "coef": 0.0, "m": 1,
+             "name": "k_pcg_spmv<T, VX, false, true>"},
+    # r -= alpha q: read r, q; write r
+    "update": {"slot": 2, "vec": 3.0, "coef": 0.0, "m": 0, "name": "k_pcg_update<T, VX, false>"},
     # u = w D^-1 r fused: read r, kx, ky, kz, dinv; write z
     "mg_presweep_l0": {"slot": 13, "vec": 2.0, "coef": 4.0, "m": 1,
                        "name": "k_mg_sweep<T, VX, 1, false, true>"},
@@ -81,7 +80,10 @@ def cpu_model():
 
 # the Jacobi-scaled form of the operator: p = S r + beta p, read r, p_in, S,
 # kx, ky, kz; write p_out, q
-SPMV_JACOBI = {"slot": 0, "vec": 8.0, "coef": 0.0, "m": 1, "name": "k_pcg_spmv<T, VX, true>"}
+SPMV_JACOBI = {"slot": 0, "vec": 8.0, "coef": 0.0, "m": 1,
+               "name": "k_pcg_spmv<T, VX, true, false>"}
+# its update: x^ += alpha p, r -= alpha q; read p, x^, r, q; write x^, r
+UPDATE_JACOBI = {"slot": 2, "vec": 6.0, "coef": 0.0, "m": 0, "name": "k_pcg_update<T, VX, true>"}
 
 
 def launched_name(name, elem, vx):
@@ -97,14 +99,15 @@ def launched_name(name, elem, vx):
     return name.replace("T, VX", f"{'float' if elem == 4 else 'double'}, {vx}")
 
 
-def kernel_table(kt, ncell, elem, coef, has_m, per_iter_launches, jacobi=False, vx=1):
-    """Per-kernel average duration and achieved algorithmic bandwidth."""
+def kernel_table(kt, ncell, elem, coef, has_m, per_iter_launches, jacobi=False, vx=1,
+                 xh_frac=1.0):
+    """Per-kernel average duration and achieved algorithmic bandwidth.
+    xh_frac: the fraction of the CG operator's launches that update x^
+    (iterations minus reliable updates, over iterations)."""
     out = {}
     table = dict(KERNELS)
     if jacobi:
-        table = {"spmv": SPMV_JACOBI, "update": KERNELS["update"]}
-    if len(kt) > 16 and kt[16][1] > 0:
-        table["update"] = dict(KERNELS["update"], vec=3.0, name="k_pcg_update<T, VX, false>")
+        table = {"spmv": SPMV_JACOBI, "update": UPDATE_JACOBI}
     fused = len(kt) > 15 and kt[15][1] == 0
     fused_pre = len(kt) > 10 and kt[10][1] == 0
     skip = {"mg_postsweep1_l0" if fused else "mg_post_l0"}
@@ -117,18 +120,22 @@ def kernel_table(kt, ncell, elem, coef, has_m, per_iter_launches, jacobi=False, 
         ms, cnt = kt[k["slot"]]
         if cnt == 0 or ms <= 0:
             continue
-        per_cell = (k["vec"] + (k["m"] if has_m else 0)) * elem + k["coef"] * coef
+        per_cell = (k["vec"] + k.get("xh_vec", 0.0) * xh_frac + (k["m"] if has_m else 0)) * elem \
+            + k["coef"] * coef
         b = per_cell * ncell
         out[key] = {"kernel": launched_name(k["name"], elem, vx), "avg_ms": round(ms, 5),
                     "launches": cnt,
-                    "alg_bytes_per_cell": per_cell, "alg_bytes_per_launch": int(b),
+                    "alg_bytes_per_cell": round(per_cell, 4), "alg_bytes_per_launch": int(b),
                     "GBps": round(b / (ms * 1e-3) / 1e9, 1),
                     "ms_per_iter": round(ms * per_iter_launches.get(key, 1), 5)}
+        if "xh_vec" in k:
+            out[key]["xh_launch_fraction"] = round(xh_frac, 4)
     return out
 
 
 def measured_traffic(config, prec, precond, kernel, li, L):
-    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE /
+    """HBM bytes per working launch of `kernel` (the launches that did work,
+    as the HIP event averages count them) from the committed FETCH_SIZE /
     WRITE_SIZE counter passes (profiles/traffic_<workload>.json, calibrated
     against a plain copy), when they cover this workload; else None."""
     if prec != L.MIXED or precond != L.PRECOND_MG:
@@ -524,8 +531,10 @@ def main():
                     "mg_post_l0": 1, "mg_pre_l0": 1}
         elem = 4 if main_prec == L.MIXED else 8
         coef = 2 if precond == L.PRECOND_MG and main_prec == L.MIXED else elem
+        nit = sum(i["iter"] for i in infos)
+        xh_frac = (nit - sum(i["refinements"] for i in infos)) / max(1, nit)
         kernels = kernel_table(kt, ncell, elem, coef, has_m, per_iter,
-                               jacobi=precond != L.PRECOND_MG, vx=li["vx"])
+                               jacobi=precond != L.PRECOND_MG, vx=li["vx"], xh_frac=xh_frac)
         if kernels:
             # the dominant kernel: the most device time per iteration
             key = max(kernels, key=lambda k: kernels[k]["ms_per_iter"])

@@ -272,11 +272,9 @@ int aphpois_set_profiling(aphpois* h, int on);
  * update, fin, ru_accum, fin, ru_resid, fin} and, with the multigrid
  * preconditioner, {V-cycle (all its launches together), level-0 sweep (per
  * launch), level-0 restriction, coarser levels (per cycle), bottom levels,
- * level-0 pre-sweep, level-0 first post-sweep, level-0 second post-sweep,
- * x^ += alpha p (its own launch when the CG vectors of the launch exceed
- * 256 MiB; the update then does r -= alpha q alone)};
+ * level-0 pre-sweep, level-0 first post-sweep, level-0 second post-sweep};
  * CONJUGATE {spmv, fin, update, fin}; JACOBI {sweep, fin}.  avg_ms and count
- * hold 17 entries; *nkernels receives the number used. */
+ * hold 16 entries; *nkernels receives the number used. */
 int aphpois_kernel_times(aphpois* h, double* avg_ms, long long* count, int* nkernels);
 /* Launch geometry of the last solve: workgroups of the stencil and streaming
  * launches and the x vector width. */
