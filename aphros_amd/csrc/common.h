@@ -38,8 +38,8 @@ struct Scalars {
   int ru, converged, nru, status;
   int resmode, singular, first, cur;  // cur: index of the current p/u buffer
   int method, pc;  // pc: 1 if a multigrid preconditioner supplies z
-  int xh_fold;     // multigrid mode: the last reliable update folded alpha p
-                   // into x64 (the next CG operator then leaves x^ alone)
+  int xh_fold;     // PCG: the last reliable update folded alpha p into x64
+                   // (the next CG operator then leaves x^ alone)
 };
 
 // One level of the multigrid preconditioner (level 0 = the fine grid).

@@ -257,11 +257,11 @@ __device__ __forceinline__ Tile tile_of(const Rec& rc, const SysDev& s, int lb) 
 //
 // SC = false (multigrid mode): S is 1 and the direction is built from the
 // preconditioned residual z: p = z + beta p_in, q = L p.
-// XH (multigrid mode): x^ += alpha p_in of the previous iteration, on the
-// tile's own cells where p_in is read (planes za .. zb-1), instead of a pass
-// of the update kernel; skipped when that iteration's reliable update folded
-// alpha p into x64 (xh_fold).
-template <typename T, int VX, bool SC, bool XH = false>
+// XH: x^ += alpha p_in of the previous iteration, on the tile's own cells
+// where p_in is read (planes za .. zb-1), instead of a pass of the update
+// kernel; skipped when that iteration's reliable update folded alpha p into
+// x64 (xh_fold).
+template <typename T, int VX, bool SC, bool XH>
 __global__ __launch_bounds__(kBlock) void k_pcg_spmv(const SysDev* __restrict__ sds,
                                                      const Rec* __restrict__ recs,
                                                      const int* __restrict__ b2r) {
@@ -437,8 +437,6 @@ __global__ __launch_bounds__(kBlock) void k_fin_pcg_spmv(const SysDev* __restric
   }
 }
 
-// x^ += alpha p ; r -= alpha q ; (r,r).  Decides whether a reliable update
-// (FP64 true residual) follows.
 // Cells [beg, end) of workgroup lb of a streaming launch of nblk workgroups:
 // contiguous chunks (multiples of kBlock * VX cells), which stream from HBM
 // faster than a grid-stride loop over the same workgroups.
@@ -451,9 +449,10 @@ __device__ __forceinline__ void stream_range(long long n, int nblk, int lb, long
   end = beg + chunk < n ? beg + chunk : n;
 }
 
-// XH = false (multigrid mode): r only; the CG operator of the next iteration
-// adds alpha p to x^ where it reads p (k_pcg_spmv, XH).
-template <typename T, int VX, bool XH>
+// r -= alpha q ; (r,r), which decides whether a reliable update (FP64 true
+// residual) follows.  x^ += alpha p is left to the CG operator of the next
+// iteration, which reads p anyway (k_pcg_spmv, XH).
+template <typename T, int VX>
 __global__ __launch_bounds__(kBlock) void k_pcg_update(const SysDev* __restrict__ sds,
                                                        const Rec* __restrict__ recs,
                                                        const int* __restrict__ b2r) {
@@ -465,19 +464,12 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(const SysDev* __restrict_
   if (sc->done) return;
   const int lb = b - rc.blk0;
   const double alpha = sc->alpha;
-  const T* __restrict__ p = (const T*)s.p[sc->cur];
   const T* __restrict__ q = (const T*)s.q;
   T* __restrict__ r = (T*)s.r;
-  T* __restrict__ xh = (T*)s.xh;
   double acc[1] = {0.0};
   long long beg, end;
   stream_range<VX>(s.n, rc.nblk, lb, beg, end);
   for (long long i = beg + (long long)threadIdx.x * VX; i < end; i += (long long)kBlock * VX) {
-    if constexpr (XH) {
-      const D pv = ldd<T, VX>(p, i);
-      const D xv = rnd<T, VX>(ldd<T, VX>(xh, i) + alpha * pv);
-      std_<T, VX>(xh, i, xv);
-    }
     const D rv = rnd<T, VX>(ldd<T, VX>(r, i) - alpha * ldd<T, VX>(q, i));
     std_<T, VX>(r, i, rv);
 #pragma unroll
@@ -511,13 +503,13 @@ __global__ __launch_bounds__(kBlock) void k_fin_pcg_update(const SysDev* __restr
 }
 
 // Reliable update, part 1: x64 = (x64 - mean_pending) + S * x^ ; x^ = 0 ;
-// mean of the new x64 (nullspace removal).  fold (multigrid mode): x^ lacks
-// this iteration's alpha p (the next iteration's CG operator adds it), so
-// alpha p is added here and the operator skips it (xh_fold).
+// mean of the new x64 (nullspace removal).  x^ lacks this iteration's alpha
+// p (the next iteration's CG operator would add it), so alpha p is added here
+// and the operator skips it (xh_fold).
 template <typename T, int VX>
 __global__ __launch_bounds__(kBlock) void k_ru_accum(const SysDev* __restrict__ sds,
                                                      const Rec* __restrict__ recs,
-                                                     const int* __restrict__ b2r, int fold) {
+                                                     const int* __restrict__ b2r) {
   using D = vec_t<double, VX>;
   const int b = blockIdx.x;
   const Rec rc = recs[b2r[b]];
@@ -529,7 +521,7 @@ __global__ __launch_bounds__(kBlock) void k_ru_accum(const SysDev* __restrict__ 
   const T* __restrict__ S = (const T*)s.S;
   T* __restrict__ xh = (T*)s.xh;
   double* __restrict__ x = s.x64;
-  const double fa = fold ? sc->alpha : 0.0;  // 0 at a start
+  const double fa = sc->alpha;  // 0 at a start
   const T* __restrict__ p = (const T*)s.p[sc->cur];
   double acc[1] = {0.0};
   long long beg, end;
@@ -3485,17 +3477,16 @@ static void launch_pcg_kernel(const LaunchSet& L, hipStream_t st, int j) {
   switch (j) {
     case 0:
       if (L.pc) APH_LAUNCH((k_pcg_spmv<T, VX, false, true>), L.nst, st, L.sds, L.rec_st, L.b2r_st);
-      else APH_LAUNCH((k_pcg_spmv<T, VX, true>), L.nst, st, L.sds, L.rec_st, L.b2r_st);
+      else APH_LAUNCH((k_pcg_spmv<T, VX, true, true>), L.nst, st, L.sds, L.rec_st, L.b2r_st);
       break;
     case 8: launch_vcycle<T>(L, st); break;
     case 1: APH_LAUNCH(k_fin_pcg_spmv, L.ns, st, L.sds); break;
     case 2:
-      if (L.pc) APH_LAUNCH((k_pcg_update<T, VX, false>), L.nsm, st, L.sds, L.rec_sm, L.b2r_sm);
-      else APH_LAUNCH((k_pcg_update<T, VX, true>), L.nsm, st, L.sds, L.rec_sm, L.b2r_sm);
+      APH_LAUNCH((k_pcg_update<T, VX>), L.nsm, st, L.sds, L.rec_sm, L.b2r_sm);
       break;
     case 3: APH_LAUNCH(k_fin_pcg_update, L.ns, st, L.sds); break;
     case 4:
-      APH_LAUNCH((k_ru_accum<T, VX>), L.nsm, st, L.sds, L.rec_sm, L.b2r_sm, L.pc);
+      APH_LAUNCH((k_ru_accum<T, VX>), L.nsm, st, L.sds, L.rec_sm, L.b2r_sm);
       break;
     case 5: APH_LAUNCH(k_fin_ru_accum, L.ns, st, L.sds); break;
     case 6:

@@ -38,7 +38,7 @@ KERNELS = {
     "spmv": {"slot": 0, "vec": 7.0, "xh_vec": 2.0, "coef": 0.0, "m": 1,
              "name": "k_pcg_spmv<T, VX, false, true>"},
     # r -= alpha q: read r, q; write r
-    "update": {"slot": 2, "vec": 3.0, "coef": 0.0, "m": 0, "name": "k_pcg_update<T, VX, false>"},
+    "update": {"slot": 2, "vec": 3.0, "coef": 0.0, "m": 0, "name": "k_pcg_update<T, VX>"},
     # u = w D^-1 r fused: read r, kx, ky, kz, dinv; write z
     "mg_presweep_l0": {"slot": 13, "vec": 2.0, "coef": 4.0, "m": 1,
                        "name": "k_mg_sweep<T, VX, 1, false, true>"},
@@ -79,11 +79,9 @@ def cpu_model():
 
 
 # the Jacobi-scaled form of the operator: p = S r + beta p, read r, p_in, S,
-# kx, ky, kz; write p_out, q
-SPMV_JACOBI = {"slot": 0, "vec": 8.0, "coef": 0.0, "m": 1,
-               "name": "k_pcg_spmv<T, VX, true, false>"}
-# its update: x^ += alpha p, r -= alpha q; read p, x^, r, q; write x^, r
-UPDATE_JACOBI = {"slot": 2, "vec": 6.0, "coef": 0.0, "m": 0, "name": "k_pcg_update<T, VX, true>"}
+# kx, ky, kz; write p_out, q; and x^ as in the multigrid form
+SPMV_JACOBI = {"slot": 0, "vec": 8.0, "xh_vec": 2.0, "coef": 0.0, "m": 1,
+               "name": "k_pcg_spmv<T, VX, true, true>"}
 
 
 def launched_name(name, elem, vx):
@@ -107,7 +105,7 @@ def kernel_table(kt, ncell, elem, coef, has_m, per_iter_launches, jacobi=False, 
     out = {}
     table = dict(KERNELS)
     if jacobi:
-        table = {"spmv": SPMV_JACOBI, "update": UPDATE_JACOBI}
+        table = {"spmv": SPMV_JACOBI, "update": KERNELS["update"]}
     fused = len(kt) > 15 and kt[15][1] == 0
     fused_pre = len(kt) > 10 and kt[10][1] == 0
     skip = {"mg_postsweep1_l0" if fused else "mg_post_l0"}
@@ -181,7 +179,8 @@ def run_c2(args):
     xb.free()
     if not (info0["converged"] and info["converged"]):
         raise RuntimeError("c2 solve did not converge")
-    kernels = kernel_table(kt, ncell, 8, 8, False, {}, jacobi=True, vx=li["vx"])
+    kernels = kernel_table(kt, ncell, 8, 8, False, {}, jacobi=True, vx=li["vx"],
+                           xh_frac=(info["iter"] - info["refinements"]) / max(1, info["iter"]))
     b_it = sum(k["alg_bytes_per_launch"] for k in kernels.values())
     ms_it = sum(k["avg_ms"] for k in kernels.values())
     log(f"[bench] c2: {info0['iter']} iterations, {plain:.3f}s; iteration kernels "
