@@ -40,6 +40,8 @@ struct Scalars {
   int method, pc;  // pc: 1 if a multigrid preconditioner supplies z
   int xh_fold;     // PCG: the last reliable update folded alpha p into x64
                    // (the next CG operator then leaves x^ alone)
+  int zero_start;  // PCG: the start's x64 and x^ are zero-filled (the start's
+                   // residual is b, with no accumulation pass); cleared by it
 };
 
 // One level of the multigrid preconditioner (level 0 = the fine grid).

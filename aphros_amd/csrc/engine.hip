@@ -761,6 +761,7 @@ int Engine::prepare(int slot, const double* x0, bool on_device, long long start,
   System& s = *sp;
   const long long n = s.n;
   double* xdst = method_ == kPcg ? s.dev.x64 : s.dev.u[0];
+  bool zero_x = false;
   if (x0) {
     const double* d = nullptr;
     APH_TRY(upload_strided(x0, on_device, start, ly, lz, s.nx, s.ny, s.nz, 1, &d));
@@ -775,6 +776,7 @@ int Engine::prepare(int slot, const double* x0, bool on_device, long long start,
     }
   } else {
     APH_HIP(launch_fill(xdst, n, 8, st_));
+    zero_x = true;
   }
   if (method_ == kPcg) {
     const int es = fp32_ ? 4 : 8;
@@ -806,6 +808,8 @@ int Engine::prepare(int slot, const double* x0, bool on_device, long long start,
   sc.cur = 0;
   sc.method = method_;
   sc.pc = pc_ ? 1 : 0;
+  // x64 and x^ zero-filled above: the start's residual is b itself
+  sc.zero_start = method_ == kPcg && zero_x ? 1 : 0;
   h_sc_[slot] = sc;
   APH_HIP(hipMemcpyAsync(d_sc_ + slot, h_sc_ + slot, sizeof(Scalars), hipMemcpyHostToDevice,
                          st_));

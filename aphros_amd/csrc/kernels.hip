@@ -515,7 +515,8 @@ __global__ __launch_bounds__(kBlock) void k_ru_accum(const SysDev* __restrict__ 
   const Rec rc = recs[b2r[b]];
   const SysDev& s = sds[rc.sys];
   Scalars* sc = s.sc;
-  if (sc->done || !sc->ru) return;
+  // a zero start: x64 and x^ are zero already, and so is their mean
+  if (sc->done || !sc->ru || sc->zero_start) return;
   const int lb = b - rc.blk0;
   const double mp = sc->mean_pending;
   const T* __restrict__ S = (const T*)s.S;
@@ -543,8 +544,8 @@ __global__ __launch_bounds__(kBlock) void k_fin_ru_accum(const SysDev* __restric
   const SysDev& s = sds[blockIdx.x];
   Scalars* sc = s.sc;
   if (sc->done || !sc->ru) return;
-  double acc[1];
-  reduce_partials<1, 0u>(acc, s.partials, s.nblk_sm);
+  double acc[1] = {0.0};  // a zero start's sum (k_ru_accum wrote no partials)
+  if (!sc->zero_start) reduce_partials<1, 0u>(acc, s.partials, s.nblk_sm);
   if (threadIdx.x == 0) {
     sc->mean_pending = sc->singular ? acc[0] / (double)s.n : 0.0;
     sc->xh_fold = 1;
@@ -592,9 +593,12 @@ __global__ __launch_bounds__(kBlock) void k_ru_resid(const SysDev* __restrict__ 
   const long long rowm = (long long)wrapm(t.y, ny) * nx;
   const long long rowp = (long long)wrapp(t.y, ny) * nx;
   const int xw = wrapm(t.x0, nx), xe = wrapp(t.x0, nx);
+  // a zero start (x64 = 0): A x is exactly zero, so b - A x = b; the same
+  // cells, order and partial sums without loading x or the coefficients
+  const bool zs = sc->zero_start;
   double acc[3] = {0.0, 0.0, 0.0};
   double um = 0.0, uc = 0.0, ezm = 0.0;
-  if (t.active) {
+  if (t.active && !zs) {
     const long long im = (long long)wrapm(t.za, nz) * pl + row + t.x0;
     um = x[im] - mp;
     ezm = ez[im];
@@ -604,7 +608,15 @@ __global__ __launch_bounds__(kBlock) void k_ru_resid(const SysDev* __restrict__ 
   __syncthreads();
   for (int z = t.za; z < t.zb; ++z) {
     const int buf = (z - t.za) & 1;
-    if (t.active) {
+    if (t.active && zs) {
+      const long long c = (long long)z * pl + row + t.x0;
+      const double r64 = bb[c];
+      const T rh = (T)(sgn * (S ? (double)S[c] : 1.0) * r64);
+      r[c] = rh;
+      acc[0] += (double)rh * (double)rh;
+      acc[1] += r64 * r64;
+      acc[2] = fmax(acc[2], fabs(r64));
+    } else if (t.active) {
       const long long base = (long long)z * pl;
       const long long c = base + row + t.x0;
       const double un = x[(long long)wrapp(z, nz) * pl + row + t.x0] - mp;
@@ -632,7 +644,7 @@ __global__ __launch_bounds__(kBlock) void k_ru_resid(const SysDev* __restrict__ 
       um = uc;
       uc = un;
     }
-    __syncthreads();
+    if (!zs) __syncthreads();
   }
   block_partial<3, 4u>(acc, s.partials, rc.nblk, lb);
 }
@@ -678,9 +690,11 @@ __global__ __launch_bounds__(kBlock) void k_ru_resid2(const SysDev* __restrict__
   const long long rowp = (long long)wrapp(t.y, ny) * nx;
   const int xw = wrapm(t.x0, nx), xe = t.x0 + 2 == nx ? 0 : t.x0 + 2;
   auto ld2 = [&](gcp<double> a, long long i) { return *(gcp<D2>)(a + i); };
+  // a zero start (x64 = 0): b - A x = b, as in k_ru_resid
+  const bool zs = sc->zero_start;
   double acc[3] = {0.0, 0.0, 0.0};
   D2 um = 0.0, uc = 0.0, ezm = 0.0;
-  if (t.active) {
+  if (t.active && !zs) {
     const long long im = (long long)wrapm(t.za, nz) * pl + row + t.x0;
     um = ld2(x, im) - mp;
     ezm = ld2(ez, im);
@@ -691,7 +705,19 @@ __global__ __launch_bounds__(kBlock) void k_ru_resid2(const SysDev* __restrict__
   __syncthreads();
   for (int z = t.za; z < t.zb; ++z) {
     const int buf = (z - t.za) & 1;
-    if (t.active) {
+    if (t.active && zs) {
+      const long long c = (long long)z * pl + row + t.x0;
+      const D2 r64 = ld2(bb, c);
+      vec_t<T, 2> rh;
+#pragma unroll
+      for (int v = 0; v < 2; ++v) {
+        rh[v] = (T)(sgn * (S ? (double)S[c + v] : 1.0) * r64[v]);
+        acc[0] += (double)rh[v] * (double)rh[v];
+        acc[1] += r64[v] * r64[v];
+        acc[2] = fmax(acc[2], fabs(r64[v]));
+      }
+      *(gmp<vec_t<T, 2>>)(r + c) = rh;
+    } else if (t.active) {
       const long long base = (long long)z * pl;
       const long long c = base + row + t.x0;
       const D2 un = ld2(x, (long long)wrapp(z, nz) * pl + row + t.x0) - mp;
@@ -745,7 +771,7 @@ __global__ __launch_bounds__(kBlock) void k_ru_resid2(const SysDev* __restrict__
       um = uc;
       uc = un;
     }
-    __syncthreads();
+    if (!zs) __syncthreads();
   }
   block_partial<3, 4u>(acc, s.partials, rc.nblk, lb);
 }
@@ -785,6 +811,7 @@ __global__ __launch_bounds__(kBlock) void k_fin_ru_resid(const SysDev* __restric
       if (!sc->pc) sc->first = 0;
     }
     sc->ru = 0;
+    sc->zero_start = 0;
   }
 }
 
