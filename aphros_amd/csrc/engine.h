@@ -175,7 +175,11 @@ class Engine {
   }
   // iteration kernels, plus (multigrid) [9] level-0 sweep, [10] level-0
   // restriction, [11] coarser levels per cycle, [12] bottom levels,
-  // [13] [14] [15] level-0 pre-sweep, first and second post-sweep
+  // [13] [14] [15] level-0 pre-sweep, first and second post-sweep.  PCG
+  // times the update's finalize and the reliable-update kernels after it
+  // together, in [3] (no events between them: each event between launches
+  // costs the event-timed solve time of its own); [4]..[7] stay empty.
+  bool prof_timed(int j) const { return method_ != kPcg || j <= 3 || j >= 8; }
   int prof_kernels() const { return pc_ ? kProfSlots : iter_kernels(method_, 0); }
   double prof_avg_ms(int j) const { return prof_n_[j] ? prof_ms_[j] / prof_n_[j] : 0.0; }
   long long prof_count(int j) const { return prof_n_[j]; }

@@ -1322,7 +1322,7 @@ int Engine::run() {
       }
       for (int sl = 0; sl < graph_k_; ++sl) {
         for (int j = 0; j < nk; ++j) {
-          APH_HIP(hipEventRecord(pev_[(size_t)sl * nk + j], st_));
+          if (prof_timed(j)) APH_HIP(hipEventRecord(pev_[(size_t)sl * nk + j], st_));
           if (j == 8 && pc_) {
             int nl = 0;
             APH_HIP(launch_vcycle_profiled(fp32_, Lp, st_, &pev_mg_[(size_t)sl * nv],
@@ -1351,9 +1351,12 @@ int Engine::run() {
       const int ran = std::min(graph_k_, iters_after - iters_before);
       for (int sl = 0; sl < ran; ++sl) {
         for (int j = 0; j < nk; ++j) {
+          if (!prof_timed(j)) continue;
+          int j2 = j + 1;  // the next event: the next timed kernel's, or the slot's end
+          while (j2 < nk && !prof_timed(j2)) ++j2;
           float ms = 0;
           APH_HIP(hipEventElapsedTime(&ms, pev_[(size_t)sl * nk + j],
-                                      pev_[(size_t)sl * nk + j + 1]));
+                                      pev_[(size_t)sl * nk + j2]));
           prof_ms_[j] += ms;
           prof_n_[j] += 1;
         }

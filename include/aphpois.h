@@ -269,7 +269,9 @@ int aphpois_set_profiling(aphpois* h, int on);
 /* Average device time (ms) and count of each iteration kernel over the
  * iterations that ran since profiling was switched on.  Kernel order (each
  * reducing kernel is followed by its finalize kernel): PCG {spmv, fin,
- * update, fin, ru_accum, fin, ru_resid, fin} and, with the multigrid
+ * update, fin, ru_accum, fin, ru_resid, fin}, where the update's finalize
+ * and the reliable-update kernels are timed together in the fourth entry
+ * (the fifth to eighth stay empty), and, with the multigrid
  * preconditioner, {V-cycle (all its launches together), level-0 sweep (per
  * launch), level-0 restriction, coarser levels (per cycle), bottom levels,
  * level-0 pre-sweep, level-0 first post-sweep, level-0 second post-sweep};
