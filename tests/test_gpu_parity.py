@@ -97,6 +97,30 @@ def test_pcg_c1_true_residual(c1, precision, precond):
     assert err < 1e-5
 
 
+@pytest.mark.parametrize("precision", [L.FP64, L.MIXED])
+@pytest.mark.parametrize("precond", [L.PRECOND_JACOBI, L.PRECOND_MG])
+def test_pcg_zero_start_equals_explicit_zero_guess(c1, precision, precond):
+    """Without an initial guess the solve starts from zero-filled vectors with
+    the residual b itself (no pass over x or the FP64 coefficients); an
+    explicit all-zero guess is uploaded and takes the full start (x64
+    accumulated, b - A x in FP64).  A x is exactly zero, so both runs have the
+    same iterates: iteration counts, reliable updates, residuals and the
+    solution, bit for bit."""
+    _say(f"pcg zero start precision={precision} precond={precond}")
+    s = Poisson(c1.shape[:3], method=L.PCG, precision=precision, resmode=L.RES_REL_TRUE,
+                tol=1e-8, maxiter=20000, precond=precond)
+    s.set_system(c1)
+    xa, ia = s.solve(None)
+    xb, ib = s.solve(np.zeros(c1.shape[:3]))
+    s.close()
+    _say(f"  iter={ia['iter']} / {ib['iter']} relres={ia['true_relres']:.6e} / "
+         f"{ib['true_relres']:.6e}")
+    for k in ("iter", "refinements", "true_relres", "residual", "converged", "status"):
+        assert ia[k] == ib[k], k
+    assert ia["converged"] == 1
+    assert np.array_equal(xa, xb)
+
+
 @pytest.mark.parametrize("precond", [L.PRECOND_JACOBI, L.PRECOND_MG])
 def test_pcg_tlinear_negative_diagonal(tlin32, precond):
     """e0 < 0 convention (t.linear / SolvePoisson)."""
