@@ -194,6 +194,7 @@ class Engine {
   int build_tables();
   int capture();
   int pull_scalars();
+  int run_pending_ru();
   void fill_info(int slot, double seconds);
   // info.max_diff of a finished slot against its reference solution
   int diff_info(int slot);

@@ -1089,8 +1089,9 @@ int Engine::capture() {
     below_graph_ = nullptr;
   }
   const int nk = iter_kernels(method_, pc_ ? 1 : 0);
+  const int ng = graph_kernels(method_, pc_ ? 1 : 0);  // of them in the graph
   // launches per iteration (the multigrid cycle counts as one kernel in nk)
-  const int nl = nk + (pc_ ? vcycle_launches(L_) - 1 : 0);
+  const int nl = ng + (pc_ ? vcycle_launches(L_) - 1 : 0);
   int k = check_every_;
   if (k <= 0) {
     // about 4 ms of work per replay (iterations past convergence cost a
@@ -1113,7 +1114,7 @@ int Engine::capture() {
   }
   APH_HIP(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
   for (int i = 0; i < k; ++i) {
-    for (int j = 0; j < nk; ++j) {
+    for (int j = 0; j < ng; ++j) {
       const hipError_t e = launch_kernel(method_, fp32_, vx_, L_, st_, j);
       if (e != hipSuccess) {
         hipGraph_t g = nullptr;
@@ -1127,6 +1128,21 @@ int Engine::capture() {
   APH_HIP(hipGraphInstantiate(&gexec_, graph_, nullptr, nullptr, 0));
   graph_k_ = k;
   return APHPOIS_OK;
+}
+
+// After a replay of a graph without the reliable-update kernels
+// (graph_kernels): systems whose update triggered one have idled since; the
+// reliable-update kernels of the start (launch_start, which skip the other
+// systems) apply it, and the scalars are read again for its stop test.
+int Engine::run_pending_ru() {
+  if (graph_kernels(method_, pc_ ? 1 : 0) == iter_kernels(method_, pc_ ? 1 : 0)) {
+    return APHPOIS_OK;
+  }
+  bool pending = false;
+  for (int i : launch_slots_) pending = pending || (h_sc_[i].ru && !h_sc_[i].done);
+  if (!pending) return APHPOIS_OK;
+  APH_HIP(launch_start(method_, fp32_, vx_, L_, st_));
+  return pull_scalars();
 }
 
 // Copies the scalars of every system in the launch tables (one contiguous
@@ -1259,6 +1275,7 @@ int Engine::step(int max_replays, std::vector<int>* finished) {
   for (int r = 0; r < max_replays && finished->empty(); ++r) {
     APH_HIP(hipGraphLaunch(gexec_, st_));
     APH_TRY(pull_scalars());
+    APH_TRY(run_pending_ru());
     bool alldone = true;
     for (int i : launch_slots_) {
       System& s = *slots_[i];
@@ -1286,7 +1303,12 @@ int Engine::run() {
   }
   APH_HIP(hipEventRecord(ev0_, st_));
   APH_HIP(launch_start(method_, fp32_, vx_, L_, st_));
-  const int maxloops = (std::max(maxiter_, miniter_) + 1) / std::max(1, graph_k_) + 4;
+  // a replay may end with a system idle until its reliable update (graph
+  // without those kernels); every replay still advances each system by at
+  // least one iteration or ends it
+  const int ng = graph_kernels(method_, pc_ ? 1 : 0);
+  const int maxloops = (std::max(maxiter_, miniter_) + 1) / std::max(1, graph_k_) + 4 +
+                       (ng < iter_kernels(method_, pc_ ? 1 : 0) ? maxiter_ + 1 : 0);
   int loops = 0;
   bool alldone = false;
   int iters_before = 0;
@@ -1339,6 +1361,7 @@ int Engine::run() {
     }
     ++loops;
     APH_TRY(pull_scalars());
+    if (!profile_) APH_TRY(run_pending_ru());
     alldone = true;
     int iters_after = 0;
     for (int i : launch_slots_) {

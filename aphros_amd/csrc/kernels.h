@@ -84,6 +84,11 @@ __host__ __device__ inline long long bottom_full_bytes(long long n, int es, int 
 // the multigrid preconditioner, [8] the V-cycle with its finalize;
 // CONJUGATE: spmv, fin, update, fin; JACOBI: sweep, fin
 int iter_kernels(int method, int pc = 0);
+// kernels of one iteration in the captured graph: PCG without the multigrid
+// preconditioner leaves the reliable-update kernels (4 .. 7) out; a system
+// whose update triggers one then idles (its CG kernels return while ru is
+// set) until the host launches them (launch_start) after the replay
+int graph_kernels(int method, int pc = 0);
 // Coefficients of the V-cycle levels: level 0 from the CG's conductances
 // kx, ky, kz (converted to the levels' coefficient type: bfloat16 in FP32
 // engines, where level 0 has arrays of its own; FP64 engines share them),

@@ -274,7 +274,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv(const SysDev* __restrict__ 
   const Rec rc = recs[b2r[b]];
   const SysDev& s = sds[rc.sys];
   Scalars* sc = s.sc;
-  if (sc->done) return;
+  if (sc->done || sc->ru) return;  // ru: a reliable update is pending
   const int lb = b - rc.blk0;
   const Tile t = tile_of<VX>(rc, s, lb);
   const int nx = s.nx, ny = s.ny, nz = s.nz;
@@ -421,7 +421,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv(const SysDev* __restrict__ 
 __global__ __launch_bounds__(kBlock) void k_fin_pcg_spmv(const SysDev* __restrict__ sds) {
   const SysDev& s = sds[blockIdx.x];
   Scalars* sc = s.sc;
-  if (sc->done) return;
+  if (sc->done || sc->ru) return;  // ru: a reliable update is pending
   double acc[1];
   reduce_partials<1, 0u>(acc, s.partials, s.nblk_st);
   if (threadIdx.x == 0) {
@@ -461,7 +461,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(const SysDev* __restrict_
   const Rec rc = recs[b2r[b]];
   const SysDev& s = sds[rc.sys];
   Scalars* sc = s.sc;
-  if (sc->done) return;
+  if (sc->done || sc->ru) return;  // ru: a reliable update is pending
   const int lb = b - rc.blk0;
   const double alpha = sc->alpha;
   const T* __restrict__ q = (const T*)s.q;
@@ -481,7 +481,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(const SysDev* __restrict_
 __global__ __launch_bounds__(kBlock) void k_fin_pcg_update(const SysDev* __restrict__ sds) {
   const SysDev& s = sds[blockIdx.x];
   Scalars* sc = s.sc;
-  if (sc->done) return;
+  if (sc->done || sc->ru) return;  // ru: a reliable update is pending
   double acc[1];
   reduce_partials<1, 0u>(acc, s.partials, s.nblk_sm);
   if (threadIdx.x == 0) {
@@ -3285,6 +3285,8 @@ __global__ __launch_bounds__(kBlock) void k_fin_diff(const double* __restrict__ 
 int iter_kernels(int method, int pc) {
   return method == kConjugate ? 4 : (method == kJacobi ? 2 : (pc ? 9 : 8));
 }
+
+int graph_kernels(int method, int pc) { return method == kPcg && !pc ? 4 : iter_kernels(method, pc); }
 
 // One V-cycle over every system of the launch (levels of systems with fewer
 // levels drop out of the deeper launches), then (r, z) and beta.
