@@ -98,7 +98,7 @@ def mg_w_auto(n):
     """The W levels APHPOIS_MG_W_AUTO gives a system of n cells (the engine's
     rule, before clamping to the system's depth)."""
     lg = math.log2(n)
-    return 0 if lg < 22.5 else (1 if lg < 24.5 else (2 if lg < 26.5 else 3))
+    return 0 if lg < 22.5 else (1 if lg < 24.5 else (2 if lg < 25.5 else 3))
 
 SIGNATURES = {
     "aphpois_desc_default": (None, [C.POINTER(Desc)]),

@@ -418,12 +418,13 @@ int Engine::add_system(int nx, int ny, int nz, double h3, int* slot) {
       }
       // W levels l < w (their coarse level l + 1 below the coarsest) keep the
       // first of two coarse corrections in zs of level l + 1.  Automatic:
-      // by cell count (the fastest solve measured at 64^3, 128^3, 192^3,
-      // 256^3, 384^3, 512^3 had w = 0, 0, 1, 1, 2 (3 within 1 %), 3)
+      // by cell count (mixed solves to 1e-8 of three sphere geometries at
+      // 128^3 .. 512^3: w = 0 fastest up to ~160^3, 1 or 2 within a few
+      // percent from 192^3 to 320^3, 3 from 384^3)
       int wsel = mg_w_;
       if (wsel < 0) {
         const double lg = std::log2((double)S.n);
-        wsel = lg < 22.5 ? 0 : (lg < 24.5 ? 1 : (lg < 26.5 ? 2 : 3));
+        wsel = lg < 22.5 ? 0 : (lg < 24.5 ? 1 : (lg < 25.5 ? 2 : 3));
       }
       const int w = std::max(0, std::min(wsel, (int)S.lev.size() - 2));
       S.dev.mg_w = w;

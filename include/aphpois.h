@@ -90,7 +90,7 @@ enum {
 #define APHPOIS_MG_W_DEFAULT 3
 /* mg_w chosen per system from its cell count n (the default): a W level's
  * extra coarse cycles cost launch latency, the iterations it saves cost
- * bandwidth, so 0 for n < 2^22.5, 1 below 2^24.5, 2 below 2^26.5, else 3 */
+ * bandwidth, so 0 for n < 2^22.5, 1 below 2^24.5, 2 below 2^25.5, else 3 */
 #define APHPOIS_MG_W_AUTO (-1)
 
 typedef struct aphpois_desc {

@@ -257,12 +257,15 @@ def test_pcg_large_fused_post_true_residual(c2):
     assert info["iter"] <= ito + 2
 
 
-@pytest.mark.parametrize("name,w", [("c1", 0), ("c2", 1)])
+@pytest.mark.parametrize("name,w", [("c1", 0), ("c2", 1), ("s384", 3)])
 def test_mg_w_auto_picks_by_size(request, name, w):
     """The default W levels (APHPOIS_MG_W_AUTO) are the engine's choice by
-    cell count: 0 at 64^3, 1 at 256^3; the default solve equals the solve
-    with that mg_w bit for bit."""
-    e = request.getfixturevalue(name)
+    cell count: 0 at 64^3, 1 at 256^3, 3 at 384^3 (2^25.75 cells); the
+    default solve equals the solve with that mg_w bit for bit."""
+    if name == "s384":
+        e = synth.system(384, 32, 0.08, 0.15, 1000.0, 2)[0].reshape(384, 384, 384, 8)
+    else:
+        e = request.getfixturevalue(name)
     assert L.mg_w_auto(e.shape[0] * e.shape[1] * e.shape[2]) == w
     out = []
     for mw in (L.MG_W_AUTO, w):
