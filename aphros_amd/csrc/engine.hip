@@ -881,7 +881,6 @@ int Engine::build_tables() {
     a.blk0 = (int)bst.size();
     b.blk0 = (int)bsm.size();
     c.blk0 = (int)bru.size();
-    const long long tgt = std::min(1024LL, std::max(1LL, cdiv(s.n, 2048)));
     Tuning none;
     for (int i = 0; i < a.nblk; ++i) bst.push_back((int)rst.size());
     for (int i = 0; i < b.nblk; ++i) bsm.push_back((int)rsm.size());

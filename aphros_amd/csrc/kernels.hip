@@ -834,6 +834,14 @@ constexpr double kOmega = 0.9;
 constexpr double kChebLo = 0.5, kChebHi = 1.8;
 constexpr double kOmegaA = 0.6212648748965077;  // 1 / (c + d cos(pi/4))
 constexpr double kOmegaB = 1.4484764074432;     // 1 / (c - d cos(pi/4))
+constexpr double kChebC = 0.5 * (kChebHi + kChebLo), kChebD = 0.5 * (kChebHi - kChebLo);
+constexpr double kCosQ = 0.70710678118654752;  // cos(pi/4)
+static_assert(kOmegaA * (kChebC + kChebD * kCosQ) - 1.0 < 1e-12 &&
+                  1.0 - kOmegaA * (kChebC + kChebD * kCosQ) < 1e-12,
+              "kOmegaA is the first Chebyshev weight of [kChebLo, kChebHi]");
+static_assert(kOmegaB * (kChebC - kChebD * kCosQ) - 1.0 < 1e-12 &&
+                  1.0 - kOmegaB * (kChebC - kChebD * kCosQ) < 1e-12,
+              "kOmegaB is the second Chebyshev weight of [kChebLo, kChebHi]");
 constexpr double kProlong = 1.1;
 
 template <typename T>
