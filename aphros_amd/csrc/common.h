@@ -97,6 +97,10 @@ struct SysDev {
   unsigned* counter;
   Scalars* sc;
   int neg;  // 1 if the diagonal is negative (A = -L): r^ = -S r64
+  // 1 if every e0 equals -(e_xm + e_xp + e_ym + e_yp + e_zm + e_zp), summed
+  // in that order, bit for bit (a zero row sum assembled face by face): the
+  // FP64 residual then forms e0 from the neighbour coefficients it loads
+  int e0_sum;
   // workgroups of this system in the stencil, streaming and residual
   // launches (= number of partials each finalize kernel reads)
   int nblk_st, nblk_sm, nblk_ru;

@@ -670,6 +670,7 @@ int Engine::set_system_staged(int slot, const double* d, long long ly, long long
     s.has_m = has_m;
     s.dev.has_m = has_m ? 1 : 0;
     s.dev.neg = s.neg ? 1 : 0;
+    s.dev.e0_sum = (flags & 128u) ? 0 : 1;
     s.singular = nullspace_ == APHPOIS_NULL_ON || (nullspace_ == APHPOIS_NULL_AUTO && !has_m);
     APH_HIP(launch_convert_pcg(fp32_, d, 0, ly, lz, s.nx, s.ny, s.nz, s.neg ? -1.0 : 1.0,
                                const_cast<void*>(s.dev.kx), const_cast<void*>(s.dev.ky),

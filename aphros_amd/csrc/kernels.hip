@@ -596,6 +596,7 @@ __global__ __launch_bounds__(kBlock) void k_ru_resid(const SysDev* __restrict__ 
   // a zero start (x64 = 0): A x is exactly zero, so b - A x = b; the same
   // cells, order and partial sums without loading x or the coefficients
   const bool zs = sc->zero_start;
+  const bool e0s = s.e0_sum;  // e0 from the six neighbour coefficients
   double acc[3] = {0.0, 0.0, 0.0};
   double um = 0.0, uc = 0.0, ezm = 0.0;
   if (t.active && !zs) {
@@ -627,11 +628,14 @@ __global__ __launch_bounds__(kBlock) void k_ru_resid(const SysDev* __restrict__ 
       const double uym = ym_in ? sp[threadIdx.x - lx] : x[base + rowm + t.x0] - mp;
       const double uyp = yp_in ? sp[threadIdx.x + lx] : x[base + rowp + t.x0] - mp;
       const double ezc = ez[c];
-      double ax = e0[c] * uc;
-      ax += ex[base + row + xw] * uxm;
-      ax += ex[c] * uxp;
-      ax += ey[base + rowm + t.x0] * uym;
-      ax += ey[c] * uyp;
+      const double exw = ex[base + row + xw], exc = ex[c];
+      const double eym = ey[base + rowm + t.x0], eyc = ey[c];
+      const double e0c = e0s ? -(((((exw + exc) + eym) + eyc) + ezm) + ezc) : e0[c];
+      double ax = e0c * uc;
+      ax += exw * uxm;
+      ax += exc * uxp;
+      ax += eym * uym;
+      ax += eyc * uyp;
       ax += ezm * um;
       ax += ezc * un;
       const double r64 = bb[c] - ax;
@@ -692,6 +696,7 @@ __global__ __launch_bounds__(kBlock) void k_ru_resid2(const SysDev* __restrict__
   auto ld2 = [&](gcp<double> a, long long i) { return *(gcp<D2>)(a + i); };
   // a zero start (x64 = 0): b - A x = b, as in k_ru_resid
   const bool zs = sc->zero_start;
+  const bool e0s = s.e0_sum;  // e0 from the six neighbour coefficients
   double acc[3] = {0.0, 0.0, 0.0};
   D2 um = 0.0, uc = 0.0, ezm = 0.0;
   if (t.active && !zs) {
@@ -741,9 +746,17 @@ __global__ __launch_bounds__(kBlock) void k_ru_resid2(const SysDev* __restrict__
       } else {
         uyp = ld2(x, base + rowp + t.x0) - mp;
       }
-      const D2 e0c = ld2(e0, c), exc = ld2(ex, c), eyc = ld2(ey, c), ezc = ld2(ez, c);
+      const D2 exc = ld2(ex, c), eyc = ld2(ey, c), ezc = ld2(ez, c);
       const D2 eym = ld2(ey, base + rowm + t.x0), bc = ld2(bb, c);
       const double exw = ex[base + row + xw];
+      D2 e0c;
+      if (e0s) {
+#pragma unroll
+        for (int v = 0; v < 2; ++v)
+          e0c[v] = -((((((v == 0 ? exw : exc[0]) + exc[v]) + eym[v]) + eyc[v]) + ezm[v]) + ezc[v]);
+      } else {
+        e0c = ld2(e0, c);
+      }
       D2 r64;
 #pragma unroll
       for (int v = 0; v < 2; ++v) {
@@ -3023,7 +3036,8 @@ __global__ __launch_bounds__(kBlock) void k_fin_true_res_f7(const SysDev* __rest
 // ===========================================================================
 
 // flags: 1 non-finite, 2 asymmetric, 4 zero diagonal, 16 positive diagonal
-// seen, 32 negative diagonal seen, 64 non-zero row sum (mass term)
+// seen, 32 negative diagonal seen, 64 non-zero row sum (mass term), 128 e0
+// other than -(((((e1 + e2) + e3) + e4) + e5) + e6) bit for bit
 __global__ __launch_bounds__(kBlock) void k_check(const double* __restrict__ aos, long long start,
                                                   long long ly, long long lz, int nx, int ny,
                                                   int nz, unsigned* __restrict__ flags) {
@@ -3051,6 +3065,7 @@ __global__ __launch_bounds__(kBlock) void k_check(const double* __restrict__ aos
     if (e[0] < 0) f |= 32u;
     const double rs = e[0] + e[1] + e[2] + e[3] + e[4] + e[5] + e[6];
     if (fabs(rs) > 1e-12 * fabs(e[0])) f |= 64u;
+    if (e[0] != -(((((e[1] + e[2]) + e[3]) + e[4]) + e[5]) + e[6])) f |= 128u;
   }
   if (f) atomicOr(&sflags, f);
   __syncthreads();

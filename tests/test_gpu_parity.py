@@ -121,6 +121,38 @@ def test_pcg_zero_start_equals_explicit_zero_guess(c1, precision, precond):
     assert np.array_equal(xa, xb)
 
 
+@pytest.mark.parametrize("precision", [L.FP64, L.MIXED])
+@pytest.mark.parametrize("case", ["c1", "c1_mass", "c1_ulp"])
+def test_pcg_residual_diagonal_formed_or_loaded(c1, precision, case):
+    """The FP64 true residual of the reliable updates forms e0 from the six
+    neighbour coefficients when every e0 equals -(e1 + ... + e6), summed in
+    neighbour order, bit for bit (the assembled c1 system), and loads e0
+    otherwise: c1 with a mass term on the diagonal, and c1 with one diagonal
+    entry one ulp larger (still singular, no mass term).  Either way the
+    residual the solver reports is the oracle's recomputation of b - A x."""
+    e = c1
+    if case == "c1_mass":
+        e = _with_mass(c1)
+    elif case == "c1_ulp":
+        e = c1.copy()
+        e[3, 5, 7, 0] = np.nextafter(e[3, 5, 7, 0], np.inf)
+    d = e[..., 0]
+    formed = -(((((e[..., 1] + e[..., 2]) + e[..., 3]) + e[..., 4]) + e[..., 5]) + e[..., 6])
+    assert np.array_equal(d, formed) == (case == "c1")
+    _say(f"pcg residual diagonal {case} precision={precision}")
+    s = Poisson(e.shape[:3], method=L.PCG, precision=precision, resmode=L.RES_REL_TRUE,
+                tol=1e-8, maxiter=20000, precond=L.PRECOND_MG)
+    s.set_system(e)
+    x, info = s.solve(None)
+    s.close()
+    rel = O.true_relres(e, x)
+    _say(f"  iter={info['iter']} ru={info['refinements']} relres={rel:.6e} "
+         f"(gpu {info['true_relres']:.6e})")
+    assert info["converged"] == 1 and info["status"] == 0
+    assert rel <= 1e-8 * (1 + 1e-6)
+    assert abs(rel - info["true_relres"]) <= 1e-3 * rel
+
+
 @pytest.mark.parametrize("precond", [L.PRECOND_JACOBI, L.PRECOND_MG])
 def test_pcg_tlinear_negative_diagonal(tlin32, precond):
     """e0 < 0 convention (t.linear / SolvePoisson)."""
