@@ -38,9 +38,11 @@ struct Scalars {
   int ru, converged, nru, status;
   int resmode, singular, first, cur;  // cur: index of the current p/u buffer
   int method, pc;  // pc: 1 if a multigrid preconditioner supplies z
-  int xh_fold;     // PCG: the last reliable update folded alpha p into x64
-                   // (the next CG operator then leaves x^ alone)
-  int zero_start;  // PCG: the start's x64 and x^ are zero-filled (the start's
+  int xh_fold;     // PCG state of x^: 0 it holds the corrections since the
+                   // last reliable update; 1 stale (that update folded x^ and
+                   // alpha p into x64; the next CG operator leaves x^ alone);
+                   // 2 stale, the next operator writes x^ = alpha p_in
+  int zero_start;  // PCG: the start's x64 is zero-filled (the start's
                    // residual is b, with no accumulation pass); cleared by it
 };
 

@@ -781,8 +781,8 @@ int Engine::prepare(int slot, const double* x0, bool on_device, long long start,
     zero_x = true;
   }
   if (method_ == kPcg) {
+    // x^ is not filled: the start's Scalars mark it stale (xh_fold 1)
     const int es = fp32_ ? 4 : 8;
-    APH_HIP(launch_fill(s.dev.xh, n, es, st_));
     APH_HIP(launch_fill(s.dev.p[0], n, es, st_));
     APH_HIP(launch_fill(s.dev.p[1], n, es, st_));
   }
@@ -794,6 +794,7 @@ int Engine::prepare(int slot, const double* x0, bool on_device, long long start,
     APH_HIP(hipStreamSynchronize(st_));
   }
   Scalars sc{};
+  sc.xh_fold = 1;  // x^ holds no value yet
   sc.tol = tol_;
   sc.delta = delta_;
   sc.h3 = s.h3;

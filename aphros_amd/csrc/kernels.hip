@@ -260,7 +260,8 @@ __device__ __forceinline__ Tile tile_of(const Rec& rc, const SysDev& s, int lb) 
 // XH: x^ += alpha p_in of the previous iteration, on the tile's own cells
 // where p_in is read (planes za .. zb-1), instead of a pass of the update
 // kernel; skipped when that iteration's reliable update folded alpha p into
-// x64 (xh_fold).
+// x64, and x^ = alpha p_in (no read) in the iteration after, x^ being stale
+// since that update (xh_fold 1, 2).
 template <typename T, int VX, bool SC, bool XH>
 __global__ __launch_bounds__(kBlock) void k_pcg_spmv(const SysDev* __restrict__ sds,
                                                      const Rec* __restrict__ recs,
@@ -311,13 +312,19 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv(const SysDev* __restrict__ 
   auto pnew_st = [&](long long i) -> V {
     return __builtin_convertvector(ldd<T, VX>(r, i) + beta * ldd<T, VX>(pin, i), V);
   };
-  const double xa = XH && !sc->xh_fold ? sc->alpha : 0.0;
+  const int xs = XH ? sc->xh_fold : 1;  // 0 x^ += xa p_in, 1 none, 2 x^ = xa p_in
+  const double xa = sc->alpha;
   const gmp<T> __restrict__ xh = (gmp<T>)s.xh;
-  // pnew_st, and x^ += xa p_in on cells the tile owns
+  // pnew_st, and the x^ update on cells the tile owns
   auto pnew_own = [&](long long i) -> V {
     const vec_t<double, VX> po = ldd<T, VX>(pin, i);
     if constexpr (XH) {
-      if (xa != 0.0) std_<T, VX>(xh, i, ldd<T, VX>(xh, i) + xa * po);
+      using D1 = vec_t<double, VX>;
+      if (xs == 0) {
+        if (xa != 0.0) std_<T, VX>(xh, i, ldd<T, VX>(xh, i) + xa * po);
+      } else if (xs == 2) {
+        std_<T, VX>(xh, i, (D1)(0.0) + xa * po);  // the sum with x^ = 0, to the bit
+      }
     }
     return __builtin_convertvector(ldd<T, VX>(r, i) + beta * po, V);
   };
@@ -426,7 +433,7 @@ __global__ __launch_bounds__(kBlock) void k_fin_pcg_spmv(const SysDev* __restric
   reduce_partials<1, 0u>(acc, s.partials, s.nblk_st);
   if (threadIdx.x == 0) {
     const double pq = acc[0];
-    sc->xh_fold = 0;
+    sc->xh_fold = sc->xh_fold == 1 ? 2 : 0;
     if (!(pq != 0.0) || !isfinite(pq)) {
       sc->status = 6;
       sc->done = 1;
@@ -502,10 +509,11 @@ __global__ __launch_bounds__(kBlock) void k_fin_pcg_update(const SysDev* __restr
   }
 }
 
-// Reliable update, part 1: x64 = (x64 - mean_pending) + S * x^ ; x^ = 0 ;
-// mean of the new x64 (nullspace removal).  x^ lacks this iteration's alpha
-// p (the next iteration's CG operator would add it), so alpha p is added here
-// and the operator skips it (xh_fold).
+// Reliable update, part 1: x64 = (x64 - mean_pending) + S * x^ ; mean of
+// the new x64 (nullspace removal).  x^ lacks this iteration's alpha p (the
+// next iteration's CG operator would add it), so alpha p is added here and
+// the operator skips it; x^ counts as zero from here on (xh_fold 1: not read
+// again before the operator after next overwrites it).
 template <typename T, int VX>
 __global__ __launch_bounds__(kBlock) void k_ru_accum(const SysDev* __restrict__ sds,
                                                      const Rec* __restrict__ recs,
@@ -523,17 +531,17 @@ __global__ __launch_bounds__(kBlock) void k_ru_accum(const SysDev* __restrict__ 
   T* __restrict__ xh = (T*)s.xh;
   double* __restrict__ x = s.x64;
   const double fa = sc->alpha;  // 0 at a start
+  const bool stale = sc->xh_fold != 0;  // x^ counts as zero
   const T* __restrict__ p = (const T*)s.p[sc->cur];
   double acc[1] = {0.0};
   long long beg, end;
   stream_range<VX>(s.n, rc.nblk, lb, beg, end);
   for (long long i = beg + (long long)threadIdx.x * VX; i < end; i += (long long)kBlock * VX) {
     const D sv = S ? ldd<T, VX>(S, i) : (D)(1.0);
-    D h = ldd<T, VX>(xh, i);
+    D h = stale ? (D)(0.0) : ldd<T, VX>(xh, i);
     if (fa != 0.0) h += fa * ldd<T, VX>(p, i);
     const D xv = (*reinterpret_cast<const D*>(x + i) - mp) + sv * h;
     *reinterpret_cast<D*>(x + i) = xv;
-    std_<T, VX>(xh, i, (D)(0.0));
 #pragma unroll
     for (int v = 0; v < VX; ++v) acc[0] += xv[v];
   }

@@ -32,10 +32,9 @@ HBM_PEAK_GBS = 8000.0
 # kernel instance in profiles.
 KERNELS = {
     # p = z + beta p: read z, p_in, kx, ky, kz (the CG's FP32 operator); write
-    # p_out, q; and x^ += alpha p_in of the previous iteration (xh_vec: read and
-    # write x^) in the launches that carry it (not the first iteration's nor
-    # the one after a reliable update, which folds it into x64)
-    "spmv": {"slot": 0, "vec": 7.0, "xh_vec": 2.0, "coef": 0.0, "m": 1,
+    # p_out, q; and the x^ update with alpha p_in of the previous iteration
+    # (xh: x^ vectors per launch, averaged by xh_vectors)
+    "spmv": {"slot": 0, "vec": 7.0, "xh": True, "coef": 0.0, "m": 1,
              "name": "k_pcg_spmv<T, VX, false, true>"},
     # r -= alpha q: read r, q; write r
     "update": {"slot": 2, "vec": 3.0, "coef": 0.0, "m": 0, "name": "k_pcg_update<T, VX>"},
@@ -80,7 +79,7 @@ def cpu_model():
 
 # the Jacobi-scaled form of the operator: p = S r + beta p, read r, p_in, S,
 # kx, ky, kz; write p_out, q; and x^ as in the multigrid form
-SPMV_JACOBI = {"slot": 0, "vec": 8.0, "xh_vec": 2.0, "coef": 0.0, "m": 1,
+SPMV_JACOBI = {"slot": 0, "vec": 8.0, "xh": True, "coef": 0.0, "m": 1,
                "name": "k_pcg_spmv<T, VX, true, true>"}
 
 
@@ -97,11 +96,25 @@ def launched_name(name, elem, vx):
     return name.replace("T, VX", f"{'float' if elem == 4 else 'double'}, {vx}")
 
 
+def xh_vectors(solves):
+    """x^ vectors the CG operator moves per launch, averaged over the
+    launches of `solves` ((iterations, reliable updates including the start)
+    pairs): 0 in the launch after each reliable update but the last (which
+    folds alpha p into x64 itself), 1 (write x^ = alpha p_in) in the launch
+    after that, 2 (read and write x^ += alpha p_in) in the others."""
+    moved = launches = 0
+    for n_iter, n_ru in solves:
+        skip = min(n_iter, max(0, n_ru - 1))
+        fresh = min(n_iter - skip, skip)
+        moved += 2 * (n_iter - skip - fresh) + fresh
+        launches += n_iter
+    return moved / max(1, launches)
+
+
 def kernel_table(kt, ncell, elem, coef, has_m, per_iter_launches, jacobi=False, vx=1,
-                 xh_frac=1.0):
+                 xh_vecs=2.0):
     """Per-kernel average duration and achieved algorithmic bandwidth.
-    xh_frac: the fraction of the CG operator's launches that update x^
-    (iterations minus reliable updates, over iterations)."""
+    xh_vecs: x^ vectors per CG-operator launch (xh_vectors)."""
     out = {}
     table = dict(KERNELS)
     if jacobi:
@@ -118,7 +131,7 @@ def kernel_table(kt, ncell, elem, coef, has_m, per_iter_launches, jacobi=False, 
         ms, cnt = kt[k["slot"]]
         if cnt == 0 or ms <= 0:
             continue
-        per_cell = (k["vec"] + k.get("xh_vec", 0.0) * xh_frac + (k["m"] if has_m else 0)) * elem \
+        per_cell = (k["vec"] + (xh_vecs if k.get("xh") else 0.0) + (k["m"] if has_m else 0)) * elem \
             + k["coef"] * coef
         b = per_cell * ncell
         out[key] = {"kernel": launched_name(k["name"], elem, vx), "avg_ms": round(ms, 5),
@@ -126,8 +139,8 @@ def kernel_table(kt, ncell, elem, coef, has_m, per_iter_launches, jacobi=False, 
                     "alg_bytes_per_cell": round(per_cell, 4), "alg_bytes_per_launch": int(b),
                     "GBps": round(b / (ms * 1e-3) / 1e9, 1),
                     "ms_per_iter": round(ms * per_iter_launches.get(key, 1), 5)}
-        if "xh_vec" in k:
-            out[key]["xh_launch_fraction"] = round(xh_frac, 4)
+        if k.get("xh"):
+            out[key]["xh_vectors_per_launch"] = round(xh_vecs, 4)
     return out
 
 
@@ -180,7 +193,7 @@ def run_c2(args):
     if not (info0["converged"] and info["converged"]):
         raise RuntimeError("c2 solve did not converge")
     kernels = kernel_table(kt, ncell, 8, 8, False, {}, jacobi=True, vx=li["vx"],
-                           xh_frac=(info["iter"] - info["refinements"]) / max(1, info["iter"]))
+                           xh_vecs=xh_vectors([(info["iter"], info["refinements"])]))
     b_it = sum(k["alg_bytes_per_launch"] for k in kernels.values())
     ms_it = sum(k["avg_ms"] for k in kernels.values())
     log(f"[bench] c2: {info0['iter']} iterations, {plain:.3f}s; iteration kernels "
@@ -530,10 +543,9 @@ def main():
                     "mg_post_l0": 1, "mg_pre_l0": 1}
         elem = 4 if main_prec == L.MIXED else 8
         coef = 2 if precond == L.PRECOND_MG and main_prec == L.MIXED else elem
-        nit = sum(i["iter"] for i in infos)
-        xh_frac = (nit - sum(i["refinements"] for i in infos)) / max(1, nit)
         kernels = kernel_table(kt, ncell, elem, coef, has_m, per_iter,
-                               jacobi=precond != L.PRECOND_MG, vx=li["vx"], xh_frac=xh_frac)
+                               jacobi=precond != L.PRECOND_MG, vx=li["vx"],
+                               xh_vecs=xh_vectors([(i["iter"], i["refinements"]) for i in infos]))
         if kernels:
             # the dominant kernel: the most device time per iteration
             key = max(kernels, key=lambda k: kernels[k]["ms_per_iter"])
