@@ -55,6 +55,7 @@ struct System {
   bool done_reported = true;     // the last solve's result has been collected
   std::vector<void*> allocs;
   long long bytes = 0;
+  double bnorm2 = -1;        // ||b||^2 of a PCG system (< 0: b changed since)
   double* xref = nullptr;    // reference solution for info.max_diff (or null)
   double* xref_store = nullptr;  // its device storage (kept once allocated)
   void* partials = nullptr;  // reduction scratch, outside the budget

@@ -646,6 +646,7 @@ int Engine::flux_correct(int slot, const double* p, const double* rho, const dou
 
 int Engine::set_system_staged(int slot, const double* d, long long ly, long long lz) {
   System& s = *sys(slot);
+  s.bnorm2 = -1;
   APH_HIP(hipMemsetAsync(d_flags_, 0, sizeof(unsigned), st_));
   APH_HIP(launch_check(d, 0, ly, lz, s.nx, s.ny, s.nz, d_flags_, st_));
   unsigned flags = 0;
@@ -721,6 +722,7 @@ int Engine::set_rhs(int slot, const double* e7, bool on_device, long long start,
   if (!e7) return fail(APHPOIS_EINVAL, "e7 is NULL");
   System& s = *sp;
   const double* d = nullptr;
+  s.bnorm2 = -1;
   APH_TRY(upload_strided(e7, on_device, start, ly, lz, s.nx, s.ny, s.nz, 1, &d));
   if (method_ == kPcg) {
     APH_HIP(launch_convert_rhs(d, 0, ly, lz, s.nx, s.ny, s.nz, -1.0,
@@ -781,17 +783,23 @@ int Engine::prepare(int slot, const double* x0, bool on_device, long long start,
     zero_x = true;
   }
   if (method_ == kPcg) {
-    // x^ is not filled: the start's Scalars mark it stale (xh_fold 1)
+    // the first CG operator reads p[0] (times beta = 0) and writes p[1]; x^
+    // is not filled either: the start's Scalars mark it stale (xh_fold 1)
     const int es = fp32_ ? 4 : 8;
     APH_HIP(launch_fill(s.dev.p[0], n, es, st_));
-    APH_HIP(launch_fill(s.dev.p[1], n, es, st_));
   }
-  // bnorm2 (PCG): sum of b^2
+  // bnorm2 (PCG): sum of b^2, kept until b changes (set_system, set_rhs,
+  // perturb_rhs)
   double bnorm2 = 0;
   if (method_ == kPcg) {
-    APH_HIP(launch_sumsq(s.dev.b64, n, d_red_, d_red_counter_, d_red_ + 1024, st_));
-    APH_HIP(hipMemcpyAsync(&bnorm2, d_red_ + 1024, sizeof(double), hipMemcpyDeviceToHost, st_));
-    APH_HIP(hipStreamSynchronize(st_));
+    if (s.bnorm2 < 0) {
+      APH_HIP(launch_sumsq(s.dev.b64, n, d_red_, d_red_counter_, d_red_ + 1024, st_));
+      APH_HIP(hipMemcpyAsync(&bnorm2, d_red_ + 1024, sizeof(double), hipMemcpyDeviceToHost,
+                             st_));
+      APH_HIP(hipStreamSynchronize(st_));
+      s.bnorm2 = bnorm2;
+    }
+    bnorm2 = s.bnorm2;
   }
   Scalars sc{};
   sc.xh_fold = 1;  // x^ holds no value yet
@@ -1109,8 +1117,10 @@ int Engine::capture() {
   if (profile_) {
     const size_t ne = (size_t)k * nk + 1;
     while (pev_.size() < ne) {
+      // timing only (read after the stream is synchronized): no system-scope
+      // fence, whose L2 writeback would slow the kernel after each event
       hipEvent_t e;
-      APH_HIP(hipEventCreate(&e));
+      APH_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
       pev_.push_back(e);
     }
   }
@@ -1201,6 +1211,7 @@ int Engine::perturb_rhs(int slot, double amp, unsigned long long key) {
   System* sp = sys(slot);
   if (!sp) return fail(APHPOIS_EINVAL, "no such slot");
   if (method_ != kPcg || !sp->has_system) return fail(APHPOIS_ESTATE, "no PCG system");
+  sp->bnorm2 = -1;
   APH_HIP(launch_perturb(const_cast<double*>(sp->dev.b64), sp->n, key, amp, d_red_, st_));
   return APHPOIS_OK;
 }
@@ -1323,8 +1334,8 @@ int Engine::run() {
       const int nk = iter_kernels(method_, pc_ ? 1 : 0);
       const int nv = pc_ ? vcycle_launches(L_) + 1 : 0;
       while ((int)pev_mg_.size() < graph_k_ * nv) {
-        hipEvent_t e;
-        APH_HIP(hipEventCreate(&e));
+        hipEvent_t e;  // timing only, as pev_
+        APH_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
         pev_mg_.push_back(e);
       }
       mg_kinds_.resize(nv);

@@ -153,6 +153,33 @@ def test_pcg_residual_diagonal_formed_or_loaded(c1, precision, case):
     assert abs(rel - info["true_relres"]) <= 1e-3 * rel
 
 
+def test_rhs_norm_follows_set_rhs(c1):
+    """||b|| of the relative stop test is kept between solves and recomputed
+    after set_rhs.  With b scaled by 1024 (exact in binary floating point)
+    every iterate scales exactly: the same iterations and relative residual,
+    and 1024 times the solution, bit for bit; a stale ||b|| would stop the
+    solve 1024 times too late.  Back to the first b, the first solve again."""
+    s = Poisson(c1.shape[:3], method=L.PCG, precision=L.MIXED, resmode=L.RES_REL_TRUE,
+                tol=1e-8, maxiter=20000, precond=L.PRECOND_MG)
+    s.set_system(c1)
+    x1, i1 = s.solve(None)
+    x1b, i1b = s.solve(None)
+    e2 = c1.copy()
+    e2[..., 7] *= 1024.0
+    s.set_rhs(e2[..., 7])
+    x2, i2 = s.solve(None)
+    s.set_rhs(c1[..., 7])
+    x3, i3 = s.solve(None)
+    s.close()
+    _say(f"  iter {i1['iter']} {i2['iter']} {i3['iter']} relres {i1['true_relres']:.6e} "
+         f"{i2['true_relres']:.6e}")
+    assert i1["converged"] == 1 and O.true_relres(c1, x1) <= 1e-8 * (1 + 1e-6)
+    assert np.array_equal(x1, x1b) and np.array_equal(x1, x3)
+    for k in ("iter", "refinements", "true_relres"):
+        assert i1[k] == i2[k] == i3[k], k
+    assert np.array_equal(x2, 1024.0 * x1)
+
+
 @pytest.mark.parametrize("precond", [L.PRECOND_JACOBI, L.PRECOND_MG])
 def test_pcg_tlinear_negative_diagonal(tlin32, precond):
     """e0 < 0 convention (t.linear / SolvePoisson)."""
