@@ -217,9 +217,10 @@ int Engine::init(int device, int method, int precision, int resmode, double tol,
   APH_HIP(hipMemset(d_counter_, 0, sizeof(unsigned) * capacity_));
   APH_HIP(hipMemset(d_sc_, 0, sizeof(Scalars) * capacity_));
   APH_HIP(hipMalloc(&d_flags_, sizeof(unsigned)));
-  // reduction scratch of k_sumsq (1024 partials + result) and of the RHS
-  // perturbation (2 x 1024 partials + 2 results)
-  APH_HIP(hipMalloc(&d_red_, sizeof(double) * 2056));
+  // reduction scratch of k_sumsq (1024 partials + result), of the RHS
+  // perturbation (2 x 1024 partials + 2 results) and of max_diff (3 x 1024
+  // partials + 2 results)
+  APH_HIP(hipMalloc(&d_red_, sizeof(double) * 3080));
   APH_HIP(hipMalloc(&d_red_counter_, sizeof(unsigned)));
   APH_HIP(hipMemset(d_red_counter_, 0, sizeof(unsigned)));
   APH_HIP(hipEventCreate(&ev0_));
@@ -1200,8 +1201,8 @@ int Engine::diff_info(int slot) {
     x = s.dev.u[0];
   }
   double out[2] = {0, 0};
-  APH_HIP(launch_max_diff(x, s.xref, s.n, sub, s.singular ? 1 : 0, d_red_, d_red_ + 1024, st_));
-  APH_HIP(hipMemcpyAsync(out, d_red_ + 1024, sizeof(out), hipMemcpyDeviceToHost, st_));
+  APH_HIP(launch_max_diff(x, s.xref, s.n, sub, s.singular ? 1 : 0, d_red_, d_red_ + 3072, st_));
+  APH_HIP(hipMemcpyAsync(out, d_red_ + 3072, sizeof(out), hipMemcpyDeviceToHost, st_));
   APH_HIP(hipStreamSynchronize(st_));
   s.info.max_diff = out[1];
   return APHPOIS_OK;

@@ -130,8 +130,8 @@ hipError_t launch_fill(void* p, long long n, int elem_bytes, hipStream_t st);
 hipError_t launch_b2r3(int* b2r, int n0, int n1, int n2, hipStream_t st);
 // x -= sub in place
 hipError_t launch_shift(double* x, long long n, double sub, hipStream_t st);
-// d = (x - sub) - xref: out[0] = mean of d (if remove_mean), out[1] = max |d -
-// out[0]| (max |d| without the mean removal); partials: 1024 doubles
+// d = (x - sub) - xref: out[0] = mean of d (0 without remove_mean), out[1] =
+// max |d - out[0]|, in one pass over x and xref; partials: 3 x 1024 doubles
 hipError_t launch_max_diff(const double* x, const double* xref, long long n, double sub,
                            int remove_mean, double* partials, double* out, hipStream_t st);
 // The projection's pressure system (dense FP64 Expr AoS, x fastest) from

@@ -3274,36 +3274,41 @@ __global__ __launch_bounds__(kBlock) void k_shift(double* __restrict__ x, long l
 // Difference of a solution from a reference solution, d = (x - sub) - xref:
 // partials of sum d (pass 0), then of max |d - mean| with the mean read from
 // stats[0] (pass 1; stats NULL: mean 0).
+// max |d - mean d| over the cells, d = (x - sub) - xref (mean 0 unless
+// remove_mean), in one pass: per workgroup the sum, the max and the max of -d.
 __global__ __launch_bounds__(kBlock) void k_diff(const double* __restrict__ x,
                                                  const double* __restrict__ xref, long long n,
-                                                 double sub, int pass,
-                                                 const double* __restrict__ stats,
-                                                 double* __restrict__ partials) {
-  double acc[1] = {0.0};
-  const double mean = (pass == 1 && stats) ? stats[0] : 0.0;
+                                                 double sub, double* __restrict__ partials) {
+  double acc[3] = {0.0, -INFINITY, -INFINITY};
   for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
        i += (long long)gridDim.x * kBlock) {
     const double d = (x[i] - sub) - xref[i];
-    acc[0] = pass == 0 ? acc[0] + d : fmax(acc[0], fabs(d - mean));
+    acc[0] += d;
+    acc[1] = fmax(acc[1], d);
+    acc[2] = fmax(acc[2], -d);
   }
-  if (pass == 0) {
-    block_partial<1, 0u>(acc, partials, gridDim.x, blockIdx.x);
-  } else {
-    block_partial<1, 1u>(acc, partials, gridDim.x, blockIdx.x);
-  }
+  block_partial<3, 6u>(acc, partials, gridDim.x, blockIdx.x);
 }
 
-// out = (sum of the partials) * scale (pass 0) or their max (pass 1)
+// out[0] = mean d (0 unless remove_mean), out[1] = max(max d - mean, mean -
+// min d, 0): the largest |d - mean| bit for bit, since rounding d - mean is
+// monotonic in d and symmetric in sign.
 __global__ __launch_bounds__(kBlock) void k_fin_diff(const double* __restrict__ partials,
-                                                     int nblk, int pass, double scale,
+                                                     int nblk, int remove_mean, double scale,
                                                      double* __restrict__ out) {
-  double acc[1];
-  if (pass == 0) {
-    reduce_partials<1, 0u>(acc, partials, nblk);
-  } else {
-    reduce_partials<1, 1u>(acc, partials, nblk);
+  double sum[1];
+  reduce_partials<1, 0u>(sum, partials, nblk);
+  double mx[2] = {-INFINITY, -INFINITY};
+  for (int i = threadIdx.x; i < nblk; i += kBlock) {
+    mx[0] = fmax(mx[0], partials[nblk + i]);
+    mx[1] = fmax(mx[1], partials[2 * nblk + i]);
   }
-  if (threadIdx.x == 0) *out = pass == 0 ? acc[0] * scale : acc[0];
+  block_reduce<2, 3u>(mx);
+  if (threadIdx.x == 0) {
+    const double mean = remove_mean ? sum[0] * scale : 0.0;
+    out[0] = mean;
+    out[1] = fmax(0.0, fmax(mx[0] - mean, mean - (-mx[1])));
+  }
 }
 
 // ===========================================================================
@@ -3719,14 +3724,9 @@ hipError_t launch_shift(double* x, long long n, double sub, hipStream_t st) {
 hipError_t launch_max_diff(const double* x, const double* xref, long long n, double sub,
                            int remove_mean, double* partials, double* out, hipStream_t st) {
   int g = grid_for(n);
-  if (g > 1024) g = 1024;
-  if (remove_mean) {
-    APH_LAUNCH(k_diff, g, st, x, xref, n, sub, 0, (const double*)nullptr, partials);
-    APH_LAUNCH(k_fin_diff, 1, st, partials, g, 0, 1.0 / (double)n, out);
-  }
-  APH_LAUNCH(k_diff, g, st, x, xref, n, sub, 1, remove_mean ? (const double*)out : nullptr,
-             partials);
-  APH_LAUNCH(k_fin_diff, 1, st, partials, g, 1, 1.0, out + 1);
+  if (g > 1024) g = 1024;  // partials: 3 x 1024 doubles
+  APH_LAUNCH(k_diff, g, st, x, xref, n, sub, partials);
+  APH_LAUNCH(k_fin_diff, 1, st, partials, g, remove_mean, 1.0 / (double)n, out);
   return hipGetLastError();
 }
 

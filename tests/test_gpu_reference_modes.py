@@ -176,6 +176,35 @@ def test_max_diff_mixed_against_fp64_solution():
     assert info["max_diff"] <= 1e-5 * np.abs(x64).max()
 
 
+@pytest.mark.parametrize("case", ["low_spike", "high_spike", "shifted"])
+def test_max_diff_either_side_and_one_signed(case):
+    """max_diff = max|d - mean d| whichever side of the mean the extreme lies
+    on, and when every d has one sign (a reference offset by a constant):
+    against numpy on references made from the solution itself."""
+    E, ue, h = O.tlinear_system(16)
+    s = Poisson(E.shape[:3], method=L.CONJUGATE, resmode=L.RES_L2, tol=1e-5, miniter=0,
+                maxiter=1000, h3=h ** 3)
+    s.set_system(E)
+    x, _ = s.solve(None)
+    ref = x.copy()
+    rng = np.random.default_rng(5)
+    ref += 1e-6 * rng.standard_normal(ref.shape)
+    if case == "low_spike":
+        ref[3, 4, 5] += 1e-3      # d = x - ref: a large negative excursion
+    elif case == "high_spike":
+        ref[3, 4, 5] -= 1e-3
+    else:
+        ref -= 0.5                # every d positive, about 0.5
+    s.set_solution_ref(ref)
+    x2, info = s.solve(None)
+    s.close()
+    assert np.array_equal(x, x2)
+    d = x2 - ref
+    want = np.abs(d - d.mean()).max()
+    _say(f"  {case}: max_diff gpu {info['max_diff']:.9e} numpy {want:.9e}")
+    assert np.isclose(info["max_diff"], want, rtol=1e-9, atol=0)
+
+
 @pytest.mark.parametrize("method", [L.CONJUGATE, L.PCG])
 def test_system_file_round_trip_solve(tmp_path, method):
     """The (nz, ny, nx, 8) system file written and read back through the C
