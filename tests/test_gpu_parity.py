@@ -153,6 +153,27 @@ def test_pcg_residual_diagonal_formed_or_loaded(c1, precision, case):
     assert abs(rel - info["true_relres"]) <= 1e-3 * rel
 
 
+@pytest.mark.parametrize("precond", [L.PRECOND_JACOBI, L.PRECOND_MG])
+def test_pcg_frequent_reliable_updates(c1, precond):
+    """delta = 0.999: a reliable update whenever the recursive residual falls
+    below its maximum since the last one (a third or more of the iterations),
+    so updates often follow each other with no x^ write in between (x^ stale,
+    counted as zero, in the accumulation of the second).  The solve still
+    reaches the oracle's 1e-8 true residual and the oracle's solution."""
+    s = Poisson(c1.shape[:3], method=L.PCG, precision=L.MIXED, resmode=L.RES_REL_TRUE,
+                tol=1e-8, maxiter=20000, precond=precond, delta=0.999)
+    s.set_system(c1)
+    x, info = s.solve(None)
+    s.close()
+    rel = O.true_relres(c1, x)
+    _say(f"  precond={precond} iter={info['iter']} ru={info['refinements']} relres={rel:.3e}")
+    assert info["converged"] == 1
+    assert info["refinements"] >= info["iter"] // 3
+    assert rel <= 1e-8 * (1 + 1e-6)
+    xo, _, _ = O.pcg_jacobi(c1, tol=1e-11)
+    assert np.abs(x - xo).max() / np.abs(xo).max() < 1e-5
+
+
 def test_rhs_norm_follows_set_rhs(c1):
     """||b|| of the relative stop test is kept between solves and recomputed
     after set_rhs.  With b scaled by 1024 (exact in binary floating point)
