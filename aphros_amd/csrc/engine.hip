@@ -451,6 +451,13 @@ int Engine::add_system(int nx, int ny, int nz, double h3, int* slot) {
                      : (fp32_ && l >= 1 && L.n <= kTileMaxCells && L.nx % 4 == 0 ? 2 : 0);
       }
       for (int l = 1; l <= w && rc == APHPOIS_OK; ++l) {
+        // a fused or tile level's second cycle leaves z[1] (the first
+        // correction) alone until its last sweep, which adds it in place;
+        // plain levels pre-smooth into z[1] and keep a copy in zs
+        if (S.lev[l].post != 0) {
+          S.lev[l].zs = S.lev[l].z[1];
+          continue;
+        }
         void* zs = nullptr;
         A(&zs, S.lev[l].n * es);
         S.lev[l].zs = zs;

@@ -1016,7 +1016,8 @@ __device__ __forceinline__ Cell cell_of(const Rec& rc, int lb, int nx, int ny, i
 //           reads one coarse array
 
 //   MODE 4: u = z_in, no z_out: r = r - L u in place and u saved to zs (the
-//           start of the second coarse correction of the W level above)
+//           start of the second coarse correction of the W level above; no
+//           copy on fused and tile levels, whose zs is z[1] itself)
 //   MODE 3: u = z_in, no z_out: the residual r - L u restricted to level
 //           l+1 (sum over the 2x2x2 children); x pairs are summed in the
 //           thread (VX >= 2), plane pairs over the z march (even plane count
@@ -1093,7 +1094,8 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
   gcp<T> __restrict__ zsa = nullptr;  // MODE 0: the first correction added
   if constexpr (MODE == 4) {
     rw = (gmp<T>)Lv.r;
-    zsv = (gmp<T>)Lv.zs;
+    // no copy where zs is z_in itself (fused and tile levels)
+    if (Lv.zs != Lv.z[zin]) zsv = (gmp<T>)Lv.zs;
   }
   if constexpr (MODE == 0) {
     if (addzs) zsa = (gcp<T>)Lv.zs;
@@ -1219,7 +1221,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(const SysDev* __restrict__ 
 #pragma unroll
         for (int j = 0; j < NP; ++j) racc[j] += res[2 * j] + res[2 * j + 1];
       } else if constexpr (MODE == 4) {
-        stv<T, VX>(zsv, ic, uc);
+        if (zsv) stv<T, VX>(zsv, ic, uc);
         stv<T, VX>(rw, ic, rv - lap);
       } else {
         V out = uc + w * dc * (rv - lap);
@@ -1513,9 +1515,9 @@ __global__ __launch_bounds__(kPostBlock) void k_mg_post(const SysDev* __restrict
   const T* __restrict__ mm = (const T*)Lv.m;
   const T* __restrict__ r = (const T*)Lv.r;
   const T* __restrict__ zi = (const T*)Lv.z[0];
-  T* __restrict__ zo = (T*)Lv.z[1];
+  T* zo = (T*)Lv.z[1];  // zsa may be z[1] itself: each cell read, then written, by its thread
   const T* __restrict__ zc = (const T*)Lc.z[l + 1 == s.nlev - 1 ? 0 : 1];
-  const T* __restrict__ zsa = addzs ? (const T*)Lv.zs : nullptr;
+  const T* zsa = addzs ? (const T*)Lv.zs : nullptr;
   const int cnx = Lc.nx, cny = Lc.ny;
   const T wa = (T)kOmegaA, wb = (T)kOmegaB;
   const long long pl = (long long)nx * ny;
