@@ -1063,7 +1063,7 @@ int Engine::build_tables() {
   L_.mg_w = 0;
   for (int i : launch_slots_) L_.mg_w = std::max(L_.mg_w, slots_[i]->dev.mg_w);
   for (int l = 0; l < kMaxLevels; ++l) L_.vx_l[l] = vxl[l];
-  L_.coarse_sweeps = 40;
+  L_.coarse_sweeps = kCoarseSweeps;
   L_.coarse_lds = (int)coarse_max;
   for (int l = 0; l < kMaxLevels; ++l) L_.bottom_at[l] = bottom_at[l];
   L_.deep = deep;

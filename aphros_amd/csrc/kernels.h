@@ -5,6 +5,13 @@
 
 namespace aph {
 
+// Damped Jacobi sweeps (weight 0.9, from zero) on the coarsest multigrid
+// level.  Mixed solves to 1e-8, 40 / 20 / 12 sweeps: 512^3 and 256^3 the same
+// iterations and time; with 32 spheres at 96^3, 160^3, 224^3, 320^3, 384^3:
+// 48 / 45 / 45, 52 / 54 / 49, 36 / 34 / 36, 30 / 28 / 32, 27 / 27 / 27
+// iterations, 20 the fastest or within 1 % at every size but 160^3 (12).
+constexpr int kCoarseSweeps = 20;
+
 // Device tables of one engine launch configuration.
 struct LaunchSet {
   const SysDev* sds = nullptr;
@@ -26,7 +33,7 @@ struct LaunchSet {
   int deep = 0;  // largest bottom level over the systems (no launches below)
   int bottom_at[kMaxLevels] = {};  // some system's bottom kernel starts here
   int start_only = 0;  // cycle launches for newly prepared systems only
-  int coarse_sweeps = 40;
+  int coarse_sweeps = kCoarseSweeps;
   int coarse_lds = 0;  // dynamic LDS bytes of the bottom-levels kernel
   int vx_l[kMaxLevels] = {};  // cells per thread in x of the level-l sweeps
   const Rec* rec_lf[kMaxLevels] = {};

@@ -182,7 +182,7 @@ def pcg_jacobi(expr, tol=1e-8, maxiter=100000, remove_mean=True, x0=None):
     return x, it, rr.value
 
 
-MG_SWEEPS = 40  # damped Jacobi sweeps on the coarsest level
+MG_SWEEPS = 20  # damped Jacobi sweeps on the coarsest level (the engine's kCoarseSweeps)
 
 
 def mg_apply(expr, r, sweeps=MG_SWEEPS, bf16=False, w=0):

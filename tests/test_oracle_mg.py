@@ -109,7 +109,7 @@ def _dense_vcycle(e, sweeps, scale=0.5, shift=0, w=0):
                                              ((4, 8, 8), 0.05, 1), ((8, 8, 4), 0.05, -1)])
 def test_vcycle_matches_dense_definition(shape, mass, sign):
     e = _system(None, 7, mass=mass, sign=sign, shape=shape)
-    for sweeps in (1, 40):
+    for sweeps in (1, 20, 40):
         M, nlev = _dense_vcycle(e, sweeps)
         rng = np.random.default_rng(3)
         r = rng.standard_normal(shape)
@@ -124,7 +124,7 @@ def test_w_levels_match_dense_definition(w):
     """Two coarse corrections on the top w levels (8^3: 3 levels, so w = 2
     acts like w = 1: level 1's coarse level is the coarsest)."""
     e = _system(None, 7, shape=(8, 8, 8), mass=0.05)
-    M, nlev = _dense_vcycle(e, 40, w=w)
+    M, nlev = _dense_vcycle(e, O.MG_SWEEPS, w=w)
     r = np.random.default_rng(3).standard_normal((8, 8, 8))
     z, nl = O.mg_apply(e, r, w=w)
     ze = (M @ r.ravel()).reshape(8, 8, 8)
@@ -184,7 +184,7 @@ def test_vcycle_check_catches_planted_mistake(scale, shift):
     """A Galerkin coarse operator without the 0.5, or aggregates offset by one
     cell, gives a different M: the dense-definition check has teeth."""
     e = _system(None, 7, shape=(4, 4, 4))
-    M, _ = _dense_vcycle(e, 40, scale=scale, shift=shift)
+    M, _ = _dense_vcycle(e, O.MG_SWEEPS, scale=scale, shift=shift)
     r = np.random.default_rng(3).standard_normal((4, 4, 4))
     z, _ = O.mg_apply(e, r)
     ze = (M @ r.ravel()).reshape(4, 4, 4)
