@@ -10,7 +10,7 @@ from . import _lib as L
 class Batch:
     def __init__(self, precision=L.MIXED, resmode=L.RES_REL_TRUE, tol=1e-8, maxiter=200000,
                  delta=0.0, check_every=0, device=0, mem_budget=0, max_systems=1024,
-                 precond=L.PRECOND_JACOBI, mg_w=L.MG_W_DEFAULT):
+                 precond=L.PRECOND_JACOBI, mg_w=L.MG_W_BATCH):
         d = L.BatchDesc()
         d.precond = precond
         d.mg_w = mg_w

@@ -314,8 +314,8 @@ typedef struct aphpois_batch_desc {
   int precond;          /* APHPOIS_PRECOND_* */
   int mg_w;             /* as in aphpois_desc (AUTO: by each system's size);
                            a batch shares its launches among its systems, so
-                           its W levels cost little latency per system and
-                           APHPOIS_MG_W_DEFAULT suits batches of small ones */
+                           its W levels cost little latency per system: 2 suit
+                           batches of systems up to ~160^3 */
 } aphpois_batch_desc;
 
 int aphpois_batch_create(const aphpois_batch_desc* d, aphpois_batch** out);

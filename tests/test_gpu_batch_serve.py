@@ -96,7 +96,7 @@ def test_batch_fused_and_plain_levels(precision):
     b.solve()
     for E, sl in zip(systems, slots):
         xb, ib = b.get(sl)
-        xs, is_ = _single(E, precision, precond=L.PRECOND_MG, mg_w=L.MG_W_DEFAULT)
+        xs, is_ = _single(E, precision, precond=L.PRECOND_MG, mg_w=L.MG_W_BATCH)
         print(f"n={E.shape[0]} iter={ib['iter']} relres={ib['true_relres']:.2e}", flush=True)
         assert ib["converged"] == 1 and ib["iter"] == is_["iter"]
         assert np.array_equal(xb, xs)
